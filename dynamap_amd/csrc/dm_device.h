@@ -1,0 +1,194 @@
+// Device-side math shared by the kernels.  The evaluation order of every
+// expression follows the reference's float code (cited per helper); the
+// library is compiled with -ffp-contract=off and correctly rounded division
+// and square root, so FP32 results are reproducible on the CPU.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "dm_internal.h"
+
+namespace dm {
+
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// include/math/cuda_math.h:409-413
+__device__ __forceinline__ float norm(f3 v) { return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
+// v / sqrt(dot): correctly rounded, no reciprocal square root
+__device__ __forceinline__ f3 normalize(f3 v) {
+  float n = sqrtf(dot(v, v));
+  return mk(v.x / n, v.y / n, v.z / n);
+}
+
+// row-major 4x4 (float4x4 layout of include/math/utils_matrix.h:829-834)
+struct M4 { float m[16]; };
+__device__ __forceinline__ M4 load_m4(const float *p) {
+  M4 r;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) r.m[k] = p[k];
+  return r;
+}
+// include/math/utils_matrix.h:923-930: ((m11 x + m12 y) + m13 z) + m14*1
+__device__ __forceinline__ f3 xform(const M4 &T, f3 v) {
+  return mk(T.m[0] * v.x + T.m[1] * v.y + T.m[2] * v.z + T.m[3],
+            T.m[4] * v.x + T.m[5] * v.y + T.m[6] * v.z + T.m[7],
+            T.m[8] * v.x + T.m[9] * v.y + T.m[10] * v.z + T.m[11]);
+}
+__device__ __forceinline__ f3 rot(const M4 &T, f3 v) {
+  return mk(T.m[0] * v.x + T.m[1] * v.y + T.m[2] * v.z, T.m[4] * v.x + T.m[5] * v.y + T.m[6] * v.z,
+            T.m[8] * v.x + T.m[9] * v.y + T.m[10] * v.z);
+}
+
+// float -> int32 with CUDA conversion semantics (truncate, saturate, NaN -> 0)
+__device__ __forceinline__ int to_i32(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0f) return 0x7fffffff;
+  if (x <= -2147483648.0f) return (int)0x80000000;
+  return (int)x;
+}
+
+__device__ __forceinline__ int to_i32d(double x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0) return 0x7fffffff;
+  if (x <= -2147483648.0) return (int)0x80000000;
+  return (int)x;
+}
+
+struct Intr { int rows, cols; float fx, fy, cx, cy; };
+
+// include/utils/utils.h:104-112 getPoint3d (scale 1)
+__device__ __forceinline__ f3 get_point3d(int i, float depth, const Intr &s) {
+  int v = i / s.cols;
+  int u = i - s.cols * v;
+  f3 p;
+  p.z = depth;
+  p.x = (((float)u - s.cx) * p.z) / s.fx;
+  p.y = (((float)v - s.cy) * p.z) / s.fy;
+  return p;
+}
+// include/utils/utils.h:114-119 Project
+__device__ __forceinline__ void project(f3 p, const Intr &s, int &u, int &v) {
+  u = to_i32(roundf((s.fx * p.x) / p.z + s.cx));
+  v = to_i32(roundf((s.fy * p.y) / p.z + s.cy));
+}
+
+// include/dataStructure/hashTable/hashTable.cu:152-163 (bucket index, i.e.
+// the reference's value divided by bucket_size); int32 products wrap.
+__device__ __forceinline__ int hash_bucket(int x, int y, int z, int num_buckets) {
+  uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349669u) ^ ((uint32_t)z * 83492791u);
+  int r = (int)h % num_buckets;
+  if (r < 0) r += num_buckets;
+  return r;
+}
+__device__ __forceinline__ bool in_key_range(int x, int y, int z) {
+  return x >= -kKeyRange && x < kKeyRange && y >= -kKeyRange && y < kKeyRange && z >= -kKeyRange &&
+         z < kKeyRange;
+}
+// occupied flag in bit 63, 21 bits per coordinate
+__device__ __forceinline__ uint64_t pack_key(int x, int y, int z) {
+  const uint64_t M = (1u << 21) - 1u;
+  return (1ull << 63) | (((uint64_t)(uint32_t)x & M) << 42) | (((uint64_t)(uint32_t)y & M) << 21) |
+         ((uint64_t)(uint32_t)z & M);
+}
+__device__ __forceinline__ int sext21(uint64_t v) { return (int)((int32_t)((uint32_t)(v & 0x1fffff) << 11) >> 11); }
+__device__ __forceinline__ void unpack_key(uint64_t k, int &x, int &y, int &z) {
+  x = sext21(k >> 42); y = sext21(k >> 21); z = sext21(k);
+}
+
+// bucket record accessors
+__device__ __forceinline__ const uint8_t *bucket_ptr(const DmView &v, int b) {
+  return v.buckets + (size_t)b * (size_t)v.bstride;
+}
+__device__ __forceinline__ int bucket_count(const DmView &v, const uint8_t *bk) {
+  return *(const int32_t *)(bk + 12 * v.bucket_size);
+}
+
+// FindHashEntry (hashTable.cu:136-150): pool index or -1.  Only the occupied
+// prefix of the bucket is scanned.
+__device__ __forceinline__ int find_block(const DmView &v, int x, int y, int z) {
+  if (!in_key_range(x, y, z)) return -1;
+  const uint64_t key = pack_key(x, y, z);
+  const uint8_t *bk = bucket_ptr(v, hash_bucket(x, y, z, v.num_buckets));
+  const int cnt = bucket_count(v, bk);
+  const uint64_t *keys = (const uint64_t *)bk;
+  for (int s = 0; s < cnt; ++s)
+    if (keys[s] == key) return ((const int32_t *)(bk + 8 * v.bucket_size))[s];
+  return -1;
+}
+
+// world -> global voxel (tsdf.cu:189-193): trunc(x/vs + sign(x)*0.5)
+__device__ __forceinline__ float signf_(float x) { return (float)((x > 0) - (x < 0)); }
+__device__ __forceinline__ int world_to_voxel(float x, float vs) { return to_i32(x / vs + signf_(x) * 0.5f); }
+__device__ __forceinline__ int floordiv8(int g) { return g < 0 ? (g - 8 + 1) / 8 : g / 8; }
+__device__ __forceinline__ int mod8(int g) { int m = g % 8; return m < 0 ? m + 8 : m; }
+
+// voxel word: low 32 bits sdf, then r, g, b, weight bytes
+__device__ __forceinline__ float vox_sdf(uint64_t w) { return __uint_as_float((uint32_t)w); }
+__device__ __forceinline__ uint32_t vox_rgbw(uint64_t w) { return (uint32_t)(w >> 32); }
+
+// Small per-thread cache of the last block looked up.
+struct BlockCache {
+  int bx, by, bz, ptr;
+  __device__ void init() { bx = 0x7fffffff; by = 0; bz = 0; ptr = -1; }
+};
+
+// GetVoxel (tsdf.cu:227-239); returns the voxel word, 0 when unallocated.
+__device__ __forceinline__ uint64_t get_voxel(const DmView &v, f3 p, BlockCache &c) {
+  const float vs = v.voxel_size;
+  int gx = world_to_voxel(p.x, vs), gy = world_to_voxel(p.y, vs), gz = world_to_voxel(p.z, vs);
+  int bx = floordiv8(gx), by = floordiv8(gy), bz = floordiv8(gz);
+  if (bx != c.bx || by != c.by || bz != c.bz) {
+    c.bx = bx; c.by = by; c.bz = bz;
+    c.ptr = find_block(v, bx, by, bz);
+  }
+  if (c.ptr < 0) return 0ull;
+  return v.vox[(size_t)c.ptr * kBlockVox + mod8(gx) * 64 + mod8(gy) * 8 + mod8(gz)];
+}
+
+// GetInterpolatedVoxel (tsdf.cu:47-175).  Returns false when the centre
+// voxel is unobserved.  A corner of weight 0 contributes the centre voxel.
+__device__ __forceinline__ bool interp_voxel(const DmView &v, f3 p, BlockCache &c, float &sdf_out,
+                                             float *col) {
+  uint64_t w0 = get_voxel(v, p, c);
+  if ((w0 >> 56) == 0) return false;
+  const float vs = v.voxel_size;
+  f3 dual = p - mk(vs / 2.0f, vs / 2.0f, vs / 2.0f);
+  f3 vp = mk(p.x / vs, p.y / vs, p.z / vs);
+  f3 wt = mk(vp.x - floorf(vp.x), vp.y - floorf(vp.y), vp.z - floorf(vp.z));
+  float dist = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ox = (0xD2 >> k) & 1, oy = (0xB4 >> k) & 1, oz = (0xE8 >> k) & 1;
+    f3 q = dual + mk(ox ? vs : 0.0f, oy ? vs : 0.0f, oz ? vs : 0.0f);
+    uint64_t w = get_voxel(v, q, c);
+    const float a = ox ? wt.x : 1.0f - wt.x;
+    const float b = oy ? wt.y : 1.0f - wt.y;
+    const float g = oz ? wt.z : 1.0f - wt.z;
+    const float coef = a * b * g;
+    const uint64_t s = (w >> 56) == 0 ? w0 : w;
+    dist += coef * vox_sdf(s);
+    if (col) {
+      const uint32_t rgbw = vox_rgbw(s);
+      c0 += coef * (float)(rgbw & 0xff);
+      c1 += coef * (float)((rgbw >> 8) & 0xff);
+      c2 += coef * (float)((rgbw >> 16) & 0xff);
+    }
+  }
+  sdf_out = dist;
+  if (col) { col[0] = c0; col[1] = c1; col[2] = c2; }
+  return true;
+}
+
+__device__ __forceinline__ uint32_t to_u8(float x) {
+  if (!(x > 0.f)) return 0u;
+  if (x >= 255.f) return 255u;
+  return (uint32_t)x;
+}
+
+}  // namespace dm

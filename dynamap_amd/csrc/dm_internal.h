@@ -1,0 +1,109 @@
+// Internal definitions shared by the HIP translation units of libdynamap_hip.
+//
+// Every kernel works on a table of DmView records (one per reconstruction
+// session) that lives in device memory, plus a list of the session slots that
+// take part in a launch (blockIdx.y indexes that list).  A single-session call
+// is the same launch with a one-entry list, so the per-frame path and the
+// batched serving path run the same kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dm {
+
+constexpr int kBlockEdge = 8;              // voxels per block edge
+constexpr int kBlockVox = 512;             // voxels per block
+constexpr int kKeyRange = 1 << 20;         // |block coordinate| limit of the packed key
+constexpr int kLevels = 3;                 // ICP pyramid levels
+constexpr int kIcpVals = 32;               // per-workgroup ICP partial record (30 used)
+
+// per-volume counters, int32 each
+enum Ctr : int {
+  C_NUM_ALLOC = 0,   // allocated blocks (pool indices 0..n-1 are in use)
+  C_N_NEW = 1,       // new keys collected this frame
+  C_N_INS = 2,       // of which inserted
+  C_CURSOR = 3,      // pool cursor within the frame
+  C_DROP_BUCKET = 4, // keys dropped because their bucket was full (cumulative)
+  C_DROP_POOL = 5,   // keys dropped because the pool was exhausted (cumulative)
+  C_DROP_RANGE = 6,  // block visits outside the packed-key range (cumulative)
+  C_DSET_OVF = 7,    // new keys lost to a full per-frame key set (cumulative)
+  C_HAS_MODEL = 8,   // a raycast model exists (tracking can run)
+  C_ICP_FAIL = 9,    // frames whose increment was reset by the guard (cumulative)
+  C_COUNT = 16
+};
+
+// pose block (double) offsets
+enum PoseD : int {
+  PD_T = 0,       // camera-to-world of the current frame (row-major 4x4)
+  PD_TPREV = 16,  // camera-to-world of the model (last integrated) frame
+  PD_TINC = 32,   // increment: current camera -> model camera
+  PD_ACC = 48,    // summed twist of the frame (6)
+  PD_TINIT = 64,  // pose used when no model exists yet
+  PD_COUNT = 80
+};
+// pose block (float) offsets
+enum PoseF : int { PF_T = 0, PF_TINV = 16, PF_TINC = 32, PF_TPREV = 48, PF_COUNT = 64 };
+
+struct DmView {
+  // sensor (level 0)
+  int rows, cols;
+  float fx, fy, cx, cy;
+  // volume parameters
+  float voxel_size, trunc, max_depth, min_depth;
+  int num_buckets, bucket_size, bstride, num_blocks, max_weight;
+  int dset_mask, newcap;
+  // volume arrays
+  uint8_t *buckets;     // num_buckets * bstride: u64 key[S] | i32 ptr[S] | i32 count
+  uint64_t *vox;        // num_blocks * 512 voxel words (sdf bits | r | g<<8 | b<<16 | w<<24 <<32)
+  int4 *block_key;      // num_blocks block coordinates
+  uint64_t *dset;       // per-frame new-key set, dset_mask + 1 slots, 0 = empty
+  uint64_t *newkeys;    // newcap
+  int32_t *newdslot;    // newcap
+  int32_t *newnext;     // newcap
+  int32_t *newrank;     // newcap (rank, or -1 when dropped)
+  int32_t *newlen;      // newcap (bucket list length, written by rank 0)
+  int32_t *bucket_head; // num_buckets, -1 = empty
+  int32_t *ctr;         // C_COUNT
+  // current frame input (device)
+  const float *depth;
+  const uint8_t *rgb;   // interleaved r,g,b (may be null)
+  // maps: vertex+intensity and normal per level, source (live frame) and
+  // target (model raycast); level l is (rows >> l) x (cols >> l)
+  float4 *svert[kLevels], *snorm[kLevels];
+  float4 *tvert[kLevels], *tnorm[kLevels];
+  // pose state
+  double *posed;        // PD_COUNT
+  float *posef;         // PF_COUNT
+  float *partials;      // ICP partial sums: max_wg * kIcpVals
+  int max_wg;
+  // optional raycast outputs (level 0), may be null
+  float *ray_depth;
+  uint8_t *ray_rgb;
+  // tracking parameters
+  int iters[kLevels];
+  float icp_dmin, icp_dmax, dist_max, w_photo, lambda, max_inc;
+  int pad_[2];
+};
+
+// ---- host-side launch helpers (defined in the .hip files) -----------------
+struct Launch {
+  const DmView *views;  // device table
+  const int *active;    // device list of session slots
+  int nact;             // entries in the list
+  int max_pixels;       // largest rows*cols among active sessions
+  int max_tiles;        // largest number of 16x16 pixel tiles among active sessions
+  hipStream_t stream;
+};
+
+void launch_alloc(const Launch &L);
+void launch_integrate(const Launch &L);
+void launch_raycast(const Launch &L);
+void launch_source_maps(const Launch &L);
+void launch_icp_reduce(const Launch &L, int level, int it);
+void launch_icp_solve(const Launch &L, int level, int it);
+void launch_pose_begin(const Launch &L);
+void launch_pose_log(const Launch &L, const int *dest, double *log);
+
+int icp_wg_count(int pixels);   // workgroups used by the level-0 ICP reduction
+
+}  // namespace dm

@@ -1,0 +1,413 @@
+// Voxel-block allocation, TSDF/colour integration and raycasting.
+//
+// Reference: include/geometry/tsdf/tsdf.cu (AllocateFromDepthKernel :345-453,
+// IntegrateScanKernel :267-344, GenerateDepthKernel/GenerateRgbKernel
+// :470-526), include/dataStructure/hashTable/hashTable.cu (:80-163).
+//
+// HBM layout per session:
+//   buckets  num_buckets x 128 B (for bucket_size <= 10): u64 key[S] (bit 63 =
+//            occupied, 21 bits per coordinate), i32 pool index[S], i32 count.
+//            One cache line per probe; the occupied slots are a prefix.
+//   vox      num_blocks x 512 x 8 B voxel words, x-major local index
+//            (x*64 + y*8 + z, include/geometry/geom_Types.h:116-119).
+//   block_key pool index -> block coordinate.
+#include "dm_device.h"
+
+namespace dm {
+
+__device__ __forceinline__ Intr intr_of(const DmView &v) {
+  return Intr{v.rows, v.cols, v.fx, v.fy, v.cx, v.cy};
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33; x *= 0xff51afd7ed558ccdULL; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ULL; x ^= x >> 33;
+  return x;
+}
+
+// Record a block that is missing from the table in the per-frame key set.
+__device__ __forceinline__ void note_new_key(const DmView &v, uint64_t key) {
+  uint32_t h = (uint32_t)mix64(key) & (uint32_t)v.dset_mask;
+  for (int probe = 0; probe <= v.dset_mask; ++probe) {
+    uint64_t prev = atomicCAS((unsigned long long *)&v.dset[h], 0ull, (unsigned long long)key);
+    if (prev == 0ull) {
+      int idx = atomicAdd(&v.ctr[C_N_NEW], 1);
+      if (idx < v.newcap) {
+        v.newkeys[idx] = key;
+        v.newdslot[idx] = (int32_t)h;
+      } else {
+        // no room in the frame's key list: release the slot so that a later
+        // frame can insert the key
+        atomicExch((unsigned long long *)&v.dset[h], 0ull);
+        atomicAdd(&v.ctr[C_DSET_OVF], 1);
+      }
+      return;
+    }
+    if (prev == key) return;
+    h = (h + 1u) & (uint32_t)v.dset_mask;
+  }
+  atomicAdd(&v.ctr[C_DSET_OVF], 1);
+}
+
+__device__ __forceinline__ void visit_block(const DmView &v, int x, int y, int z) {
+  if (!in_key_range(x, y, z)) { atomicAdd(&v.ctr[C_DROP_RANGE], 1); return; }
+  if (find_block(v, x, y, z) >= 0) return;
+  note_new_key(v, pack_key(x, y, z));
+}
+
+// AllocateFromDepthKernel (tsdf.cu:357-452), one thread per pixel, 16x16
+// pixel tiles so that neighbouring rays (which share most blocks) run in the
+// same workgroup.  K: ray from the camera centre to surface + trunc, the
+// (b + step) * bs boundary for both directions and the extra "neg_ray" block.
+// D: the traversal is bounded by the L1 distance to the end block.
+__global__ __launch_bounds__(256) void k_alloc_rays(const DmView *__restrict__ views,
+                                                     const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  const int tiles_x = (v.cols + 15) >> 4;
+  const int tile = blockIdx.x;
+  if (tile >= tiles_x * ((v.rows + 15) >> 4)) return;
+  const int u = (tile % tiles_x) * 16 + (threadIdx.x & 15);
+  const int r = (tile / tiles_x) * 16 + (threadIdx.x >> 4);
+  if (u >= v.cols || r >= v.rows) return;
+  const int i = r * v.cols + u;
+  const float d = v.depth[i];
+  if (d < v.min_depth || d > v.max_depth) return;
+  const Intr s = intr_of(v);
+  const M4 T = load_m4(v.posef + PF_T);
+  const float block_m = 8.0f * v.voxel_size;
+  const f3 start = mk(T.m[3], T.m[7], T.m[11]);
+  const f3 point = xform(T, get_point3d(i, d, s));
+  if (point.x == 0 && point.y == 0 && point.z == 0) return;
+  const f3 dir = normalize(point - start);
+  const float sd = norm(start - point);
+  const f3 end = start + dir * (sd + v.trunc);
+  int bx = to_i32(floorf(start.x / block_m)), by = to_i32(floorf(start.y / block_m)),
+      bz = to_i32(floorf(start.z / block_m));
+  const int ex = to_i32(floorf(end.x / block_m)), ey = to_i32(floorf(end.y / block_m)),
+            ez = to_i32(floorf(end.z / block_m));
+  const int stx = (dir.x > 0) - (dir.x < 0), sty = (dir.y > 0) - (dir.y < 0), stz = (dir.z > 0) - (dir.z < 0);
+  const float FMAX = 3.40282347e+38f;
+  const float dtx = dir.x != 0 ? fabsf(block_m / dir.x) : FMAX;
+  const float dty = dir.y != 0 ? fabsf(block_m / dir.y) : FMAX;
+  const float dtz = dir.z != 0 ? fabsf(block_m / dir.z) : FMAX;
+  const float bdx = ((float)bx + (float)stx) * block_m, bdy = ((float)by + (float)sty) * block_m,
+              bdz = ((float)bz + (float)stz) * block_m;
+  float mtx = dir.x != 0 ? (bdx - start.x) / dir.x : FMAX;
+  float mty = dir.y != 0 ? (bdy - start.y) / dir.y : FMAX;
+  float mtz = dir.z != 0 ? (bdz - start.z) / dir.z : FMAX;
+  int dx = 0, dy = 0, dz = 0;
+  bool neg = false;
+  if (bx != ex && dir.x < 0) { dx--; neg = true; }
+  if (by != ey && dir.y < 0) { dy--; neg = true; }
+  if (bz != ez && dir.z < 0) { dz--; neg = true; }
+  visit_block(v, bx, by, bz);
+  if (neg) {
+    bx += dx; by += dy; bz += dz;
+    visit_block(v, bx, by, bz);
+  }
+  const long long bound = llabs((long long)ex - bx) + llabs((long long)ey - by) + llabs((long long)ez - bz);
+  for (long long it = 0; it < bound && (bx != ex || by != ey || bz != ez); ++it) {
+    if (mtx < mty) {
+      if (mtx < mtz) { bx += stx; mtx += dtx; } else { bz += stz; mtz += dtz; }
+    } else {
+      if (mty < mtz) { by += sty; mty += dty; } else { bz += stz; mtz += dtz; }
+    }
+    visit_block(v, bx, by, bz);
+  }
+}
+
+__device__ __forceinline__ int key_bucket(const DmView &v, uint64_t key) {
+  int x, y, z;
+  unpack_key(key, x, y, z);
+  return hash_bucket(x, y, z, v.num_buckets);
+}
+
+// Chain the new keys of each bucket into a list (order irrelevant).
+__global__ __launch_bounds__(256) void k_alloc_link(const DmView *__restrict__ views, const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  int n = min(v.ctr[C_N_NEW], v.newcap);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    int b = key_bucket(v, v.newkeys[i]);
+    v.newnext[i] = atomicExch(&v.bucket_head[b], i);
+  }
+}
+
+// Canonical rank of each new key within its bucket (ascending packed key);
+// the first `free` keys of a bucket win (K: no overflow chain, the rest drop).
+__global__ __launch_bounds__(256) void k_alloc_rank(const DmView *__restrict__ views, const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  int n = min(v.ctr[C_N_NEW], v.newcap);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const uint64_t key = v.newkeys[i];
+    const int b = key_bucket(v, key);
+    int rank = 0, len = 0;
+    for (int j = v.bucket_head[b]; j >= 0; j = v.newnext[j]) {
+      rank += v.newkeys[j] < key;
+      ++len;
+    }
+    const int freeb = v.bucket_size - bucket_count(v, bucket_ptr(v, b));
+    if (rank < freeb) {
+      v.newrank[i] = rank;
+      atomicAdd(&v.ctr[C_N_INS], 1);
+    } else {
+      v.newrank[i] = -1;
+    }
+    v.newlen[i] = len;
+  }
+}
+
+// Insert the winners.  D: if the pool cannot take every insertion of the
+// frame, nothing is inserted and all new keys count as dropped_pool.
+__global__ __launch_bounds__(256) void k_alloc_commit(const DmView *__restrict__ views, const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  const int n = min(v.ctr[C_N_NEW], v.newcap);
+  const int nalloc = v.ctr[C_NUM_ALLOC];
+  const int nins = v.ctr[C_N_INS];
+  if (nalloc + nins > v.num_blocks) return;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int rank = v.newrank[i];
+    if (rank < 0) continue;
+    const uint64_t key = v.newkeys[i];
+    int x, y, z;
+    unpack_key(key, x, y, z);
+    const int b = hash_bucket(x, y, z, v.num_buckets);
+    uint8_t *bk = v.buckets + (size_t)b * (size_t)v.bstride;
+    const int slot = *(const int32_t *)(bk + 12 * v.bucket_size) + rank;
+    const int ptr = nalloc + atomicAdd(&v.ctr[C_CURSOR], 1);
+    ((uint64_t *)bk)[slot] = key;
+    ((int32_t *)(bk + 8 * v.bucket_size))[slot] = ptr;
+    v.block_key[ptr] = make_int4(x, y, z, 1);
+  }
+}
+
+// Publish bucket counts, clear the per-frame structures.
+__global__ __launch_bounds__(256) void k_alloc_finish(const DmView *__restrict__ views, const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  const int n = min(v.ctr[C_N_NEW], v.newcap);
+  const bool pool_ok = v.ctr[C_NUM_ALLOC] + v.ctr[C_N_INS] <= v.num_blocks;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const uint64_t key = v.newkeys[i];
+    const int b = key_bucket(v, key);
+    v.dset[v.newdslot[i]] = 0ull;
+    v.bucket_head[b] = -1;
+    if (pool_ok && v.newrank[i] == 0) {
+      int32_t *cnt = (int32_t *)(v.buckets + (size_t)b * (size_t)v.bstride + 12 * v.bucket_size);
+      const int freeb = v.bucket_size - *cnt;
+      *cnt += min(v.newlen[i], freeb);
+    }
+  }
+}
+
+// Counter epilogue (one thread per session): runs after k_alloc_finish.
+__global__ void k_alloc_close(const DmView *__restrict__ views, const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.x];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  if (threadIdx.x != 0) return;
+  const int n_all = v.ctr[C_N_NEW];
+  const int n = min(n_all, v.newcap);
+  const int nins = v.ctr[C_N_INS];
+  if (v.ctr[C_NUM_ALLOC] + nins <= v.num_blocks) {
+    v.ctr[C_NUM_ALLOC] += nins;
+    v.ctr[C_DROP_BUCKET] += n - nins;
+  } else {
+    v.ctr[C_DROP_POOL] += n;
+  }
+  v.ctr[C_N_NEW] = 0;
+  v.ctr[C_N_INS] = 0;
+  v.ctr[C_CURSOR] = 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// Integration (IntegrateScanKernel, tsdf.cu:267-344; merge geom_Types.h:79-102)
+// One workgroup per voxel block (grid-stride over the allocated blocks), two
+// voxels per thread, coalesced 8-byte voxel words; unchanged voxels are not
+// written back.
+__global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ views, const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  const int nalloc = v.ctr[C_NUM_ALLOC];
+  const Intr s = intr_of(v);
+  const M4 Ti = load_m4(v.posef + PF_TINV);
+  const float vs = v.voxel_size, tau = v.trunc;
+  const double half = 0.5 * (double)vs * 8.0;
+  for (int blk = blockIdx.x; blk < nalloc; blk += gridDim.x) {
+    const int4 key = v.block_key[blk];
+    const f3 pos = mk((float)key.x * vs * 8.0f, (float)key.y * vs * 8.0f, (float)key.z * vs * 8.0f);
+    const f3 pc = xform(Ti, pos);
+    if (pc.z < 0) continue;
+    const f3 bc = mk((float)((double)pc.x + half), (float)((double)pc.y + half), (float)((double)pc.z + half));
+    int iu, iv;
+    project(bc, s, iu, iv);
+    if (!(iu >= 0 && iv >= 0 && iu < s.cols && iv < s.rows)) continue;
+    uint64_t *bv = v.vox + (size_t)blk * kBlockVox;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int t = threadIdx.x + 256 * h;
+      const int bx = t >> 6, by = (t >> 3) & 7, bz = t & 7;
+      const f3 vp = mk(pos.x + (float)bx * vs, pos.y + (float)by * vs, pos.z + (float)bz * vs);
+      const f3 vc = xform(Ti, vp);
+      int pu, pv;
+      project(vc, s, pu, pv);
+      if (!(pu >= 0 && pv >= 0 && pu < s.cols && pv < s.rows)) continue;
+      const int idx = pv * s.cols + pu;
+      const float d = v.depth[idx];
+      if (d < v.min_depth) continue;
+      if (d > v.max_depth) continue;
+      const f3 p3 = get_point3d(idx, d, s);
+      const float surf = norm(mk(0.f, 0.f, 0.f) - p3);
+      const float vox = norm(mk(0.f, 0.f, 0.f) - vc);
+      float nsdf;
+      if (vox > surf - tau && vox < surf + tau && d < v.max_depth) nsdf = surf - vox;
+      else if (vox < surf - tau) nsdf = tau;
+      else continue;
+      uint32_t nr = 0, ng = 0, nb = 0;
+      if (v.rgb) {
+        nr = v.rgb[3 * idx]; ng = v.rgb[3 * idx + 1]; nb = v.rgb[3 * idx + 2];
+      }
+      const uint64_t old = bv[t];
+      const uint32_t rgbw = vox_rgbw(old);
+      const float w = (float)(rgbw >> 24);
+      const float sdf = (vox_sdf(old) * w + nsdf * 1.0f) / (w + 1.0f);
+      const float cr = ((float)(rgbw & 0xff) * w + (float)nr * 1.0f) / (w + 1.0f);
+      const float cg = ((float)((rgbw >> 8) & 0xff) * w + (float)ng * 1.0f) / (w + 1.0f);
+      const float cb = ((float)((rgbw >> 16) & 0xff) * w + (float)nb * 1.0f) / (w + 1.0f);
+      int nw = (int)(rgbw >> 24) + 1;
+      if (nw > v.max_weight) nw = v.max_weight;
+      const uint32_t nrgbw = ((uint32_t)cr & 0xff) | (((uint32_t)cg & 0xff) << 8) | (((uint32_t)cb & 0xff) << 16) |
+                             ((uint32_t)nw << 24);
+      const uint64_t nword = ((uint64_t)nrgbw << 32) | (uint64_t)__float_as_uint(sdf);
+      if (nword != old) bv[t] = nword;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Raycast (GenerateDepthKernel / GenerateRgbKernel, tsdf.cu:470-526).
+// K: samples are GetInterpolatedVoxel at T * getPoint3d(i, z).
+// D: step 0.5*trunc through unobserved space, min(sdf, 0.5*trunc) (at least
+// half a voxel) through observed space; the hit is the first +/- zero
+// crossing, linearly interpolated; normals are the central-difference
+// gradient of the interpolated SDF, in camera coordinates.  The model maps of
+// all pyramid levels are written by subsampling (pixel (2^l u, 2^l v)).
+__device__ __forceinline__ float intensity_of(uint32_t r, uint32_t g, uint32_t b) {
+  return (0.2126f * (float)r + 0.7152f * (float)g + 0.0722f * (float)b) * (1.0f / 255.0f);
+}
+
+__global__ __launch_bounds__(256) void k_raycast(const DmView *__restrict__ views, const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  const int tiles_x = (v.cols + 15) >> 4;
+  const int tile = blockIdx.x;
+  if (tile >= tiles_x * ((v.rows + 15) >> 4)) return;
+  const int u = (tile % tiles_x) * 16 + (threadIdx.x & 15);
+  const int r = (tile / tiles_x) * 16 + (threadIdx.x >> 4);
+  if (u >= v.cols || r >= v.rows) return;
+  const int i = r * v.cols + u;
+  const Intr s = intr_of(v);
+  const M4 T = load_m4(v.posef + PF_T);
+  const M4 Ti = load_m4(v.posef + PF_TINV);
+  const float vs = v.voxel_size, tau = v.trunc;
+  const float big = 0.5f * tau, small = 0.5f * vs;
+  const int max_steps = (int)ceil((double)((v.max_depth - v.min_depth) / small)) + 2;
+  BlockCache c;
+  c.init();
+  float z = v.min_depth, pz = 0.f, ps = 0.f, zh = 0.f;
+  bool pobs = false, hit = false;
+  for (int it = 0; it < max_steps && z < v.max_depth; ++it) {
+    float sdf;
+    const f3 p = xform(T, get_point3d(i, z, s));
+    if (!interp_voxel(v, p, c, sdf, nullptr)) { pobs = false; z = z + big; continue; }
+    if (sdf <= 0.f) {
+      if (pobs && ps > 0.f) { zh = pz + (z - pz) * (ps / (ps - sdf)); hit = true; }
+      break;
+    }
+    pobs = true; ps = sdf; pz = z;
+    float st = sdf < big ? sdf : big;
+    if (st < small) st = small;
+    z = z + st;
+  }
+  float4 vert = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 nrm = make_float4(__int_as_float(0x7fc00000), __int_as_float(0x7fc00000), __int_as_float(0x7fc00000), 0.f);
+  uint32_t cr = 0, cg = 0, cb = 0;
+  float dout = 0.f;
+  if (hit) {
+    const f3 pc = get_point3d(i, zh, s);
+    const f3 pw = xform(T, pc);
+    dout = zh;
+    float sp[3], sm[3], col[3];
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const f3 e = mk(a == 0 ? vs : 0.0f, a == 1 ? vs : 0.0f, a == 2 ? vs : 0.0f);
+      ok &= interp_voxel(v, pw + e, c, sp[a], nullptr);
+      ok &= interp_voxel(v, pw - e, c, sm[a], nullptr);
+    }
+    if (ok) {
+      const f3 n = rot(Ti, normalize(mk(sp[0] - sm[0], sp[1] - sm[1], sp[2] - sm[2])));
+      nrm = make_float4(n.x, n.y, n.z, 0.f);
+    }
+    float sdf;
+    if (interp_voxel(v, pw, c, sdf, col)) { cr = to_u8(col[0]); cg = to_u8(col[1]); cb = to_u8(col[2]); }
+    vert = make_float4(pc.x, pc.y, pc.z, intensity_of(cr, cg, cb));
+  }
+  v.tvert[0][i] = vert;
+  v.tnorm[0][i] = nrm;
+  if (v.ray_depth) v.ray_depth[i] = dout;
+  if (v.ray_rgb) {
+    uint8_t *o = v.ray_rgb + 3 * (size_t)i;
+    o[0] = (uint8_t)cr; o[1] = (uint8_t)cg; o[2] = (uint8_t)cb;
+  }
+  // coarser model levels by nearest subsampling
+  if (((u | r) & 1) == 0 && (u >> 1) < (v.cols >> 1) && (r >> 1) < (v.rows >> 1)) {
+    const int j = (r >> 1) * (v.cols >> 1) + (u >> 1);
+    v.tvert[1][j] = vert;
+    v.tnorm[1][j] = nrm;
+  }
+  if (((u | r) & 3) == 0 && (u >> 2) < (v.cols >> 2) && (r >> 2) < (v.rows >> 2)) {
+    const int j = (r >> 2) * (v.cols >> 2) + (u >> 2);
+    v.tvert[2][j] = vert;
+    v.tnorm[2][j] = nrm;
+  }
+  if (i == 0) v.ctr[C_HAS_MODEL] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+static constexpr unsigned kListWG = 64;  // workgroups per session for the new-key passes
+
+void launch_alloc(const Launch &L) {
+  if (L.nact <= 0) return;
+  hipLaunchKernelGGL(k_alloc_rays, dim3((unsigned)L.max_tiles, (unsigned)L.nact), dim3(256), 0, L.stream, L.views,
+                     L.active);
+  hipLaunchKernelGGL(k_alloc_link, dim3(kListWG, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
+  hipLaunchKernelGGL(k_alloc_rank, dim3(kListWG, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
+  hipLaunchKernelGGL(k_alloc_commit, dim3(kListWG, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
+  hipLaunchKernelGGL(k_alloc_finish, dim3(kListWG, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
+  hipLaunchKernelGGL(k_alloc_close, dim3((unsigned)L.nact), dim3(64), 0, L.stream, L.views, L.active);
+}
+
+void launch_integrate(const Launch &L) {
+  if (L.nact <= 0) return;
+  hipLaunchKernelGGL(k_integrate, dim3(2048, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
+}
+
+void launch_raycast(const Launch &L) {
+  if (L.nact <= 0) return;
+  hipLaunchKernelGGL(k_raycast, dim3((unsigned)L.max_tiles, (unsigned)L.nact), dim3(256), 0, L.stream, L.views,
+                     L.active);
+}
+
+}  // namespace dm

@@ -1,0 +1,344 @@
+/*
+ * Seeded synthetic RGB-D generator for the rigid-fusion workloads.
+ *
+ * Scenes are analytic: the interior of an axis-aligned room, solid boxes and
+ * spheres.  A frame is rendered by exact ray casting in FP64 (pinhole camera,
+ * z-depth), then Kinect-like axial noise N(0, 0.0012 + 0.0019 (z - 0.4)^2) and
+ * 1 % dropout (depth = 0) are applied.  Colour is a seeded value-noise albedo
+ * evaluated at the hit point in world coordinates, so it is view independent
+ * (Lambertian, no shading) and a photometric residual is consistent across
+ * frames.  Trajectories are a smooth random walk clamped to 2 cm and 1 degree
+ * per frame.
+ *
+ * All randomness comes from SplitMix64 on (seed, stream, counter), so every
+ * output is a pure function of its arguments and independent of threading.
+ * This file is harness code for tests and benchmarks, not part of the mapping
+ * path.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#define DMS_MAX_PRIMS 64
+
+enum { PRIM_ROOM = 0, PRIM_BOX = 1, PRIM_SPHERE = 2 };
+
+typedef struct {
+  int kind;
+  double lo[3], hi[3]; /* boxes and room */
+  double c[3], r;      /* spheres */
+  double rgb[3];       /* base albedo in [0,1] */
+} dms_prim;
+
+typedef struct {
+  int nprims;
+  dms_prim prims[DMS_MAX_PRIMS];
+  double room_lo[3], room_hi[3];
+  double cam_center[3]; /* trajectories stay near this point */
+  double cam_radius;    /* ... within this radius */
+  uint64_t seed;
+} dms_scene;
+
+static uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+
+static uint64_t hash3(uint64_t seed, uint64_t a, uint64_t b) {
+  return splitmix64(splitmix64(splitmix64(seed) ^ a) ^ b);
+}
+
+/* uniform in [0,1) with 53 random bits */
+static double u01(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+
+typedef struct { uint64_t seed, ctr; } dms_rng;
+static double rng_u(dms_rng *r) { return u01(hash3(r->seed, 0x5EEDULL, r->ctr++)); }
+static double rng_range(dms_rng *r, double a, double b) { return a + (b - a) * rng_u(r); }
+static double rng_normal(dms_rng *r) {
+  double u1 = rng_u(r), u2 = rng_u(r);
+  if (u1 < 1e-300) u1 = 1e-300;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+static void add_sphere(dms_scene *s, double x, double y, double z, double r, dms_rng *g) {
+  dms_prim *p = &s->prims[s->nprims++];
+  memset(p, 0, sizeof(*p));
+  p->kind = PRIM_SPHERE;
+  p->c[0] = x; p->c[1] = y; p->c[2] = z; p->r = r;
+  for (int k = 0; k < 3; ++k) p->rgb[k] = rng_range(g, 0.25, 1.0);
+}
+
+static void add_box(dms_scene *s, const double lo[3], const double hi[3], int kind, dms_rng *g) {
+  dms_prim *p = &s->prims[s->nprims++];
+  memset(p, 0, sizeof(*p));
+  p->kind = kind;
+  for (int k = 0; k < 3; ++k) { p->lo[k] = lo[k]; p->hi[k] = hi[k]; p->rgb[k] = rng_range(g, 0.25, 1.0); }
+}
+
+static double dist_point_box(const double q[3], const double lo[3], const double hi[3]) {
+  double d2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double e = q[k] < lo[k] ? lo[k] - q[k] : (q[k] > hi[k] ? q[k] - hi[k] : 0.0);
+    d2 += e * e;
+  }
+  return sqrt(d2);
+}
+
+/*
+ * kind 1: 4 x 3 x 3 m room with 5 spheres, r ~ U[0.2, 0.5].
+ * kind 5: 8 x 8 x 3 m multi-room floor with 12 boxes (4 of them wall
+ *         segments splitting the floor into rooms) and 20 spheres.
+ * Objects keep clear of a ball around cam_center where the camera moves.
+ */
+int dms_scene_init(dms_scene *s, int kind, uint64_t seed) {
+  memset(s, 0, sizeof(*s));
+  s->seed = seed;
+  dms_rng g = {seed, 0};
+  if (kind == 1) {
+    double lo[3] = {-2.0, -1.5, -1.5}, hi[3] = {2.0, 1.5, 1.5};
+    memcpy(s->room_lo, lo, sizeof lo); memcpy(s->room_hi, hi, sizeof hi);
+    add_box(s, lo, hi, PRIM_ROOM, &g);
+    s->cam_center[0] = 0.0; s->cam_center[1] = 0.0; s->cam_center[2] = 0.0;
+    s->cam_radius = 0.35;
+    int placed = 0, tries = 0;
+    while (placed < 5 && tries < 100000) {
+      ++tries;
+      double r = rng_range(&g, 0.2, 0.5);
+      double c[3];
+      for (int k = 0; k < 3; ++k) c[k] = rng_range(&g, lo[k] + r, hi[k] - r);
+      double dc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+      if (dc < r + s->cam_radius + 0.45) continue; /* keep the camera ball clear */
+      int ok = 1;
+      for (int i = 1; i < s->nprims; ++i) {
+        const dms_prim *o = &s->prims[i];
+        double dx = o->c[0] - c[0], dy = o->c[1] - c[1], dz = o->c[2] - c[2];
+        if (sqrt(dx * dx + dy * dy + dz * dz) < o->r + r + 0.05) ok = 0;
+      }
+      if (!ok) continue;
+      add_sphere(s, c[0], c[1], c[2], r, &g);
+      ++placed;
+    }
+    return placed == 5 ? 0 : -1;
+  }
+  if (kind == 5) {
+    double lo[3] = {-4.0, -1.5, -4.0}, hi[3] = {4.0, 1.5, 4.0};
+    memcpy(s->room_lo, lo, sizeof lo); memcpy(s->room_hi, hi, sizeof hi);
+    add_box(s, lo, hi, PRIM_ROOM, &g);
+    s->cam_center[0] = -1.2; s->cam_center[1] = 0.0; s->cam_center[2] = -1.2;
+    s->cam_radius = 0.5;
+    /* four wall segments with door gaps: rooms around the origin */
+    double w[4][6] = {
+        {-4.0, -1.5, 0.9, -1.9, 1.5, 1.05}, {-0.6, -1.5, 0.9, 4.0, 1.5, 1.05},
+        {0.9, -1.5, -4.0, 1.05, 1.5, -2.1}, {0.9, -1.5, -0.2, 1.05, 1.5, 0.9}};
+    for (int i = 0; i < 4; ++i) add_box(s, w[i], w[i] + 3, PRIM_BOX, &g);
+    int boxes = 0, tries = 0;
+    while (boxes < 8 && tries < 100000) {
+      ++tries;
+      double e[3] = {rng_range(&g, 0.3, 0.9), rng_range(&g, 0.3, 1.2), rng_range(&g, 0.3, 0.9)};
+      double c[3] = {rng_range(&g, lo[0] + e[0], hi[0] - e[0]), hi[1] - e[1],
+                     rng_range(&g, lo[2] + e[2], hi[2] - e[2])};
+      double blo[3] = {c[0] - e[0], c[1] - e[1], c[2] - e[2]};
+      double bhi[3] = {c[0] + e[0], c[1] + e[1], c[2] + e[2]};
+      if (dist_point_box(s->cam_center, blo, bhi) < s->cam_radius + 0.6) continue;
+      add_box(s, blo, bhi, PRIM_BOX, &g);
+      ++boxes;
+    }
+    int sph = 0;
+    tries = 0;
+    while (sph < 20 && tries < 100000) {
+      ++tries;
+      double r = rng_range(&g, 0.15, 0.45);
+      double c[3];
+      for (int k = 0; k < 3; ++k) c[k] = rng_range(&g, lo[k] + r, hi[k] - r);
+      double dx = c[0] - s->cam_center[0], dy = c[1] - s->cam_center[1], dz = c[2] - s->cam_center[2];
+      if (sqrt(dx * dx + dy * dy + dz * dz) < r + s->cam_radius + 0.6) continue;
+      add_sphere(s, c[0], c[1], c[2], r, &g);
+      ++sph;
+    }
+    return (boxes == 8 && sph == 20) ? 0 : -1;
+  }
+  return -2;
+}
+
+/* Value noise in [0,1] on a lattice of `cell` metres. */
+static double value_noise(uint64_t seed, const double x[3], double cell) {
+  double q[3];
+  int64_t i0[3];
+  double f[3];
+  for (int k = 0; k < 3; ++k) {
+    q[k] = x[k] / cell;
+    double fl = floor(q[k]);
+    i0[k] = (int64_t)fl;
+    double t = q[k] - fl;
+    f[k] = t * t * (3.0 - 2.0 * t);
+  }
+  double acc = 0.0;
+  for (int c = 0; c < 8; ++c) {
+    int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+    uint64_t h = hash3(seed, (uint64_t)(i0[0] + dx) * 0x100000001B3ULL ^ (uint64_t)(i0[1] + dy),
+                       (uint64_t)(i0[2] + dz));
+    double w = (dx ? f[0] : 1.0 - f[0]) * (dy ? f[1] : 1.0 - f[1]) * (dz ? f[2] : 1.0 - f[2]);
+    acc += w * u01(h);
+  }
+  return acc;
+}
+
+/* Nearest hit along o + t d (t > 0); returns t and the primitive index. */
+static double cast(const dms_scene *s, const double o[3], const double d[3], int *hit_prim) {
+  double best = INFINITY;
+  int bp = -1;
+  for (int i = 0; i < s->nprims; ++i) {
+    const dms_prim *p = &s->prims[i];
+    if (p->kind == PRIM_SPHERE) {
+      double oc[3] = {o[0] - p->c[0], o[1] - p->c[1], o[2] - p->c[2]};
+      double a = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double b = oc[0] * d[0] + oc[1] * d[1] + oc[2] * d[2];
+      double c = oc[0] * oc[0] + oc[1] * oc[1] + oc[2] * oc[2] - p->r * p->r;
+      double disc = b * b - a * c;
+      if (disc < 0.0) continue;
+      double sq = sqrt(disc);
+      double t = (-b - sq) / a;
+      if (t <= 1e-9) t = (-b + sq) / a;
+      if (t > 1e-9 && t < best) { best = t; bp = i; }
+    } else {
+      double tmin = -INFINITY, tmax = INFINITY;
+      int ok = 1;
+      for (int k = 0; k < 3; ++k) {
+        if (d[k] == 0.0) {
+          if (o[k] < p->lo[k] || o[k] > p->hi[k]) ok = 0;
+          continue;
+        }
+        double t1 = (p->lo[k] - o[k]) / d[k], t2 = (p->hi[k] - o[k]) / d[k];
+        if (t1 > t2) { double tt = t1; t1 = t2; t2 = tt; }
+        if (t1 > tmin) tmin = t1;
+        if (t2 < tmax) tmax = t2;
+      }
+      if (!ok || tmin > tmax) continue;
+      double t = (p->kind == PRIM_ROOM) ? tmax : (tmin > 1e-9 ? tmin : INFINITY);
+      if (t > 1e-9 && t < best) { best = t; bp = i; }
+    }
+  }
+  *hit_prim = bp;
+  return best;
+}
+
+/*
+ * Render one frame.  pose: row-major 4x4 camera-to-world (x right, y down,
+ * z forward).  depth: rows*cols float32 metres (0 = no return).  rgb: rows*cols*3
+ * uint8 or NULL.  noise != 0 applies axial noise and 1 % dropout.
+ */
+void dms_render(const dms_scene *s, const double pose[16], int rows, int cols, double fx,
+                double fy, double cx, double cy, uint64_t frame_seed, int noise, float *depth,
+                uint8_t *rgb) {
+  const double o[3] = {pose[3], pose[7], pose[11]};
+#pragma omp parallel for schedule(static)
+  for (int v = 0; v < rows; ++v) {
+    for (int u = 0; u < cols; ++u) {
+      int i = v * cols + u;
+      double dc[3] = {((double)u - cx) / fx, ((double)v - cy) / fy, 1.0};
+      double d[3];
+      for (int r = 0; r < 3; ++r) d[r] = pose[4 * r + 0] * dc[0] + pose[4 * r + 1] * dc[1] + pose[4 * r + 2] * dc[2];
+      int prim;
+      double t = cast(s, o, d, &prim);
+      double z = (prim >= 0 && isfinite(t)) ? t : 0.0;
+      if (rgb) {
+        uint8_t c3[3] = {0, 0, 0};
+        if (prim >= 0) {
+          double x[3] = {o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
+          double n = value_noise(s->seed ^ 0xA1BED0ULL, x, 0.07);
+          double n2 = value_noise(s->seed ^ 0xC0102ULL, x, 0.23);
+          for (int k = 0; k < 3; ++k) {
+            double val = s->prims[prim].rgb[k] * (0.25 + 0.5 * n + 0.25 * n2);
+            int q = (int)floor(val * 255.0 + 0.5);
+            c3[k] = (uint8_t)(q < 0 ? 0 : (q > 255 ? 255 : q));
+          }
+        }
+        rgb[3 * i + 0] = c3[0]; rgb[3 * i + 1] = c3[1]; rgb[3 * i + 2] = c3[2];
+      }
+      if (noise && z > 0.0) {
+        dms_rng g = {hash3(frame_seed, 0xD5EULL, (uint64_t)i), 0};
+        if (rng_u(&g) < 0.01) {
+          z = 0.0;
+        } else {
+          double sig = 0.0012 + 0.0019 * (z - 0.4) * (z - 0.4);
+          z += sig * rng_normal(&g);
+          if (z < 0.0) z = 0.0;
+        }
+      }
+      depth[i] = (float)z;
+    }
+  }
+}
+
+/* Rotation matrix (row-major) from axis-angle w (radians). */
+static void rodrigues(const double w[3], double R[9]) {
+  double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  double k[3] = {0, 0, 0};
+  if (th > 0) { k[0] = w[0] / th; k[1] = w[1] / th; k[2] = w[2] / th; }
+  double c = cos(th), sn = sin(th), C = 1.0 - c;
+  R[0] = c + k[0] * k[0] * C;        R[1] = k[0] * k[1] * C - k[2] * sn; R[2] = k[0] * k[2] * C + k[1] * sn;
+  R[3] = k[1] * k[0] * C + k[2] * sn; R[4] = c + k[1] * k[1] * C;        R[5] = k[1] * k[2] * C - k[0] * sn;
+  R[6] = k[2] * k[0] * C - k[1] * sn; R[7] = k[2] * k[1] * C + k[0] * sn; R[8] = c + k[2] * k[2] * C;
+}
+
+static void matmul3(const double A[9], const double B[9], double C[9]) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) C[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
+}
+
+/*
+ * Smooth seeded random walk: per-frame translation <= 2 cm and rotation
+ * <= 1 degree, staying within cam_radius of cam_center.  poses: nframes * 16
+ * row-major camera-to-world.
+ */
+void dms_trajectory(const dms_scene *s, int nframes, uint64_t seed, double *poses) {
+  dms_rng g = {seed ^ 0x7A3EC7ULL, 0};
+  const double max_step = 0.02, max_rot = 1.0 * 3.141592653589793 / 180.0;
+  double p[3] = {s->cam_center[0], s->cam_center[1], s->cam_center[2]};
+  double yaw0 = rng_range(&g, -3.14159, 3.14159);
+  /* look about 23 degrees down (world +y is down) so that the floor and the
+     floor-wall edges stay in view and constrain the tracking */
+  double R[9], Ry[9], Rx[9];
+  double wy[3] = {0.0, yaw0, 0.0}, wx[3] = {-0.4, 0.0, 0.0};
+  rodrigues(wy, Ry);
+  rodrigues(wx, Rx);
+  matmul3(Ry, Rx, R);
+  double v[3] = {0, 0, 0}, om[3] = {0, 0, 0};
+  for (int f = 0; f < nframes; ++f) {
+    if (f > 0) {
+      for (int k = 0; k < 3; ++k) {
+        double pull = (s->cam_center[k] - p[k]) * 0.02;
+        v[k] = 0.9 * v[k] + 0.003 * rng_normal(&g) + pull;
+        om[k] = 0.9 * om[k] + 0.0025 * rng_normal(&g);
+      }
+      om[1] += 0.001; /* slow pan so the camera sweeps the scene */
+      double vn = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      if (vn > max_step) for (int k = 0; k < 3; ++k) v[k] *= max_step / vn;
+      double on = sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
+      if (on > max_rot) for (int k = 0; k < 3; ++k) om[k] *= max_rot / on;
+      for (int k = 0; k < 3; ++k) p[k] += v[k];
+      double dx = p[0] - s->cam_center[0], dy = p[1] - s->cam_center[1], dz = p[2] - s->cam_center[2];
+      double dn = sqrt(dx * dx + dy * dy + dz * dz);
+      if (dn > s->cam_radius) {
+        p[0] = s->cam_center[0] + dx * s->cam_radius / dn;
+        p[1] = s->cam_center[1] + dy * s->cam_radius / dn;
+        p[2] = s->cam_center[2] + dz * s->cam_radius / dn;
+      }
+      /* keep the camera roughly upright: damp roll (camera z) and pitch */
+      double dR[9], Rn[9];
+      rodrigues(om, dR);
+      matmul3(dR, R, Rn);
+      memcpy(R, Rn, sizeof R);
+      om[0] *= 0.8; om[2] *= 0.8;
+    }
+    double *P = poses + 16 * (size_t)f;
+    P[0] = R[0]; P[1] = R[1]; P[2] = R[2];  P[3] = p[0];
+    P[4] = R[3]; P[5] = R[4]; P[6] = R[5];  P[7] = p[1];
+    P[8] = R[6]; P[9] = R[7]; P[10] = R[8]; P[11] = p[2];
+    P[12] = 0;   P[13] = 0;   P[14] = 0;    P[15] = 1;
+  }
+}
+
+int dms_scene_size(void) { return (int)sizeof(dms_scene); }
