@@ -71,9 +71,24 @@ def build_synth():
     return out
 
 
+def build_cpp_tests():
+    """The C++ program that drives the reference-shaped classes
+    (tests/cpp/test_builder.cpp), linked against the HIP library."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_builder.cpp")
+    if not os.path.exists(src):
+        return None
+    out_dir = os.path.join(ROOT, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "test_builder")
+    _run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), src, "-L", LIB, "-ldynamap_hip",
+          "-Wl,-rpath," + LIB, "-Wl,-rpath,$ORIGIN/../../../dynamap_amd/_lib", "-o", out])
+    return out
+
+
 def build_all():
     build_synth()
     build_hip()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":

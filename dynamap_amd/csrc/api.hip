@@ -43,7 +43,7 @@ size_t session_bytes(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   const size_t P = (size_t)rows * cols;
   const size_t nc = (size_t)new_cap(p.num_blocks);
   size_t b = (size_t)p.num_buckets * bucket_stride(p.bucket_size);
-  b += (size_t)p.num_blocks * (kBlockVox * 8 + 16);
+  b += (size_t)p.num_blocks * (kBlockVox * 8 + 16 + 1);
   b += (size_t)pow2_at_least((int)(2 * nc)) * 8 + nc * (8 + 4 * 4) + (size_t)p.num_buckets * 4;
   size_t maps = 0;
   for (int l = 0; l < kLevels; ++l) maps += (size_t)(rows >> l) * (cols >> l) * 16 * 4;
@@ -97,6 +97,7 @@ int Sess::init(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   DM_TRY(dalloc(&h.buckets, (size_t)p.num_buckets * h.bstride));
   DM_TRY(dalloc(&h.vox, (size_t)p.num_blocks * kBlockVox));
   DM_TRY(dalloc(&h.block_key, (size_t)p.num_blocks));
+  DM_TRY(dalloc(&h.blk_near, (size_t)p.num_blocks));
   DM_TRY(dalloc(&h.dset, (size_t)dsz));
   DM_TRY(dalloc(&h.newkeys, (size_t)h.newcap, false));
   DM_TRY(dalloc(&h.newdslot, (size_t)h.newcap, false));

@@ -29,6 +29,9 @@ enum Ctr : int {
   C_DSET_OVF = 7,    // new keys lost to a full per-frame key set (cumulative)
   C_HAS_MODEL = 8,   // a raycast model exists (tracking can run)
   C_ICP_FAIL = 9,    // frames whose increment was reset by the guard (cumulative)
+  C_NVIS = 10,       // blocks that passed the frustum test in the last integration
+  C_NVISIT = 11,     // block visits of the last allocation's ray traversal
+  C_NNEW = 12,       // keys inserted by the last allocation
   C_COUNT = 16
 };
 
@@ -56,6 +59,7 @@ struct DmView {
   uint8_t *buckets;     // num_buckets * bstride: u64 key[S] | i32 ptr[S] | i32 count
   uint64_t *vox;        // num_blocks * 512 voxel words (sdf bits | r | g<<8 | b<<16 | w<<24 <<32)
   int4 *block_key;      // num_blocks block coordinates
+  uint8_t *blk_near;    // num_blocks: 1 when an observed voxel lies below the raycast far threshold
   uint64_t *dset;       // per-frame new-key set, dset_mask + 1 slots, 0 = empty
   uint64_t *newkeys;    // newcap
   int32_t *newdslot;    // newcap
@@ -96,6 +100,7 @@ struct Launch {
 };
 
 void launch_alloc(const Launch &L);
+void launch_alloc_rays(const Launch &L);  // traversal kernel only (profiling)
 void launch_integrate(const Launch &L);
 void launch_raycast(const Launch &L);
 void launch_source_maps(const Launch &L);

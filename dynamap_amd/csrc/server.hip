@@ -65,6 +65,7 @@ int sess_reset(Sess &S, const double init_pose[16]) {
   n = std::min(std::max(n, 0), S.h.num_blocks);
   DM_HIP(hipMemsetAsync(S.h.buckets, 0, (size_t)S.h.num_buckets * S.h.bstride, S.stream));
   DM_HIP(hipMemsetAsync(S.h.vox, 0, (size_t)n * kBlockVox * 8, S.stream));
+  DM_HIP(hipMemsetAsync(S.h.blk_near, 0, (size_t)n, S.stream));
   DM_HIP(hipMemsetAsync(S.h.ctr, 0, C_COUNT * 4, S.stream));
   double pd[PD_COUNT] = {0};
   for (int k = 0; k < 16; ++k) {
@@ -498,6 +499,92 @@ int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int32_t nfr
   }
   if (wall_seconds) *wall_seconds = secs_since(t0);
   if (batches) *batches = nbatch;
+  return DM_OK;
+}
+
+// Time each stage of the batched pipeline with HIP events on the server's
+// stream, over the admitted sessions at trace frame `frame` (the sessions'
+// state is advanced by the timed launches).  stage_ms[k] = mean duration of
+// one launch of stage k: 0 source maps, 1 ICP reduction (level 0),
+// 2 ICP solve, 3 allocation (all six kernels), 4 integration, 5 raycast,
+// 6 ray traversal kernel alone.  work (summed over sessions, per launch):
+// [0] pixels, [1] visible blocks, [2] block visits of the traversal, [3] new
+// blocks, [4] sessions, [5] allocated blocks.
+int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage_ms, int64_t *work) {
+  if (!srv || reps <= 0 || !stage_ms) { set_error("invalid profile arguments"); return DM_E_INVALID; }
+  std::vector<int> act;
+  for (int t = 0; t < (int)srv->traces.size(); ++t)
+    if (srv->traces[t].sess && frame < srv->traces[t].nframes) act.push_back(t);
+  if (act.empty()) { set_error("no admitted session to profile"); return DM_E_STATE; }
+  DM_HIP(hipStreamSynchronize(srv->stream));
+  DmView *tab = srv->h_table[0];
+  int *lst = srv->h_list[0];
+  for (int s = 0; s < srv->cfg.max_sessions; ++s) lst[s] = -1;
+  int64_t pixels = 0;
+  for (size_t q = 0; q < act.size(); ++q) {
+    Trace &tr = srv->traces[act[q]];
+    const size_t P = (size_t)tr.sensor.rows * tr.sensor.cols;
+    tr.sess->h.depth = tr.d_depth + P * (size_t)frame;
+    tr.sess->h.rgb = tr.d_rgb ? tr.d_rgb + P * 3 * (size_t)frame : nullptr;
+    tab[tr.slot] = tr.sess->h;
+    lst[q] = tr.slot;
+    pixels += (int64_t)P;
+  }
+  DM_HIP(hipMemcpyAsync(srv->d_table, tab, sizeof(DmView) * srv->cfg.max_sessions, hipMemcpyHostToDevice, srv->stream));
+  DM_HIP(hipMemcpyAsync(srv->d_list, lst, sizeof(int) * srv->cfg.max_sessions, hipMemcpyHostToDevice, srv->stream));
+  Launch L = server_launch(srv);
+  // a full frame first so every session has a model and a pose
+  launch_pose_begin(L);
+  launch_source_maps(L);
+  launch_alloc(L);
+  launch_integrate(L);
+  launch_raycast(L);
+  DM_HIP(hipStreamSynchronize(srv->stream));
+  hipEvent_t e0, e1;
+  DM_HIP(hipEventCreate(&e0));
+  DM_HIP(hipEventCreate(&e1));
+  auto reset_ctr = [&]() -> int {
+    for (int t : act) {
+      Sess &S = *srv->traces[t].sess;
+      DM_HIP(hipMemsetAsync(S.h.ctr + C_NVIS, 0, 3 * sizeof(int32_t), srv->stream));
+    }
+    return DM_OK;
+  };
+  for (int st = 0; st < 7; ++st) {
+    int r = reset_ctr();
+    if (r != DM_OK) return r;
+    DM_HIP(hipEventRecord(e0, srv->stream));
+    for (int k = 0; k < reps; ++k) {
+      switch (st) {
+        case 0: launch_source_maps(L); break;
+        case 1: launch_icp_reduce(L, 0, 1); break;
+        case 2: launch_icp_solve(L, 0, 1); break;
+        case 3: launch_alloc(L); break;
+        case 4: launch_integrate(L); break;
+        case 5: launch_raycast(L); break;
+        case 6: launch_alloc_rays(L); break;
+      }
+    }
+    DM_HIP(hipEventRecord(e1, srv->stream));
+    DM_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    stage_ms[st] = (double)ms / reps;
+    if (work && (st == 4 || st == 6)) {
+      int64_t sum_vis = 0, sum_visit = 0, sum_new = 0, sum_alloc = 0;
+      for (int t : act) {
+        int32_t c[C_COUNT];
+        DM_HIP(hipMemcpy(c, srv->traces[t].sess->h.ctr, sizeof c, hipMemcpyDeviceToHost));
+        sum_vis += c[C_NVIS]; sum_visit += c[C_NVISIT]; sum_new += c[C_NNEW]; sum_alloc += c[C_NUM_ALLOC];
+      }
+      if (st == 4) { work[1] = sum_vis / reps; work[5] = sum_alloc; }
+      if (st == 6) { work[2] = sum_visit / reps; }
+    }
+  }
+  if (work) { work[0] = pixels; work[3] = 0; work[4] = (int64_t)act.size(); }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  DM_HIP(hipGetLastError());
   return DM_OK;
 }
 

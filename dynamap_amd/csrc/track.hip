@@ -288,6 +288,8 @@ __global__ void k_pose_begin(const DmView *__restrict__ views, const int *__rest
   }
   rigid_inverse_ff(v.posef + PF_T, v.posef + PF_TINV);
   for (int k = 0; k < 6; ++k) P[PD_ACC + k] = 0.0;
+  v.ctr[C_NVIS] = 0;
+  v.ctr[C_NVISIT] = 0;
 }
 
 void launch_source_maps(const Launch &L) {

@@ -59,26 +59,17 @@ __device__ __forceinline__ void visit_block(const DmView &v, int x, int y, int z
 // same workgroup.  K: ray from the camera centre to surface + trunc, the
 // (b + step) * bs boundary for both directions and the extra "neg_ray" block.
 // D: the traversal is bounded by the L1 distance to the end block.
-__global__ __launch_bounds__(256) void k_alloc_rays(const DmView *__restrict__ views,
-                                                     const int *__restrict__ active) {
-  const int slot_ = active[blockIdx.y];
-  if (slot_ < 0) return;
-  const DmView &v = views[slot_];
-  const int tiles_x = (v.cols + 15) >> 4;
-  const int tile = blockIdx.x;
-  if (tile >= tiles_x * ((v.rows + 15) >> 4)) return;
-  const int u = (tile % tiles_x) * 16 + (threadIdx.x & 15);
-  const int r = (tile / tiles_x) * 16 + (threadIdx.x >> 4);
-  if (u >= v.cols || r >= v.rows) return;
+__device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r) {
+  if (u >= v.cols || r >= v.rows) return 0;
   const int i = r * v.cols + u;
   const float d = v.depth[i];
-  if (d < v.min_depth || d > v.max_depth) return;
+  if (d < v.min_depth || d > v.max_depth) return 0;
   const Intr s = intr_of(v);
   const M4 T = load_m4(v.posef + PF_T);
   const float block_m = 8.0f * v.voxel_size;
   const f3 start = mk(T.m[3], T.m[7], T.m[11]);
   const f3 point = xform(T, get_point3d(i, d, s));
-  if (point.x == 0 && point.y == 0 && point.z == 0) return;
+  if (point.x == 0 && point.y == 0 && point.z == 0) return 0;
   const f3 dir = normalize(point - start);
   const float sd = norm(start - point);
   const f3 end = start + dir * (sd + v.trunc);
@@ -102,9 +93,11 @@ __global__ __launch_bounds__(256) void k_alloc_rays(const DmView *__restrict__ v
   if (by != ey && dir.y < 0) { dy--; neg = true; }
   if (bz != ez && dir.z < 0) { dz--; neg = true; }
   visit_block(v, bx, by, bz);
+  int nvisit = 1;
   if (neg) {
     bx += dx; by += dy; bz += dz;
     visit_block(v, bx, by, bz);
+    ++nvisit;
   }
   const long long bound = llabs((long long)ex - bx) + llabs((long long)ey - by) + llabs((long long)ez - bz);
   for (long long it = 0; it < bound && (bx != ex || by != ey || bz != ez); ++it) {
@@ -114,8 +107,30 @@ __global__ __launch_bounds__(256) void k_alloc_rays(const DmView *__restrict__ v
       if (mty < mtz) { by += sty; mty += dty; } else { bz += stz; mtz += dtz; }
     }
     visit_block(v, bx, by, bz);
+    ++nvisit;
   }
+  return nvisit;
 }
+
+__global__ __launch_bounds__(256) void k_alloc_rays(const DmView *__restrict__ views,
+                                                     const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  const int tiles_x = (v.cols + 15) >> 4;
+  const int tile = blockIdx.x;
+  if (tile >= tiles_x * ((v.rows + 15) >> 4)) return;
+  __shared__ int visits;
+  if (threadIdx.x == 0) visits = 0;
+  __syncthreads();
+  const int u = (tile % tiles_x) * 16 + (threadIdx.x & 15);
+  const int r = (tile / tiles_x) * 16 + (threadIdx.x >> 4);
+  const int n = alloc_ray(v, u, r);
+  if (n) atomicAdd(&visits, n);
+  __syncthreads();
+  if (threadIdx.x == 0 && visits) atomicAdd(&v.ctr[C_NVISIT], visits);
+}
+
 
 __device__ __forceinline__ int key_bucket(const DmView &v, uint64_t key) {
   int x, y, z;
@@ -219,8 +234,10 @@ __global__ void k_alloc_close(const DmView *__restrict__ views, const int *__res
   if (v.ctr[C_NUM_ALLOC] + nins <= v.num_blocks) {
     v.ctr[C_NUM_ALLOC] += nins;
     v.ctr[C_DROP_BUCKET] += n - nins;
+    v.ctr[C_NNEW] = nins;
   } else {
     v.ctr[C_DROP_POOL] += n;
+    v.ctr[C_NNEW] = 0;
   }
   v.ctr[C_N_NEW] = 0;
   v.ctr[C_N_INS] = 0;
@@ -233,6 +250,54 @@ __global__ void k_alloc_close(const DmView *__restrict__ views, const int *__res
 // One workgroup per voxel block (grid-stride over the allocated blocks), two
 // voxels per thread, coalesced 8-byte voxel words; unchanged voxels are not
 // written back.
+// One voxel of IntegrateScanKernel (tsdf.cu:301-341): returns the new voxel
+// word (the old one when the voxel is not updated).
+__device__ __forceinline__ uint64_t integrate_voxel(const DmView &v, const Intr &s, const M4 &Ti, f3 pos, int t,
+                                                    uint64_t old) {
+  const float vs = v.voxel_size, tau = v.trunc;
+  const int bx = t >> 6, by = (t >> 3) & 7, bz = t & 7;
+  const f3 vp = mk(pos.x + (float)bx * vs, pos.y + (float)by * vs, pos.z + (float)bz * vs);
+  const f3 vc = xform(Ti, vp);
+  int pu, pv;
+  project(vc, s, pu, pv);
+  if (!(pu >= 0 && pv >= 0 && pu < s.cols && pv < s.rows)) return old;
+  const int idx = pv * s.cols + pu;
+  const float d = v.depth[idx];
+  if (d < v.min_depth) return old;
+  if (d > v.max_depth) return old;
+  const f3 p3 = get_point3d(idx, d, s);
+  const float surf = norm(mk(0.f, 0.f, 0.f) - p3);
+  const float vox = norm(mk(0.f, 0.f, 0.f) - vc);
+  float nsdf;
+  if (vox > surf - tau && vox < surf + tau && d < v.max_depth) nsdf = surf - vox;
+  else if (vox < surf - tau) nsdf = tau;
+  else return old;
+  uint32_t nr = 0, ng = 0, nb = 0;
+  if (v.rgb) {
+    nr = v.rgb[3 * idx]; ng = v.rgb[3 * idx + 1]; nb = v.rgb[3 * idx + 2];
+  }
+  const uint32_t rgbw = vox_rgbw(old);
+  const float w = (float)(rgbw >> 24);
+  const float sdf = (vox_sdf(old) * w + nsdf * 1.0f) / (w + 1.0f);
+  const float cr = ((float)(rgbw & 0xff) * w + (float)nr * 1.0f) / (w + 1.0f);
+  const float cg = ((float)((rgbw >> 8) & 0xff) * w + (float)ng * 1.0f) / (w + 1.0f);
+  const float cb = ((float)((rgbw >> 16) & 0xff) * w + (float)nb * 1.0f) / (w + 1.0f);
+  int nw = (int)(rgbw >> 24) + 1;
+  if (nw > v.max_weight) nw = v.max_weight;
+  const uint32_t nrgbw = ((uint32_t)cr & 0xff) | (((uint32_t)cg & 0xff) << 8) | (((uint32_t)cb & 0xff) << 16) |
+                         ((uint32_t)nw << 24);
+  return ((uint64_t)nrgbw << 32) | (uint64_t)__float_as_uint(sdf);
+}
+
+// Far threshold of the raycast's empty-space test: a sample whose voxels all
+// have sdf at or above it (or are unobserved) interpolates to at least
+// 0.5*trunc, the march's full step (margin for the rounding of the
+// trilinear weights).
+__device__ __forceinline__ float far_threshold(float trunc) {
+  const float big = 0.5f * trunc;
+  return big + big * (1.0f / 256.0f);
+}
+
 __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ views, const int *__restrict__ active) {
   const int slot_ = active[blockIdx.y];
   if (slot_ < 0) return;
@@ -242,6 +307,8 @@ __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ vi
   const M4 Ti = load_m4(v.posef + PF_TINV);
   const float vs = v.voxel_size, tau = v.trunc;
   const double half = 0.5 * (double)vs * 8.0;
+  const float far_thr = far_threshold(tau);
+  int nvis = 0;
   for (int blk = blockIdx.x; blk < nalloc; blk += gridDim.x) {
     const int4 key = v.block_key[blk];
     const f3 pos = mk((float)key.x * vs * 8.0f, (float)key.y * vs * 8.0f, (float)key.z * vs * 8.0f);
@@ -251,46 +318,22 @@ __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ vi
     int iu, iv;
     project(bc, s, iu, iv);
     if (!(iu >= 0 && iv >= 0 && iu < s.cols && iv < s.rows)) continue;
+    ++nvis;
     uint64_t *bv = v.vox + (size_t)blk * kBlockVox;
+    bool near = false;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int t = threadIdx.x + 256 * h;
-      const int bx = t >> 6, by = (t >> 3) & 7, bz = t & 7;
-      const f3 vp = mk(pos.x + (float)bx * vs, pos.y + (float)by * vs, pos.z + (float)bz * vs);
-      const f3 vc = xform(Ti, vp);
-      int pu, pv;
-      project(vc, s, pu, pv);
-      if (!(pu >= 0 && pv >= 0 && pu < s.cols && pv < s.rows)) continue;
-      const int idx = pv * s.cols + pu;
-      const float d = v.depth[idx];
-      if (d < v.min_depth) continue;
-      if (d > v.max_depth) continue;
-      const f3 p3 = get_point3d(idx, d, s);
-      const float surf = norm(mk(0.f, 0.f, 0.f) - p3);
-      const float vox = norm(mk(0.f, 0.f, 0.f) - vc);
-      float nsdf;
-      if (vox > surf - tau && vox < surf + tau && d < v.max_depth) nsdf = surf - vox;
-      else if (vox < surf - tau) nsdf = tau;
-      else continue;
-      uint32_t nr = 0, ng = 0, nb = 0;
-      if (v.rgb) {
-        nr = v.rgb[3 * idx]; ng = v.rgb[3 * idx + 1]; nb = v.rgb[3 * idx + 2];
-      }
       const uint64_t old = bv[t];
-      const uint32_t rgbw = vox_rgbw(old);
-      const float w = (float)(rgbw >> 24);
-      const float sdf = (vox_sdf(old) * w + nsdf * 1.0f) / (w + 1.0f);
-      const float cr = ((float)(rgbw & 0xff) * w + (float)nr * 1.0f) / (w + 1.0f);
-      const float cg = ((float)((rgbw >> 8) & 0xff) * w + (float)ng * 1.0f) / (w + 1.0f);
-      const float cb = ((float)((rgbw >> 16) & 0xff) * w + (float)nb * 1.0f) / (w + 1.0f);
-      int nw = (int)(rgbw >> 24) + 1;
-      if (nw > v.max_weight) nw = v.max_weight;
-      const uint32_t nrgbw = ((uint32_t)cr & 0xff) | (((uint32_t)cg & 0xff) << 8) | (((uint32_t)cb & 0xff) << 16) |
-                             ((uint32_t)nw << 24);
-      const uint64_t nword = ((uint64_t)nrgbw << 32) | (uint64_t)__float_as_uint(sdf);
+      const uint64_t nword = integrate_voxel(v, s, Ti, pos, t, old);
       if (nword != old) bv[t] = nword;
+      near |= (nword >> 56) != 0 && vox_sdf(nword) < far_thr;
     }
+    // block summary used by the raycast's empty-space test
+    near = __syncthreads_or(near);
+    if (threadIdx.x == 0) v.blk_near[blk] = near ? 1 : 0;
   }
+  if (threadIdx.x == 0 && nvis) atomicAdd(&v.ctr[C_NVIS], nvis);
 }
 
 // ---------------------------------------------------------------------------
@@ -303,6 +346,49 @@ __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ vi
 // all pyramid levels are written by subsampling (pixel (2^l u, 2^l v)).
 __device__ __forceinline__ float intensity_of(uint32_t r, uint32_t g, uint32_t b) {
   return (0.2126f * (float)r + 0.7152f * (float)g + 0.0722f * (float)b) * (1.0f / 255.0f);
+}
+
+// Empty-space test of the raycast.  A sample is "far" when every block that
+// can hold one of the voxels GetInterpolatedVoxel reads at it is unallocated
+// or has no observed voxel below the far threshold: the sample is then
+// unobserved or interpolates to at least 0.5*trunc, and the march takes its
+// full step without interpolating.  The test uses an approximate sample
+// position (ray origin + z * direction, no divisions) and a voxel range
+// widened by a margin far larger than its error, so it is conservative: a
+// sample it calls far is far for the exact evaluation too.
+struct FarCache {
+  int bx[2], by[2], bz[2];
+  bool far[2];
+  int next;
+  __device__ void init() { bx[0] = bx[1] = 0x7fffffff; by[0] = by[1] = bz[0] = bz[1] = 0; next = 0; }
+};
+
+__device__ __forceinline__ bool block_far(const DmView &v, int bx, int by, int bz, FarCache &c) {
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (c.bx[k] == bx && c.by[k] == by && c.bz[k] == bz) return c.far[k];
+  const int ptr = find_block(v, bx, by, bz);
+  const bool f = ptr < 0 || v.blk_near[ptr] == 0;
+  const int k = c.next;
+  c.bx[k] = bx; c.by[k] = by; c.bz[k] = bz; c.far[k] = f;
+  c.next = k ^ 1;
+  return f;
+}
+
+__device__ __forceinline__ int vrange_block(float pv, float m) { return floordiv8((int)floorf(pv + m)); }
+
+__device__ __forceinline__ bool sample_is_far(const DmView &v, f3 pa, float inv_vs, FarCache &c) {
+  const float m = 1.05f;
+  const float px = pa.x * inv_vs, py = pa.y * inv_vs, pz = pa.z * inv_vs;
+  if (!(fabsf(px) < 1.0e6f && fabsf(py) < 1.0e6f && fabsf(pz) < 1.0e6f)) return false;
+  const int x0 = vrange_block(px, -m), x1 = vrange_block(px, m + 1.0f);
+  const int y0 = vrange_block(py, -m), y1 = vrange_block(py, m + 1.0f);
+  const int z0 = vrange_block(pz, -m), z1 = vrange_block(pz, m + 1.0f);
+  for (int bx = x0; bx <= x1; ++bx)
+    for (int by = y0; by <= y1; ++by)
+      for (int bz = z0; bz <= z1; ++bz)
+        if (!block_far(v, bx, by, bz, c)) return false;
+  return true;
 }
 
 __global__ __launch_bounds__(256) void k_raycast(const DmView *__restrict__ views, const int *__restrict__ active) {
@@ -324,19 +410,40 @@ __global__ __launch_bounds__(256) void k_raycast(const DmView *__restrict__ view
   const int max_steps = (int)ceil((double)((v.max_depth - v.min_depth) / small)) + 2;
   BlockCache c;
   c.init();
-  float z = v.min_depth, pz = 0.f, ps = 0.f, zh = 0.f;
-  bool pobs = false, hit = false;
+  FarCache fc;
+  fc.init();
+  const float inv_vs = 1.0f / vs;
+  // ray for the approximate empty-space test: origin + z * direction
+  const f3 org = mk(T.m[3], T.m[7], T.m[11]);
+  const f3 dir = rot(T, mk(((float)u - s.cx) / s.fx, ((float)r - s.cy) / s.fy, 1.0f));
+  float z = v.min_depth, pz = 0.f, ps = 0.f, zh = 0.f, zprev = 0.f;
+  bool pobs = false, hit = false, prev_fast = false;
   for (int it = 0; it < max_steps && z < v.max_depth; ++it) {
     float sdf;
+    // far sample: the step is 0.5*trunc whether it is observed or not; its
+    // interpolation is recomputed below only if the next sample needs it
+    if (sample_is_far(v, org + dir * z, inv_vs, fc)) {
+      prev_fast = true;
+      zprev = z;
+      z = z + big;
+      continue;
+    }
     const f3 p = xform(T, get_point3d(i, z, s));
-    if (!interp_voxel(v, p, c, sdf, nullptr)) { pobs = false; z = z + big; continue; }
+    if (!interp_voxel(v, p, c, sdf, nullptr)) { pobs = false; prev_fast = false; zprev = z; z = z + big; continue; }
     if (sdf <= 0.f) {
+      if (prev_fast) {
+        float s2;
+        pobs = interp_voxel(v, xform(T, get_point3d(i, zprev, s)), c, s2, nullptr);
+        ps = s2;
+        pz = zprev;
+      }
       if (pobs && ps > 0.f) { zh = pz + (z - pz) * (ps / (ps - sdf)); hit = true; }
       break;
     }
-    pobs = true; ps = sdf; pz = z;
+    pobs = true; ps = sdf; pz = z; prev_fast = false;
     float st = sdf < big ? sdf : big;
     if (st < small) st = small;
+    zprev = z;
     z = z + st;
   }
   float4 vert = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -397,6 +504,12 @@ void launch_alloc(const Launch &L) {
   hipLaunchKernelGGL(k_alloc_commit, dim3(kListWG, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
   hipLaunchKernelGGL(k_alloc_finish, dim3(kListWG, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
   hipLaunchKernelGGL(k_alloc_close, dim3((unsigned)L.nact), dim3(64), 0, L.stream, L.views, L.active);
+}
+
+void launch_alloc_rays(const Launch &L) {
+  if (L.nact <= 0) return;
+  hipLaunchKernelGGL(k_alloc_rays, dim3((unsigned)L.max_tiles, (unsigned)L.nact), dim3(256), 0, L.stream, L.views,
+                     L.active);
 }
 
 void launch_integrate(const Launch &L) {

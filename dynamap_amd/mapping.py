@@ -243,3 +243,13 @@ class Server:
 
     def reset(self):
         N.check(self._lib.dm_server_reset(self._h))
+
+    STAGES = ("source_maps", "icp_reduce_l0", "icp_solve", "alloc_all", "integrate", "raycast", "alloc_rays")
+
+    def profile(self, frame, reps=5):
+        ms = np.zeros(7)
+        work = np.zeros(6, np.int64)
+        N.check(self._lib.dm_server_profile(self._h, frame, reps, N.ptr(ms), N.ptr(work)))
+        return dict(zip(self.STAGES, ms.tolist())), dict(
+            pixels=int(work[0]), visible_blocks=int(work[1]), block_visits=int(work[2]), sessions=int(work[4]),
+            allocated_blocks=int(work[5]))

@@ -108,6 +108,7 @@ int dms_scene_init(dms_scene *s, int kind, uint64_t seed) {
       double r = rng_range(&g, 0.2, 0.5);
       double c[3];
       for (int k = 0; k < 3; ++k) c[k] = rng_range(&g, lo[k] + r, hi[k] - r);
+      c[1] = rng_range(&g, hi[1] - r - 0.5, hi[1] - r); /* resting near the floor (+y is down) */
       double dc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
       if (dc < r + s->cam_radius + 0.45) continue; /* keep the camera ball clear */
       int ok = 1;
@@ -288,36 +289,64 @@ static void matmul3(const double A[9], const double B[9], double C[9]) {
     for (int c = 0; c < 3; ++c) C[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
 }
 
+/* camera-to-world rotation (columns = camera x right, y down, z forward)
+ * looking along f with the camera y axis close to world +y (down) */
+static void look_rotation(const double f_in[3], double R[9]) {
+  double fn = sqrt(f_in[0] * f_in[0] + f_in[1] * f_in[1] + f_in[2] * f_in[2]);
+  double f[3] = {f_in[0] / fn, f_in[1] / fn, f_in[2] / fn};
+  double down[3] = {0.0, 1.0, 0.0};
+  double x[3] = {down[1] * f[2] - down[2] * f[1], down[2] * f[0] - down[0] * f[2], down[0] * f[1] - down[1] * f[0]};
+  double xn = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  for (int k = 0; k < 3; ++k) x[k] /= xn;
+  double y[3] = {f[1] * x[2] - f[2] * x[1], f[2] * x[0] - f[0] * x[2], f[0] * x[1] - f[1] * x[0]};
+  for (int r = 0; r < 3; ++r) { R[3 * r] = x[r]; R[3 * r + 1] = y[r]; R[3 * r + 2] = f[r]; }
+}
+
+/* axis-angle of the rotation A^T B (angle in [0, pi]) */
+static double rel_axis_angle(const double A[9], const double B[9], double w[3]) {
+  double M[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) M[3 * r + c] = A[r] * B[c] + A[3 + r] * B[3 + c] + A[6 + r] * B[6 + c];
+  double cs = (M[0] + M[4] + M[8] - 1.0) * 0.5;
+  if (cs > 1.0) cs = 1.0;
+  if (cs < -1.0) cs = -1.0;
+  double th = acos(cs);
+  double v[3] = {M[7] - M[5], M[2] - M[6], M[3] - M[1]};
+  double vn = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  for (int k = 0; k < 3; ++k) w[k] = vn > 1e-12 ? v[k] / vn * th : 0.0;
+  return th;
+}
+
 /*
  * Smooth seeded random walk: per-frame translation <= 2 cm and rotation
- * <= 1 degree, staying within cam_radius of cam_center.  poses: nframes * 16
- * row-major camera-to-world.
+ * <= 1 degree, staying within cam_radius of cam_center.  The camera looks at
+ * the scene's objects: its heading follows a look-at target (the centroid of
+ * the spheres, swept +-25 degrees in yaw), approached at most 1 degree per
+ * frame including a small random jitter, so that the view keeps enough
+ * geometry to constrain tracking.  poses: nframes * 16 row-major
+ * camera-to-world.
  */
 void dms_trajectory(const dms_scene *s, int nframes, uint64_t seed, double *poses) {
   dms_rng g = {seed ^ 0x7A3EC7ULL, 0};
-  const double max_step = 0.02, max_rot = 1.0 * 3.141592653589793 / 180.0;
+  const double PI = 3.141592653589793;
+  const double max_step = 0.02, max_rot = 1.0 * PI / 180.0;
   double p[3] = {s->cam_center[0], s->cam_center[1], s->cam_center[2]};
-  double yaw0 = rng_range(&g, -3.14159, 3.14159);
-  /* look about 23 degrees down (world +y is down) so that the floor and the
-     floor-wall edges stay in view and constrain the tracking */
-  double R[9], Ry[9], Rx[9];
-  double wy[3] = {0.0, yaw0, 0.0}, wx[3] = {-0.4, 0.0, 0.0};
-  rodrigues(wy, Ry);
-  rodrigues(wx, Rx);
-  matmul3(Ry, Rx, R);
-  double v[3] = {0, 0, 0}, om[3] = {0, 0, 0};
+  double tgt[3] = {0, 0, 0};
+  int ns = 0;
+  for (int i = 0; i < s->nprims; ++i)
+    if (s->prims[i].kind == PRIM_SPHERE) { for (int k = 0; k < 3; ++k) tgt[k] += s->prims[i].c[k]; ++ns; }
+  if (ns) for (int k = 0; k < 3; ++k) tgt[k] /= ns;
+  const double phase = rng_range(&g, 0.0, 2.0 * PI);
+  double v[3] = {0, 0, 0};
+  double R[9];
   for (int f = 0; f < nframes; ++f) {
     if (f > 0) {
       for (int k = 0; k < 3; ++k) {
         double pull = (s->cam_center[k] - p[k]) * 0.02;
         v[k] = 0.9 * v[k] + 0.003 * rng_normal(&g) + pull;
-        om[k] = 0.9 * om[k] + 0.0025 * rng_normal(&g);
       }
-      om[1] += 0.001; /* slow pan so the camera sweeps the scene */
       double vn = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
       if (vn > max_step) for (int k = 0; k < 3; ++k) v[k] *= max_step / vn;
-      double on = sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
-      if (on > max_rot) for (int k = 0; k < 3; ++k) om[k] *= max_rot / on;
       for (int k = 0; k < 3; ++k) p[k] += v[k];
       double dx = p[0] - s->cam_center[0], dy = p[1] - s->cam_center[1], dz = p[2] - s->cam_center[2];
       double dn = sqrt(dx * dx + dy * dy + dz * dz);
@@ -326,12 +355,32 @@ void dms_trajectory(const dms_scene *s, int nframes, uint64_t seed, double *pose
         p[1] = s->cam_center[1] + dy * s->cam_radius / dn;
         p[2] = s->cam_center[2] + dz * s->cam_radius / dn;
       }
-      /* keep the camera roughly upright: damp roll (camera z) and pitch */
+    }
+    /* desired heading: towards the target, yaw swept, pitched 15..35 deg down */
+    double d[3] = {tgt[0] - p[0], tgt[1] - p[1], tgt[2] - p[2]};
+    double yaw = atan2(d[0], d[2]) + 25.0 * PI / 180.0 * sin(2.0 * PI * f / 150.0 + phase);
+    double hor = sqrt(d[0] * d[0] + d[2] * d[2]);
+    double pitch = atan2(d[1], hor);
+    if (pitch < 15.0 * PI / 180.0) pitch = 15.0 * PI / 180.0;
+    if (pitch > 35.0 * PI / 180.0) pitch = 35.0 * PI / 180.0;
+    double fw[3] = {sin(yaw) * cos(pitch), sin(pitch), cos(yaw) * cos(pitch)};
+    double Rd[9];
+    look_rotation(fw, Rd);
+    if (f == 0) {
+      memcpy(R, Rd, sizeof R);
+    } else {
+      /* rotate towards Rd by at most 0.8 deg, plus up to 0.2 deg of jitter */
+      double w[3];
+      double th = rel_axis_angle(R, Rd, w);
+      double lim = 0.8 * PI / 180.0;
+      if (th > lim) for (int k = 0; k < 3; ++k) w[k] *= lim / th;
+      double jit[3] = {rng_normal(&g), rng_normal(&g), rng_normal(&g)};
+      double jn = sqrt(jit[0] * jit[0] + jit[1] * jit[1] + jit[2] * jit[2]);
+      for (int k = 0; k < 3; ++k) w[k] += jit[k] / (jn > 0 ? jn : 1.0) * (max_rot - lim) * rng_u(&g);
       double dR[9], Rn[9];
-      rodrigues(om, dR);
-      matmul3(dR, R, Rn);
+      rodrigues(w, dR);  /* w is in the camera frame: R <- R dR */
+      matmul3(R, dR, Rn);
       memcpy(R, Rn, sizeof R);
-      om[0] *= 0.8; om[2] *= 0.8;
     }
     double *P = poses + 16 * (size_t)f;
     P[0] = R[0]; P[1] = R[1]; P[2] = R[2];  P[3] = p[0];

@@ -1,0 +1,45 @@
+// Drives the reference-shaped C++ classes (tsdfVolume, builder) through the
+// C ABI, as the reference's own test programs do (src/test/testDynaMap.cpp):
+// a synthetic wall is integrated, raycast and tracked.  Exit code 0 = pass.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "dynamap/dynamap.hpp"
+
+int main() {
+  using namespace DynaMap;
+  rgbdSensor s{120, 160, 150.f, 150.f, 79.5f, 59.5f, 1.f};
+  tsdf::tsdfProperties p{0.01f, 1 << 14, 10, 1 << 14, 8, 64, 0.03f, 5.0f, 0.1f};
+  std::vector<float> depth(120 * 160, 1.0f);
+  rgbdImage img{depth.data(), nullptr, s};
+  tsdf::tsdfVolume vol;
+  vol.init(p, 120, 160);
+  float4x4 I;
+  vol.IntegrateScan(img, I);
+  const int nb = vol.GetNumAllocatedBlocks();
+  std::vector<float> d = vol.GenerateDepth(I, s);
+  const float centre = d[60 * 160 + 80];
+  std::printf("blocks %d centre depth %.5f\n", nb, centre);
+  if (nb <= 0 || std::fabs(centre - 1.0f) > 2e-3f) return 1;
+  // builder: two identical frames -> identity motion
+  builder b;
+  solver::Properties op;
+  op.minDepth = 0.1f;
+  b.init(p, op, s);
+  b.buildModelLS(img);
+  auto T = b.buildModelLS(img, true);
+  std::printf("pose t = %.3g %.3g %.3g\n", T[3], T[7], T[11]);
+  if (std::fabs(T[3]) > 1e-4 || std::fabs(T[7]) > 1e-4 || std::fabs(T[11]) > 1e-4) return 2;
+  try {
+    tsdf::tsdfProperties bad = p;
+    bad.block_size = 4;
+    tsdf::tsdfVolume v2;
+    v2.init(bad);
+    return 3;
+  } catch (const Error &e) {
+    if (e.code != DM_E_INVALID) return 4;
+  }
+  std::printf("ok\n");
+  return 0;
+}
