@@ -37,13 +37,16 @@ int validate_props(const dm_tsdf_props &p) {
 }
 
 static int bucket_stride(int S) { return (int)round_up((size_t)(12 * S + 4), 128); }
-static int new_cap(int num_blocks) { return std::max(1024, std::min(num_blocks, 1 << 22)); }
+// Capacity of the per-frame new-key list.  A frame with more new keys than
+// this cannot fit the pool either (it holds at least num_blocks), so it is
+// rejected as a whole; only its drop counts are then lower bounds.
+static int new_cap(int num_blocks) { return std::max(4096, std::min(num_blocks, 1 << 22)); }
 
 size_t session_bytes(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   const size_t P = (size_t)rows * cols;
   const size_t nc = (size_t)new_cap(p.num_blocks);
   size_t b = (size_t)p.num_buckets * bucket_stride(p.bucket_size);
-  b += (size_t)p.num_blocks * (kBlockVox * 8 + 16 + 1);
+  b += (size_t)p.num_blocks * (kBlockVox * 8 + 16 + 2);
   b += (size_t)pow2_at_least((int)(2 * nc)) * 8 + nc * (8 + 4 * 4) + (size_t)p.num_buckets * 4;
   size_t maps = 0;
   for (int l = 0; l < kLevels; ++l) maps += (size_t)(rows >> l) * (cols >> l) * 16 * 4;
@@ -98,6 +101,7 @@ int Sess::init(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   DM_TRY(dalloc(&h.vox, (size_t)p.num_blocks * kBlockVox));
   DM_TRY(dalloc(&h.block_key, (size_t)p.num_blocks));
   DM_TRY(dalloc(&h.blk_near, (size_t)p.num_blocks));
+  DM_TRY(dalloc(&h.blk_ndil, (size_t)p.num_blocks));
   DM_TRY(dalloc(&h.dset, (size_t)dsz));
   DM_TRY(dalloc(&h.newkeys, (size_t)h.newcap, false));
   DM_TRY(dalloc(&h.newdslot, (size_t)h.newcap, false));

@@ -60,6 +60,7 @@ struct DmView {
   uint64_t *vox;        // num_blocks * 512 voxel words (sdf bits | r | g<<8 | b<<16 | w<<24 <<32)
   int4 *block_key;      // num_blocks block coordinates
   uint8_t *blk_near;    // num_blocks: 1 when an observed voxel lies below the raycast far threshold
+  uint8_t *blk_ndil;    // num_blocks: blk_near dilated over the 26 neighbouring blocks
   uint64_t *dset;       // per-frame new-key set, dset_mask + 1 slots, 0 = empty
   uint64_t *newkeys;    // newcap
   int32_t *newdslot;    // newcap

@@ -25,33 +25,58 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 }
 
 // Record a block that is missing from the table in the per-frame key set.
+// A plain read first: once a key is in the set, later visitors return
+// without an atomic (a missing free-space block near the camera is visited
+// by thousands of rays).
 __device__ __forceinline__ void note_new_key(const DmView &v, uint64_t key) {
   uint32_t h = (uint32_t)mix64(key) & (uint32_t)v.dset_mask;
   for (int probe = 0; probe <= v.dset_mask; ++probe) {
-    uint64_t prev = atomicCAS((unsigned long long *)&v.dset[h], 0ull, (unsigned long long)key);
-    if (prev == 0ull) {
-      int idx = atomicAdd(&v.ctr[C_N_NEW], 1);
-      if (idx < v.newcap) {
-        v.newkeys[idx] = key;
-        v.newdslot[idx] = (int32_t)h;
-      } else {
-        // no room in the frame's key list: release the slot so that a later
-        // frame can insert the key
-        atomicExch((unsigned long long *)&v.dset[h], 0ull);
-        atomicAdd(&v.ctr[C_DSET_OVF], 1);
+    uint64_t cur = __atomic_load_n((unsigned long long *)&v.dset[h], __ATOMIC_RELAXED);
+    if (cur == key) return;
+    if (cur == 0ull) {
+      const uint64_t prev = atomicCAS((unsigned long long *)&v.dset[h], 0ull, (unsigned long long)key);
+      if (prev == 0ull) {
+        int idx = atomicAdd(&v.ctr[C_N_NEW], 1);
+        if (idx < v.newcap) {
+          v.newkeys[idx] = key;
+          v.newdslot[idx] = (int32_t)h;
+        } else {
+          // no room in the frame's key list: release the slot so that a
+          // later frame can insert the key
+          atomicExch((unsigned long long *)&v.dset[h], 0ull);
+          atomicAdd(&v.ctr[C_DSET_OVF], 1);
+        }
+        return;
       }
-      return;
+      if (prev == key) return;
     }
-    if (prev == key) return;
     h = (h + 1u) & (uint32_t)v.dset_mask;
   }
   atomicAdd(&v.ctr[C_DSET_OVF], 1);
 }
 
-__device__ __forceinline__ void visit_block(const DmView &v, int x, int y, int z) {
+// Per-workgroup set of the blocks its rays have already handled (looked up,
+// and noted when missing): neighbouring rays visit mostly the same blocks,
+// so each block is looked up in the global table about once per workgroup.
+constexpr int kSeenSlots = 1024;
+constexpr int kSeenProbes = 8;
+
+__device__ __forceinline__ void visit_block(const DmView &v, int x, int y, int z, unsigned long long *seen) {
   if (!in_key_range(x, y, z)) { atomicAdd(&v.ctr[C_DROP_RANGE], 1); return; }
+  const uint64_t key = pack_key(x, y, z);
+  uint32_t h = (uint32_t)mix64(key) & (kSeenSlots - 1);
+  for (int probe = 0; probe < kSeenProbes; ++probe) {
+    const unsigned long long cur = seen[h];
+    if (cur == key) return;
+    if (cur == 0ull) {
+      const unsigned long long prev = atomicCAS(&seen[h], 0ull, (unsigned long long)key);
+      if (prev == key) return;
+      if (prev == 0ull) break;  // this thread handles the block for the workgroup
+    }
+    h = (h + 1u) & (kSeenSlots - 1);
+  }
   if (find_block(v, x, y, z) >= 0) return;
-  note_new_key(v, pack_key(x, y, z));
+  note_new_key(v, key);
 }
 
 // AllocateFromDepthKernel (tsdf.cu:357-452), one thread per pixel, 16x16
@@ -59,7 +84,7 @@ __device__ __forceinline__ void visit_block(const DmView &v, int x, int y, int z
 // same workgroup.  K: ray from the camera centre to surface + trunc, the
 // (b + step) * bs boundary for both directions and the extra "neg_ray" block.
 // D: the traversal is bounded by the L1 distance to the end block.
-__device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r) {
+__device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r, unsigned long long *seen) {
   if (u >= v.cols || r >= v.rows) return 0;
   const int i = r * v.cols + u;
   const float d = v.depth[i];
@@ -92,11 +117,11 @@ __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r) {
   if (bx != ex && dir.x < 0) { dx--; neg = true; }
   if (by != ey && dir.y < 0) { dy--; neg = true; }
   if (bz != ez && dir.z < 0) { dz--; neg = true; }
-  visit_block(v, bx, by, bz);
+  visit_block(v, bx, by, bz, seen);
   int nvisit = 1;
   if (neg) {
     bx += dx; by += dy; bz += dz;
-    visit_block(v, bx, by, bz);
+    visit_block(v, bx, by, bz, seen);
     ++nvisit;
   }
   const long long bound = llabs((long long)ex - bx) + llabs((long long)ey - by) + llabs((long long)ez - bz);
@@ -106,7 +131,7 @@ __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r) {
     } else {
       if (mty < mtz) { by += sty; mty += dty; } else { bz += stz; mtz += dtz; }
     }
-    visit_block(v, bx, by, bz);
+    visit_block(v, bx, by, bz, seen);
     ++nvisit;
   }
   return nvisit;
@@ -121,11 +146,13 @@ __global__ __launch_bounds__(256) void k_alloc_rays(const DmView *__restrict__ v
   const int tile = blockIdx.x;
   if (tile >= tiles_x * ((v.rows + 15) >> 4)) return;
   __shared__ int visits;
+  __shared__ unsigned long long seen[kSeenSlots];
+  for (int k = threadIdx.x; k < kSeenSlots; k += 256) seen[k] = 0ull;
   if (threadIdx.x == 0) visits = 0;
   __syncthreads();
   const int u = (tile % tiles_x) * 16 + (threadIdx.x & 15);
   const int r = (tile / tiles_x) * 16 + (threadIdx.x >> 4);
-  const int n = alloc_ray(v, u, r);
+  const int n = alloc_ray(v, u, r, seen);
   if (n) atomicAdd(&visits, n);
   __syncthreads();
   if (threadIdx.x == 0 && visits) atomicAdd(&v.ctr[C_NVISIT], visits);
@@ -310,6 +337,7 @@ __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ vi
   const float far_thr = far_threshold(tau);
   int nvis = 0;
   for (int blk = blockIdx.x; blk < nalloc; blk += gridDim.x) {
+    if (threadIdx.x == 0) v.blk_ndil[blk] = 0;  // rebuilt by k_dilate_near
     const int4 key = v.block_key[blk];
     const f3 pos = mk((float)key.x * vs * 8.0f, (float)key.y * vs * 8.0f, (float)key.z * vs * 8.0f);
     const f3 pc = xform(Ti, pos);
@@ -334,6 +362,27 @@ __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ vi
     if (threadIdx.x == 0) v.blk_near[blk] = near ? 1 : 0;
   }
   if (threadIdx.x == 0 && nvis) atomicAdd(&v.ctr[C_NVIS], nvis);
+}
+
+// Dilate the near flags over the 26 neighbouring blocks: blk_ndil[b] == 0
+// means b and every allocated neighbour have no observed voxel below the far
+// threshold, so every voxel within one block of b is far.  Runs after the
+// integration (which clears blk_ndil of every allocated block).
+__global__ __launch_bounds__(256) void k_dilate_near(const DmView *__restrict__ views, const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  const int nalloc = v.ctr[C_NUM_ALLOC];
+  for (int blk = blockIdx.x * 256 + threadIdx.x; blk < nalloc; blk += gridDim.x * 256) {
+    if (!v.blk_near[blk]) continue;
+    const int4 key = v.block_key[blk];
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int ptr = (dx | dy | dz) == 0 ? blk : find_block(v, key.x + dx, key.y + dy, key.z + dz);
+          if (ptr >= 0) v.blk_ndil[ptr] = 1;
+        }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -375,19 +424,44 @@ __device__ __forceinline__ bool block_far(const DmView &v, int bx, int by, int b
   return f;
 }
 
-__device__ __forceinline__ int vrange_block(float pv, float m) { return floordiv8((int)floorf(pv + m)); }
-
-__device__ __forceinline__ bool sample_is_far(const DmView &v, f3 pa, float inv_vs, FarCache &c) {
-  const float m = 1.05f;
-  const float px = pa.x * inv_vs, py = pa.y * inv_vs, pz = pa.z * inv_vs;
+// Conservative empty-space run.  pv = (origin + z * direction) / vs is the
+// approximate sample position in voxel units (error about 1e-3 voxel); the
+// voxels GetInterpolatedVoxel reads at the exact position lie in
+// [floor(pv - e) - 1, floor(pv + 1 + e)] per axis (the exact indices round
+// p/vs -+ 0.5 half away from zero).  When the block holding the (approximate)
+// centre voxel is allocated and its dilated flag is clear, all those voxels
+// are far; the run continues while they stay within that block's 27
+// neighbours: returns true and zmax, the last z of the run.  An unallocated
+// centre block falls back to checking the blocks of the voxel range.
+__device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, float z, FarCache &c, float &zmax) {
+  const float e = 0.05f;
+  const float px = po.x + pd.x * z, py = po.y + pd.y * z, pz = po.z + pd.z * z;
   if (!(fabsf(px) < 1.0e6f && fabsf(py) < 1.0e6f && fabsf(pz) < 1.0e6f)) return false;
-  const int x0 = vrange_block(px, -m), x1 = vrange_block(px, m + 1.0f);
-  const int y0 = vrange_block(py, -m), y1 = vrange_block(py, m + 1.0f);
-  const int z0 = vrange_block(pz, -m), z1 = vrange_block(pz, m + 1.0f);
-  for (int bx = x0; bx <= x1; ++bx)
-    for (int by = y0; by <= y1; ++by)
-      for (int bz = z0; bz <= z1; ++bz)
-        if (!block_far(v, bx, by, bz, c)) return false;
+  const int bx = floordiv8((int)floorf(px + 0.5f)), by = floordiv8((int)floorf(py + 0.5f)),
+            bz = floordiv8((int)floorf(pz + 0.5f));
+  int x0, x1, y0, y1, z0, z1;
+  const int ptr = find_block(v, bx, by, bz);
+  if (ptr >= 0) {
+    if (v.blk_ndil[ptr]) return false;
+    x0 = bx - 1; x1 = bx + 1; y0 = by - 1; y1 = by + 1; z0 = bz - 1; z1 = bz + 1;
+  } else {
+    x0 = floordiv8((int)floorf(px - e) - 1); x1 = floordiv8((int)floorf(px + 1.0f + e));
+    y0 = floordiv8((int)floorf(py - e) - 1); y1 = floordiv8((int)floorf(py + 1.0f + e));
+    z0 = floordiv8((int)floorf(pz - e) - 1); z1 = floordiv8((int)floorf(pz + 1.0f + e));
+    for (int qx = x0; qx <= x1; ++qx)
+      for (int qy = y0; qy <= y1; ++qy)
+        for (int qz = z0; qz <= z1; ++qz)
+          if (!block_far(v, qx, qy, qz, c)) return false;
+  }
+  float zm = 3.0e38f;
+  const float m0 = 1.0f + 2.0f * e, m1 = 7.0f - 2.0f * e;
+  if (pd.x > 0.f) zm = fminf(zm, ((float)(8 * x1) + m1 - po.x) / pd.x);
+  else if (pd.x < 0.f) zm = fminf(zm, ((float)(8 * x0) + m0 - po.x) / pd.x);
+  if (pd.y > 0.f) zm = fminf(zm, ((float)(8 * y1) + m1 - po.y) / pd.y);
+  else if (pd.y < 0.f) zm = fminf(zm, ((float)(8 * y0) + m0 - po.y) / pd.y);
+  if (pd.z > 0.f) zm = fminf(zm, ((float)(8 * z1) + m1 - po.z) / pd.z);
+  else if (pd.z < 0.f) zm = fminf(zm, ((float)(8 * z0) + m0 - po.z) / pd.z);
+  zmax = zm;
   return true;
 }
 
@@ -413,19 +487,25 @@ __global__ __launch_bounds__(256) void k_raycast(const DmView *__restrict__ view
   FarCache fc;
   fc.init();
   const float inv_vs = 1.0f / vs;
-  // ray for the approximate empty-space test: origin + z * direction
-  const f3 org = mk(T.m[3], T.m[7], T.m[11]);
-  const f3 dir = rot(T, mk(((float)u - s.cx) / s.fx, ((float)r - s.cy) / s.fy, 1.0f));
+  // ray in voxel units for the approximate empty-space test
+  const f3 po = mk(T.m[3], T.m[7], T.m[11]) * inv_vs;
+  const f3 pd = rot(T, mk(((float)u - s.cx) / s.fx, ((float)r - s.cy) / s.fy, 1.0f)) * inv_vs;
   float z = v.min_depth, pz = 0.f, ps = 0.f, zh = 0.f, zprev = 0.f;
   bool pobs = false, hit = false, prev_fast = false;
   for (int it = 0; it < max_steps && z < v.max_depth; ++it) {
-    float sdf;
-    // far sample: the step is 0.5*trunc whether it is observed or not; its
-    // interpolation is recomputed below only if the next sample needs it
-    if (sample_is_far(v, org + dir * z, inv_vs, fc)) {
+    float sdf, zmax;
+    // far samples: the step is 0.5*trunc whether they are observed or not;
+    // the run of far samples up to zmax is stepped over without lookups (the
+    // last one's interpolation is recomputed below if the next sample needs it)
+    if (far_run(v, po, pd, z, fc, zmax)) {
       prev_fast = true;
       zprev = z;
       z = z + big;
+      while (z <= zmax && it + 1 < max_steps && z < v.max_depth) {
+        ++it;
+        zprev = z;
+        z = z + big;
+      }
       continue;
     }
     const f3 p = xform(T, get_point3d(i, z, s));
@@ -515,6 +595,7 @@ void launch_alloc_rays(const Launch &L) {
 void launch_integrate(const Launch &L) {
   if (L.nact <= 0) return;
   hipLaunchKernelGGL(k_integrate, dim3(2048, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
+  hipLaunchKernelGGL(k_dilate_near, dim3(256, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
 }
 
 void launch_raycast(const Launch &L) {
