@@ -80,8 +80,11 @@ __device__ __forceinline__ void project(f3 p, const Intr &s, int &u, int &v) {
 
 // include/dataStructure/hashTable/hashTable.cu:152-163 (bucket index, i.e.
 // the reference's value divided by bucket_size); int32 products wrap.
+// For a power-of-two bucket count the C remainder plus the negative fix-up
+// equals the low bits of the two's-complement hash, so no division is needed.
 __device__ __forceinline__ int hash_bucket(int x, int y, int z, int num_buckets) {
   uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349669u) ^ ((uint32_t)z * 83492791u);
+  if ((num_buckets & (num_buckets - 1)) == 0) return (int)(h & (uint32_t)(num_buckets - 1));
   int r = (int)h % num_buckets;
   if (r < 0) r += num_buckets;
   return r;

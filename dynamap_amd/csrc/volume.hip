@@ -433,7 +433,8 @@ __device__ __forceinline__ bool block_far(const DmView &v, int bx, int by, int b
 // are far; the run continues while they stay within that block's 27
 // neighbours: returns true and zmax, the last z of the run.  An unallocated
 // centre block falls back to checking the blocks of the voxel range.
-__device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, float z, FarCache &c, float &zmax) {
+__device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, float z, FarCache &c,
+                                        float &zmax) {
   const float e = 0.05f;
   const float px = po.x + pd.x * z, py = po.y + pd.y * z, pz = po.z + pd.z * z;
   if (!(fabsf(px) < 1.0e6f && fabsf(py) < 1.0e6f && fabsf(pz) < 1.0e6f)) return false;
@@ -441,8 +442,7 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, float z, 
             bz = floordiv8((int)floorf(pz + 0.5f));
   int x0, x1, y0, y1, z0, z1;
   const int ptr = find_block(v, bx, by, bz);
-  if (ptr >= 0) {
-    if (v.blk_ndil[ptr]) return false;
+  if (ptr >= 0 && v.blk_ndil[ptr] == 0) {
     x0 = bx - 1; x1 = bx + 1; y0 = by - 1; y1 = by + 1; z0 = bz - 1; z1 = bz + 1;
   } else {
     x0 = floordiv8((int)floorf(px - e) - 1); x1 = floordiv8((int)floorf(px + 1.0f + e));
@@ -455,12 +455,13 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, float z, 
   }
   float zm = 3.0e38f;
   const float m0 = 1.0f + 2.0f * e, m1 = 7.0f - 2.0f * e;
-  if (pd.x > 0.f) zm = fminf(zm, ((float)(8 * x1) + m1 - po.x) / pd.x);
-  else if (pd.x < 0.f) zm = fminf(zm, ((float)(8 * x0) + m0 - po.x) / pd.x);
-  if (pd.y > 0.f) zm = fminf(zm, ((float)(8 * y1) + m1 - po.y) / pd.y);
-  else if (pd.y < 0.f) zm = fminf(zm, ((float)(8 * y0) + m0 - po.y) / pd.y);
-  if (pd.z > 0.f) zm = fminf(zm, ((float)(8 * z1) + m1 - po.z) / pd.z);
-  else if (pd.z < 0.f) zm = fminf(zm, ((float)(8 * z0) + m0 - po.z) / pd.z);
+  // reciprocal direction: zmax only needs to be accurate to well within e
+  if (pd.x > 0.f) zm = fminf(zm, ((float)(8 * x1) + m1 - po.x) * ipd.x);
+  else if (pd.x < 0.f) zm = fminf(zm, ((float)(8 * x0) + m0 - po.x) * ipd.x);
+  if (pd.y > 0.f) zm = fminf(zm, ((float)(8 * y1) + m1 - po.y) * ipd.y);
+  else if (pd.y < 0.f) zm = fminf(zm, ((float)(8 * y0) + m0 - po.y) * ipd.y);
+  if (pd.z > 0.f) zm = fminf(zm, ((float)(8 * z1) + m1 - po.z) * ipd.z);
+  else if (pd.z < 0.f) zm = fminf(zm, ((float)(8 * z0) + m0 - po.z) * ipd.z);
   zmax = zm;
   return true;
 }
@@ -490,6 +491,7 @@ __global__ __launch_bounds__(256) void k_raycast(const DmView *__restrict__ view
   // ray in voxel units for the approximate empty-space test
   const f3 po = mk(T.m[3], T.m[7], T.m[11]) * inv_vs;
   const f3 pd = rot(T, mk(((float)u - s.cx) / s.fx, ((float)r - s.cy) / s.fy, 1.0f)) * inv_vs;
+  const f3 ipd = mk(pd.x != 0.f ? 1.0f / pd.x : 0.f, pd.y != 0.f ? 1.0f / pd.y : 0.f, pd.z != 0.f ? 1.0f / pd.z : 0.f);
   float z = v.min_depth, pz = 0.f, ps = 0.f, zh = 0.f, zprev = 0.f;
   bool pobs = false, hit = false, prev_fast = false;
   for (int it = 0; it < max_steps && z < v.max_depth; ++it) {
@@ -497,7 +499,7 @@ __global__ __launch_bounds__(256) void k_raycast(const DmView *__restrict__ view
     // far samples: the step is 0.5*trunc whether they are observed or not;
     // the run of far samples up to zmax is stepped over without lookups (the
     // last one's interpolation is recomputed below if the next sample needs it)
-    if (far_run(v, po, pd, z, fc, zmax)) {
+    if (far_run(v, po, pd, ipd, z, fc, zmax)) {
       prev_fast = true;
       zprev = z;
       z = z + big;
