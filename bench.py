@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-frame", action="store_true")
     ap.add_argument("--no-poisson", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, fill then copy a buffer of this size (known HBM bytes for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -203,6 +206,15 @@ def main():
         "config": {"workload": "serve-16: 8 x 320x240 @ 2 cm + 8 x 640x480 @ 8 mm", "sessions": ns,
                    "frames_per_session": nf, "icp_iterations": [10, 5, 4], "batch": "one frame per session"},
     }
+    if args.copy_calibration_mib:
+        import torch
+        nb = args.copy_calibration_mib << 20
+        a = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(7)       # writes nb bytes
+        b.copy_(a)       # reads nb bytes, writes nb bytes
+        torch.cuda.synchronize()
+        log(f"copy calibration: fill + copy of {nb} bytes")
     res.update(out)
     res["roofline"] = roofline
     res["cpu_baseline"] = cpu

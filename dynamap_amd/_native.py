@@ -50,6 +50,16 @@ _vp = C.c_void_p
 _P = C.POINTER
 
 # name -> (restype, argtypes); every function declared in include/*.h
+class dm_graph_props(C.Structure):
+    _fields_ = [("node_radius", C.c_float), ("dgw", C.c_float), ("max_nodes", C.c_int32)]
+
+
+class dm_nonrigid_props(C.Structure):
+    _fields_ = [("gn_iterations", C.c_int32), ("pcg_iterations", C.c_int32), ("pcg_tol", C.c_double),
+                ("lambda_reg", C.c_double), ("damping", C.c_double), ("dist_max", C.c_double),
+                ("normal_cos", C.c_double), ("min_depth", C.c_double), ("max_depth", C.c_double)]
+
+
 PROTOTYPES = {
     "dm_last_error": (C.c_char_p, []),
     "dm_device_count": (C.c_int, [_P(C.c_int32)]),
@@ -63,6 +73,18 @@ PROTOTYPES = {
     "dm_volume_get_entries": (C.c_int, [_vp, _vp]),
     "dm_volume_get_blocks": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp, _P(C.c_int32)]),
     "dm_volume_counters": (C.c_int, [_vp, _vp]),
+    "dm_mc_tri_table": (C.c_int, [_vp]),
+    "dm_graph_create": (C.c_int, [_vp, _vp, C.c_int64, _P(dm_graph_props), _P(_vp)]),
+    "dm_graph_destroy": (C.c_int, [_vp]),
+    "dm_graph_num_nodes": (C.c_int, [_vp, _P(C.c_int32)]),
+    "dm_graph_get_nodes": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "dm_graph_get_vertex_nn": (C.c_int, [_vp, _vp, _vp]),
+    "dm_graph_num_blocks": (C.c_int, [_vp, _P(C.c_int32)]),
+    "dm_graph_set_transforms": (C.c_int, [_vp, _vp]),
+    "dm_graph_get_transforms": (C.c_int, [_vp, _vp]),
+    "dm_blend": (C.c_int, [_vp, _vp, _vp]),
+    "dm_nonrigid_solve": (C.c_int, [_vp, _vp, _P(dm_sensor), _vp, _P(dm_nonrigid_props), _vp]),
+    "dm_mesh_extract": (C.c_int, [_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _P(C.c_int64)]),
     "dm_session_create": (C.c_int, [_P(dm_tsdf_props), _P(dm_icp_props), _P(dm_sensor), _vp, _P(_vp)]),
     "dm_session_destroy": (C.c_int, [_vp]),
     "dm_session_volume": (_vp, [_vp]),
@@ -124,6 +146,15 @@ def synth():
                                  C.c_uint64, C.c_int, _vp, _vp]
         S.dms_trajectory.restype = None
         S.dms_trajectory.argtypes = [_vp, C.c_int, C.c_uint64, _vp]
+        S.dms_cylinder_warp.restype = None
+        S.dms_cylinder_warp.argtypes = [C.c_double, C.c_int64, _vp, _vp]
+        S.dms_cylinder_mesh.restype = None
+        S.dms_cylinder_mesh.argtypes = [C.c_int, C.c_int, _vp, _vp]
+        S.dms_cylinder_render.restype = None
+        S.dms_cylinder_render.argtypes = [C.c_double, _vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_uint64, C.c_int, _vp]
+        S.dms_cylinder_trajectory.restype = None
+        S.dms_cylinder_trajectory.argtypes = [C.c_int, C.c_uint64, _vp]
         _synth = S
     return _synth
 

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "_lib")
 CSRC = os.path.join(HERE, "csrc")
 
-HIP_SOURCES = ["volume.hip", "track.hip", "mesh.hip", "nonrigid.hip", "api.hip", "server.hip"]
+HIP_SOURCES = ["volume.hip", "track.hip", "scan.hip", "mesh.hip", "nonrigid.hip", "api.hip", "server.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",

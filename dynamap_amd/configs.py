@@ -40,6 +40,7 @@ def sensor(d):
 
 def tsdf_props(d):
     p = N.dm_tsdf_props()
+    p.block_size = 8  # tsdfProperties::blockSize; 8 is the only supported edge
     for k, v in d.items():
         setattr(p, k, v)
     return p

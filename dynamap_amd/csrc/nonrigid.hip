@@ -1,0 +1,1158 @@
+// Deformation graph, dual-quaternion blending and the non-rigid
+// Gauss-Newton step.
+//
+// Reference (paths relative to the reference root):
+//   include/geometry/defGraph/defGraph.cu      defGraph::init, defGraphUpdateNodes,
+//                                              updateNodeDistnacesKernel, sortNeighbourNodesKernel,
+//                                              updateNodeWeightsKernel, updateNodesDQ
+//   include/blender/dqBlender/dqBlender.cu     blendVertex :162-186, blendMesh :187-218
+//   include/math/dualQuat.h, Quaternion.h      transformPosition :293-302, rotate :278-282
+//   include/solver/nonRigidICP/nonRigidICP.cu  solve :557-704, getRegResiduals :494-515
+//
+// Layout (one dm_graph): canonical vertices / normals (float3), per-vertex
+// 4 nearest nodes (int32) with raw and normalised weights, per-node vertex
+// id, 4 nearest nodes and weights, node transforms T_i (FP64 4x4) and their
+// unit dual quaternions (FP32 x 8).  The normal equations are block-sparse:
+// the 6x6 blocks (a, b >= a) that some data residual or ARAP edge couples,
+// sorted by (a, b), each with the list of its contributions (vertex and
+// neighbour slots, or ARAP edge and role) in a stable order fixed when the
+// graph is built, so every sum has a fixed order and the solve is
+// deterministic.  A row index (block, transposed?) sorted by column serves
+// the symmetric SpMV.
+//
+// Per Gauss-Newton iteration (all on the graph's stream, no host round
+// trip): blend -> per-vertex data linearisation (association in FP64) ->
+// per-edge ARAP terms -> block assembly (one wavefront per block, FP64
+// accumulation, FP32 storage; diagonal blocks also produce the rhs, the
+// damping and the FP64 inverse used as block-Jacobi preconditioner) ->
+// PCG with FP64 vectors and fixed-order FP64 dot products -> per-node
+// T_i <- exp(xi_i) T_i.  The formulation (residuals, Jacobians, gates)
+// is the one of the CPU restatement in oracle/dm_oracle_nr.c.
+#include <hipcub/hipcub.hpp>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+#include "../../include/dynamap_nonrigid.h"
+#include "dm_device.h"
+#include "dm_scan.h"
+#include "dm_solve.h"
+
+namespace dm {
+
+constexpr int kNK = 4;              // neighbours per vertex and per node
+constexpr int kMaxNodes = 1 << 15;  // node ids fit 15 bits of the block keys
+constexpr int kNodeBits = 15;
+constexpr uint32_t kArapTag = 0x80000000u;
+constexpr int kPcgThreads = 192;    // 32 nodes x 6 rows per workgroup
+
+struct dq8 { float rw, rx, ry, rz, dw, dx, dy, dz; };
+
+// ---------------------------------------------------------------- graph build
+__device__ __forceinline__ uint64_t cell_key(int cx, int cy, int cz) {
+  const uint64_t m = (1u << 21) - 1;
+  return (1ull << 63) | (((uint64_t)(uint32_t)(cx + (1 << 20)) & m) << 42) |
+         (((uint64_t)(uint32_t)(cy + (1 << 20)) & m) << 21) | ((uint64_t)(uint32_t)(cz + (1 << 20)) & m);
+}
+__device__ __forceinline__ uint32_t cell_hash(uint64_t k) {
+  k ^= k >> 33; k *= 0xff51afd7ed558ccdull; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ull; k ^= k >> 33;
+  return (uint32_t)k;
+}
+__device__ __forceinline__ void vertex_cell(const float *p, float r, int c[3], float &d2) {
+  d2 = 0.f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    c[k] = (int)floorf(p[k] / r);
+    const float e = p[k] - ((float)c[k] + 0.5f) * r;
+    d2 = d2 + e * e;
+  }
+}
+
+// per occupied cell: min over (d2 to the cell centre, vertex index)
+__global__ __launch_bounds__(256) void k_nr_cells(const float *__restrict__ verts, int64_t nv, float r,
+                                                 uint64_t *__restrict__ keys, unsigned long long *__restrict__ vals,
+                                                 uint32_t mask) {
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nv) return;
+  int c[3];
+  float d2;
+  vertex_cell(verts + 3 * v, r, c, d2);
+  const uint64_t key = cell_key(c[0], c[1], c[2]);
+  const unsigned long long val = ((unsigned long long)__float_as_uint(d2) << 32) | (unsigned long long)v;
+  uint32_t s = cell_hash(key) & mask;
+  for (uint32_t probe = 0; probe <= mask; ++probe) {  // the table has >= 2 nv slots: a free slot exists
+    const unsigned long long prev = atomicCAS((unsigned long long *)&keys[s], 0ull, (unsigned long long)key);
+    if (prev == 0ull || prev == key) {
+      atomicMin(&vals[s], val);
+      return;
+    }
+    s = (s + 1) & mask;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_nr_mark(const float *__restrict__ verts, int64_t nv, float r,
+                                                const uint64_t *__restrict__ keys,
+                                                const unsigned long long *__restrict__ vals, uint32_t mask,
+                                                int *__restrict__ isnode) {
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nv) return;
+  int c[3];
+  float d2;
+  vertex_cell(verts + 3 * v, r, c, d2);
+  const uint64_t key = cell_key(c[0], c[1], c[2]);
+  uint32_t s = cell_hash(key) & mask;
+  int flag = 0;
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
+    if (keys[s] == key) {
+      flag = (vals[s] & 0xffffffffull) == (unsigned long long)v;
+      break;
+    }
+    if (keys[s] == 0) break;
+    s = (s + 1) & mask;
+  }
+  isnode[v] = flag;
+}
+
+__global__ __launch_bounds__(256) void k_nr_nodes(const int *__restrict__ isnode, const long long *__restrict__ rank,
+                                                 int64_t nv, int *__restrict__ node_vid,
+                                                 int *__restrict__ node_of_vertex) {
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nv) return;
+  if (isnode[v]) {
+    node_vid[rank[v]] = (int)v;
+    node_of_vertex[v] = (int)rank[v];
+  } else {
+    node_of_vertex[v] = -1;
+  }
+}
+
+__device__ __forceinline__ float clamp_dist(float d2) {
+  const float d = sqrtf(d2);
+  return d < 1e-4f ? 1e-4f : d;
+}
+__device__ __forceinline__ float weight_of(float d, float dgw) {
+  return (float)exp(-(double)((d * d) / (dgw * dgw)));
+}
+
+// K nearest nodes of each query point (nodes streamed through LDS);
+// distance ascending, ties to the lower node index; `self_is_node` excludes
+// the query's own node (node queries).  Also the weights and, for vertex
+// queries, the weights normalised in FP64.
+constexpr int kKnnTile = 1024;
+__global__ __launch_bounds__(256) void k_nr_knn(const float *__restrict__ verts, const int *__restrict__ node_vid,
+                                               int nn, const int *__restrict__ query_vid, int64_t nq, float dgw,
+                                               int *__restrict__ ids, float *__restrict__ w,
+                                               float *__restrict__ wn) {
+  __shared__ float tile[kKnnTile * 3];
+  const int64_t qi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool active = qi < nq;
+  const int64_t qv = active ? (query_vid ? (int64_t)query_vid[qi] : qi) : 0;
+  const int self = query_vid && active ? (int)qi : -1;
+  const float q0 = verts[3 * qv], q1 = verts[3 * qv + 1], q2 = verts[3 * qv + 2];
+  float bd[kNK];
+  int bi[kNK];
+#pragma unroll
+  for (int k = 0; k < kNK; ++k) { bd[k] = INFINITY; bi[k] = -1; }
+  for (int base = 0; base < nn; base += kKnnTile) {
+    const int m = min(kKnnTile, nn - base);
+    __syncthreads();
+    for (int t = threadIdx.x; t < m; t += 256) {
+      const int64_t nvid = node_vid[base + t];
+      tile[3 * t] = verts[3 * nvid];
+      tile[3 * t + 1] = verts[3 * nvid + 1];
+      tile[3 * t + 2] = verts[3 * nvid + 2];
+    }
+    __syncthreads();
+    if (!active) continue;
+    for (int t = 0; t < m; ++t) {
+      const int j = base + t;
+      const float dx = q0 - tile[3 * t], dy = q1 - tile[3 * t + 1], dz = q2 - tile[3 * t + 2];
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      if (!(d2 < bd[kNK - 1]) || j == self) continue;
+      // insertion (strict <: equal distances keep the lower index first)
+      int k = kNK - 1;
+      while (k > 0 && d2 < bd[k - 1]) { bd[k] = bd[k - 1]; bi[k] = bi[k - 1]; --k; }
+      bd[k] = d2;
+      bi[k] = j;
+    }
+  }
+  if (!active) return;
+  double s = 0.0;
+  float ww[kNK];
+#pragma unroll
+  for (int k = 0; k < kNK; ++k) {
+    ww[k] = weight_of(clamp_dist(bd[k]), dgw);
+    s += (double)ww[k];
+    ids[kNK * qi + k] = bi[k];
+    w[kNK * qi + k] = ww[k];
+  }
+  if (wn)
+#pragma unroll
+    for (int k = 0; k < kNK; ++k) wn[kNK * qi + k] = (float)((double)ww[k] / s);
+}
+
+// number of block contributions per vertex (10, or 1 for a node vertex)
+__global__ __launch_bounds__(256) void k_nr_contrib_count(const int *__restrict__ node_of_vertex, int64_t nv,
+                                                         int *__restrict__ cnt) {
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v < nv) cnt[v] = node_of_vertex[v] >= 0 ? 1 : 10;
+}
+
+__device__ __forceinline__ uint32_t pair_key(int a, int b) {
+  return a <= b ? ((uint32_t)a << kNodeBits) | (uint32_t)b : ((uint32_t)b << kNodeBits) | (uint32_t)a;
+}
+
+// contributions: data (v, slot a <= slot b) then ARAP (edge, role)
+__global__ __launch_bounds__(256) void k_nr_contrib_emit(const int *__restrict__ node_of_vertex,
+                                                        const int *__restrict__ vnn, const long long *__restrict__ off,
+                                                        int64_t nv, const int *__restrict__ nbr, int nn,
+                                                        long long arap_base, uint32_t *__restrict__ keys,
+                                                        uint32_t *__restrict__ codes) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < nv) {
+    long long o = off[t];
+    const int nd = node_of_vertex[t];
+    if (nd >= 0) {
+      keys[o] = pair_key(nd, nd);
+      codes[o] = (uint32_t)(t * 16);
+      return;
+    }
+    for (int a = 0; a < kNK; ++a)
+      for (int b = a; b < kNK; ++b) {
+        keys[o] = pair_key(vnn[kNK * t + a], vnn[kNK * t + b]);
+        codes[o] = (uint32_t)(t * 16 + a * 4 + b);
+        ++o;
+      }
+    return;
+  }
+  const int64_t e = t - nv;  // ARAP edge e = i * K + k
+  if (e >= (int64_t)nn * kNK) return;
+  const int i = (int)(e / kNK), j = nbr[e];
+  const long long o = arap_base + 3 * e;
+  keys[o] = pair_key(i, i);      codes[o] = kArapTag | (uint32_t)(e * 4 + 0);
+  keys[o + 1] = pair_key(j, j);  codes[o + 1] = kArapTag | (uint32_t)(e * 4 + 1);
+  keys[o + 2] = pair_key(i, j);  codes[o + 2] = kArapTag | (uint32_t)(e * 4 + 2);
+}
+
+__global__ __launch_bounds__(256) void k_flag_heads(const uint32_t *__restrict__ keys, long long n, uint32_t shift,
+                                                   int *__restrict__ flag) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) flag[i] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1 : 0;
+}
+
+// block table from the sorted contribution keys
+__global__ __launch_bounds__(256) void k_nr_blocks(const uint32_t *__restrict__ keys, const int *__restrict__ flag,
+                                                  const long long *__restrict__ rank, long long n,
+                                                  int *__restrict__ blk_a, int *__restrict__ blk_b,
+                                                  int *__restrict__ cstart) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  const long long k = rank[i];
+  blk_a[k] = (int)(keys[i] >> kNodeBits);
+  blk_b[k] = (int)(keys[i] & ((1u << kNodeBits) - 1));
+  cstart[k] = (int)i;
+}
+
+// row entries of the symmetric SpMV: (row, col) -> block | transposed << 31
+__global__ __launch_bounds__(256) void k_nr_row_emit(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
+                                                    int nb, const long long *__restrict__ off,
+                                                    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nb) return;
+  const int a = blk_a[k], b = blk_b[k];
+  long long o = off[k];
+  keys[o] = ((uint32_t)a << kNodeBits) | (uint32_t)b;
+  vals[o] = (uint32_t)k;
+  if (a != b) {
+    keys[o + 1] = ((uint32_t)b << kNodeBits) | (uint32_t)a;
+    vals[o + 1] = (uint32_t)k | 0x80000000u;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_nr_row_start(const uint32_t *__restrict__ keys, long long n,
+                                                     int *__restrict__ row_start, int nn) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i > n) return;
+  if (i == n) { row_start[nn] = (int)n; return; }
+  const int r = (int)(keys[i] >> kNodeBits);
+  const int rp = i == 0 ? -1 : (int)(keys[i - 1] >> kNodeBits);
+  for (int q = rp + 1; q <= r; ++q) row_start[q] = (int)i;
+}
+
+// ---------------------------------------------------------------- dual quaternions
+// unit quaternion of the rotation of T (FP64, Shepperd), w >= 0, and the
+// dual part 0.5 (0, t) q; rounded to float
+__device__ __forceinline__ dq8 dq_of_T(const double *T) {
+  const double m00 = T[0], m01 = T[1], m02 = T[2], m10 = T[4], m11 = T[5], m12 = T[6], m20 = T[8], m21 = T[9],
+               m22 = T[10];
+  const double tr = m00 + m11 + m22;
+  double w, x, y, z;
+  if (tr > 0) {
+    const double s = sqrt(tr + 1.0) * 2.0;
+    w = 0.25 * s; x = (m21 - m12) / s; y = (m02 - m20) / s; z = (m10 - m01) / s;
+  } else if (m00 > m11 && m00 > m22) {
+    const double s = sqrt(1.0 + m00 - m11 - m22) * 2.0;
+    w = (m21 - m12) / s; x = 0.25 * s; y = (m01 + m10) / s; z = (m02 + m20) / s;
+  } else if (m11 > m22) {
+    const double s = sqrt(1.0 + m11 - m00 - m22) * 2.0;
+    w = (m02 - m20) / s; x = (m01 + m10) / s; y = 0.25 * s; z = (m12 + m21) / s;
+  } else {
+    const double s = sqrt(1.0 + m22 - m00 - m11) * 2.0;
+    w = (m10 - m01) / s; x = (m02 + m20) / s; y = (m12 + m21) / s; z = 0.25 * s;
+  }
+  if (w < 0) { w = -w; x = -x; y = -y; z = -z; }
+  const double n = sqrt(w * w + x * x + y * y + z * z);
+  const double qw = w / n, qx = x / n, qy = y / n, qz = z / n;
+  const double tx = T[3], ty = T[7], tz = T[11];
+  const double dw = -(tx * qx + ty * qy + tz * qz);
+  const double dx = qw * tx + (ty * qz - tz * qy);
+  const double dy = qw * ty + (tz * qx - tx * qz);
+  const double dz = qw * tz + (tx * qy - ty * qx);
+  return dq8{(float)qw, (float)qx, (float)qy, (float)qz, (float)(0.5 * dw), (float)(0.5 * dx), (float)(0.5 * dy),
+             (float)(0.5 * dz)};
+}
+
+__global__ __launch_bounds__(256) void k_nr_node_dq(const double *__restrict__ T, int nn, dq8 *__restrict__ dq) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nn) dq[i] = dq_of_T(T + 16 * (size_t)i);
+}
+
+// Quaternion::rotate (Quaternion.h:278-282): v + 2 qv x (qv x v + w v)
+__device__ __forceinline__ void q_rotate(float w, float x, float y, float z, const float v[3], float o[3]) {
+  const float qx = x * 2.f, qy = y * 2.f, qz = z * 2.f;
+  const float cx = y * v[2] - z * v[1], cy = z * v[0] - x * v[2], cz = x * v[1] - y * v[0];
+  const float ax = cx + v[0] * w, ay = cy + v[1] * w, az = cz + v[2] * w;
+  o[0] = v[0] + (qy * az - qz * ay);
+  o[1] = v[1] + (qz * ax - qx * az);
+  o[2] = v[2] + (qx * ay - qy * ax);
+}
+
+// dqBlender::blendVertex (dqBlender.cu:162-186) with Kavan's sign rule and
+// one normalisation; a node vertex takes its node's dq (K)
+__global__ __launch_bounds__(256) void k_nr_blend(const float *__restrict__ verts, const float *__restrict__ normals,
+                                                 int64_t nv, const int *__restrict__ node_of_vertex,
+                                                 const int *__restrict__ vnn, const float *__restrict__ vnn_w,
+                                                 const dq8 *__restrict__ dq, float *__restrict__ out_v,
+                                                 float *__restrict__ out_n) {
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nv) return;
+  dq8 b;
+  const int nd = node_of_vertex[v];
+  if (nd >= 0) {
+    b = dq[nd];
+  } else {
+    b = dq8{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const dq8 r0 = dq[vnn[kNK * v]];
+#pragma unroll
+    for (int k = 0; k < kNK; ++k) {
+      const dq8 q = dq[vnn[kNK * v + k]];
+      float w = vnn_w[kNK * v + k];
+      const float dt = r0.rw * q.rw + r0.rx * q.rx + r0.ry * q.ry + r0.rz * q.rz;
+      if (dt < 0.f) w = -w;
+      b.rw = b.rw + w * q.rw; b.rx = b.rx + w * q.rx; b.ry = b.ry + w * q.ry; b.rz = b.rz + w * q.rz;
+      b.dw = b.dw + w * q.dw; b.dx = b.dx + w * q.dx; b.dy = b.dy + w * q.dy; b.dz = b.dz + w * q.dz;
+    }
+    const float n = sqrtf(b.rw * b.rw + b.rx * b.rx + b.ry * b.ry + b.rz * b.rz);
+    b.rw = b.rw / n; b.rx = b.rx / n; b.ry = b.ry / n; b.rz = b.rz / n;
+    b.dw = b.dw / n; b.dx = b.dx / n; b.dy = b.dy / n; b.dz = b.dz / n;
+  }
+  const float p[3] = {verts[3 * v], verts[3 * v + 1], verts[3 * v + 2]};
+  float rp[3];
+  q_rotate(b.rw, b.rx, b.ry, b.rz, p, rp);
+  const float tx = ((b.dx * b.rw - b.rx * b.dw) + (b.ry * b.dz - b.rz * b.dy)) * 2.f;
+  const float ty = ((b.dy * b.rw - b.ry * b.dw) + (b.rz * b.dx - b.rx * b.dz)) * 2.f;
+  const float tz = ((b.dz * b.rw - b.rz * b.dw) + (b.rx * b.dy - b.ry * b.dx)) * 2.f;
+  out_v[3 * v] = rp[0] + tx;
+  out_v[3 * v + 1] = rp[1] + ty;
+  out_v[3 * v + 2] = rp[2] + tz;
+  if (out_n) {
+    const float n[3] = {normals[3 * v], normals[3 * v + 1], normals[3 * v + 2]};
+    float rn[3];
+    q_rotate(b.rw, b.rx, b.ry, b.rz, n, rn);
+    out_n[3 * v] = rn[0]; out_n[3 * v + 1] = rn[1]; out_n[3 * v + 2] = rn[2];
+  }
+}
+
+// ---------------------------------------------------------------- Gauss-Newton
+struct NrFrame {
+  int rows, cols;
+  double fx, fy, cx, cy;
+  double pose[16];  // camera-to-world
+  double min_depth, max_depth, dist_max, normal_cos, lambda, damping, pcg_tol;
+};
+
+// target vertex (camera) per pixel, components rounded to float (w = 1 valid)
+__global__ __launch_bounds__(256) void k_nr_target_vert(const float *__restrict__ depth, NrFrame F,
+                                                       float4 *__restrict__ tv) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= F.rows * F.cols) return;
+  const float d = depth[i];
+  if (!(d > 0.0f)) { tv[i] = make_float4(0.f, 0.f, 0.f, 0.f); return; }
+  const int vv = i / F.cols, uu = i - vv * F.cols;
+  const double dd = (double)d;
+  tv[i] = make_float4((float)((((double)uu - F.cx) * dd) / F.fx), (float)((((double)vv - F.cy) * dd) / F.fy), d, 1.f);
+}
+
+// target normal (camera): cross(V[i+2]-V[i-2], V[i+W]-V[i-W]) in FP64 with
+// out-of-range neighbours replaced by the pixel itself; w = 1 valid
+__global__ __launch_bounds__(256) void k_nr_target_norm(const float4 *__restrict__ tv, NrFrame F,
+                                                       float4 *__restrict__ tn) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int size = F.rows * F.cols;
+  if (i >= size) return;
+  const int idx[5] = {i, i - 2 < 0 ? i : i - 2, i + 2 >= size ? i : i + 2, i - F.cols <= 0 ? i : i - F.cols,
+                      i + F.cols >= size ? i : i + F.cols};
+  float4 V[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    V[k] = tv[idx[k]];
+    if (V[k].w == 0.f) { tn[i] = make_float4(0.f, 0.f, 0.f, 0.f); return; }
+  }
+  const double ax = (double)V[2].x - (double)V[1].x, ay = (double)V[2].y - (double)V[1].y,
+               az = (double)V[2].z - (double)V[1].z;
+  const double bx = (double)V[4].x - (double)V[3].x, by = (double)V[4].y - (double)V[3].y,
+               bz = (double)V[4].z - (double)V[3].z;
+  const double c0 = ay * bz - az * by, c1 = az * bx - ax * bz, c2 = ax * by - ay * bx;
+  const double nn = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  if (!(nn > 0)) { tn[i] = make_float4(0.f, 0.f, 0.f, 0.f); return; }
+  tn[i] = make_float4((float)(c0 / nn), (float)(c1 / nn), (float)(c2 / nn), 1.f);
+}
+
+// per-vertex data term: g = [p' x n, n] (world), residual r = n . (p' - q);
+// out[8 * v] = g0..g5, r, valid
+__global__ __launch_bounds__(256) void k_nr_data(const float *__restrict__ pv, const float *__restrict__ pn,
+                                                int64_t nv, const float4 *__restrict__ tv,
+                                                const float4 *__restrict__ tn, const float *__restrict__ depth,
+                                                NrFrame F, float *__restrict__ out) {
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nv) return;
+  float *o = out + 8 * v;
+  o[7] = 0.f;
+  const double *pose = F.pose;
+  const double p[3] = {pv[3 * v], pv[3 * v + 1], pv[3 * v + 2]};
+  const double nw[3] = {pn[3 * v], pn[3 * v + 1], pn[3 * v + 2]};
+  double pc[3], ncam[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    pc[r] = pose[r] * (p[0] - pose[3]) + pose[4 + r] * (p[1] - pose[7]) + pose[8 + r] * (p[2] - pose[11]);
+    ncam[r] = pose[r] * nw[0] + pose[4 + r] * nw[1] + pose[8 + r] * nw[2];
+  }
+  if (!(pc[2] > 0)) return;
+  const double uf = F.fx * pc[0] / pc[2] + F.cx, vf = F.fy * pc[1] / pc[2] + F.cy;
+  if (!(uf > -1.0 && vf > -1.0 && uf < (double)F.cols && vf < (double)F.rows)) return;
+  const int u = (int)floor(uf + 0.5), vv = (int)floor(vf + 0.5);
+  if (u < 0 || vv < 0 || u >= F.cols || vv >= F.rows) return;
+  const int pix = vv * F.cols + u;
+  const float d = depth[pix];
+  if (!(d >= (float)F.min_depth && d <= (float)F.max_depth)) return;
+  const float4 qv = tv[pix], qn = tn[pix];
+  if (qn.w == 0.f) return;
+  const double q[3] = {qv.x, qv.y, qv.z}, nt[3] = {qn.x, qn.y, qn.z};
+  const double dx = pc[0] - q[0], dy = pc[1] - q[1], dz = pc[2] - q[2];
+  if (dx * dx + dy * dy + dz * dz > F.dist_max * F.dist_max) return;
+  if (fabs(ncam[0] * nt[0] + ncam[1] * nt[1] + ncam[2] * nt[2]) < F.normal_cos) return;
+  double nW[3], qW[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    nW[r] = pose[4 * r] * nt[0] + pose[4 * r + 1] * nt[1] + pose[4 * r + 2] * nt[2];
+    qW[r] = pose[4 * r] * q[0] + pose[4 * r + 1] * q[1] + pose[4 * r + 2] * q[2] + pose[4 * r + 3];
+  }
+  const double res = nW[0] * (p[0] - qW[0]) + nW[1] * (p[1] - qW[1]) + nW[2] * (p[2] - qW[2]);
+  o[0] = (float)(p[1] * nW[2] - p[2] * nW[1]);
+  o[1] = (float)(p[2] * nW[0] - p[0] * nW[2]);
+  o[2] = (float)(p[0] * nW[1] - p[1] * nW[0]);
+  o[3] = (float)nW[0]; o[4] = (float)nW[1]; o[5] = (float)nW[2];
+  o[6] = (float)res;
+  o[7] = 1.f;
+}
+
+// ARAP edge e = i * K + k (j = k-th neighbour of i): a = T_i g_j, b = T_j g_j,
+// w_ij = max(w_i[k], w_j[k]) (nonRigidICP.cu:494-515); out[8 e] = a, b, w
+__global__ __launch_bounds__(256) void k_nr_arap(const double *__restrict__ T, const float *__restrict__ verts,
+                                                const int *__restrict__ node_vid, const int *__restrict__ nbr,
+                                                const float *__restrict__ nbr_w, int nn, double *__restrict__ out) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= nn * kNK) return;
+  const int i = e / kNK, k = e - i * kNK, j = nbr[e];
+  const double *Ti = T + 16 * (size_t)i, *Tj = T + 16 * (size_t)j;
+  const int64_t gv = node_vid[j];
+  const double g0 = verts[3 * gv], g1 = verts[3 * gv + 1], g2 = verts[3 * gv + 2];
+  double *o = out + 8 * (size_t)e;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    o[r] = Ti[4 * r] * g0 + Ti[4 * r + 1] * g1 + Ti[4 * r + 2] * g2 + Ti[4 * r + 3];
+    o[3 + r] = Tj[4 * r] * g0 + Tj[4 * r + 1] * g1 + Tj[4 * r + 2] * g2 + Tj[4 * r + 3];
+  }
+  double wij = (double)nbr_w[e];
+  const double wj = (double)nbr_w[kNK * j + k];
+  if (wj > wij) wij = wj;
+  o[6] = wij;
+  o[7] = 0.0;
+}
+
+// ARAP Jacobian rows for component c: Ji = w [row c of -[a]x, e_c],
+// Jj = -w [row c of -[b]x, e_c]
+__device__ __forceinline__ void arap_rows(const double *ed, int c, double Ji[6], double Jj[6]) {
+  const double *A = ed, *B = ed + 3;
+  const double w = ed[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) { Ji[q] = 0.0; Jj[q] = 0.0; }
+  if (c == 0) { Ji[1] = A[2]; Ji[2] = -A[1]; Jj[1] = B[2]; Jj[2] = -B[1]; }
+  if (c == 1) { Ji[0] = -A[2]; Ji[2] = A[0]; Jj[0] = -B[2]; Jj[2] = B[0]; }
+  if (c == 2) { Ji[0] = A[1]; Ji[1] = -A[0]; Jj[0] = B[1]; Jj[1] = -B[0]; }
+  Ji[3 + c] = 1.0;
+  Jj[3 + c] = 1.0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) { Ji[q] *= w; Jj[q] *= -w; }
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+// One wavefront per block: H_ab (and for a == b the rhs, the damping and the
+// FP64 inverse of the block).  Lanes take contributions strided by 64; the
+// lane sums are combined by a fixed xor tree.
+__global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
+                                                    const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
+                                                    int nb, const float *__restrict__ vdata,
+                                                    const int *__restrict__ vnn, const float *__restrict__ vnn_wn,
+                                                    const int *__restrict__ node_of_vertex, const int *__restrict__ nbr,
+                                                    const double *__restrict__ edata, NrFrame F,
+                                                    float *__restrict__ H, double *__restrict__ rhs,
+                                                    double *__restrict__ Minv) {
+  const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (blk >= nb) return;  // uniform per wavefront
+  const int a = blk_a[blk], b = blk_b[blk];
+  const bool diag = a == b;
+  double h[36], g[6];
+#pragma unroll
+  for (int q = 0; q < 36; ++q) h[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) g[q] = 0.0;
+  const double lam = F.lambda;
+  for (int c = cstart[blk] + lane; c < cstart[blk + 1]; c += 64) {
+    const uint32_t code = codes[c];
+    if (!(code & kArapTag)) {
+      const int64_t v = code >> 4;
+      const float *d = vdata + 8 * v;
+      if (d[7] == 0.f) continue;
+      double wa, wb;
+      if (node_of_vertex[v] >= 0) {
+        wa = 1.0; wb = 1.0;
+      } else {
+        const int sa = (code >> 2) & 3, sb = code & 3;
+        wa = (double)vnn_wn[kNK * v + sa];
+        wb = (double)vnn_wn[kNK * v + sb];
+      }
+      const double gv[6] = {d[0], d[1], d[2], d[3], d[4], d[5]};
+      const double s = wa * wb;
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) h[6 * r + q] += s * gv[r] * gv[q];
+      if (diag) {
+        const double rr = (double)d[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) g[r] -= wa * gv[r] * rr;
+      }
+    } else {
+      const int e = (int)((code & ~kArapTag) >> 2), role = code & 3;
+      const double *ed = edata + 8 * (size_t)e;
+      const int i = e / kNK, j = nbr[e];
+#pragma unroll
+      for (int c3 = 0; c3 < 3; ++c3) {
+        double Ji[6], Jj[6];
+        arap_rows(ed, c3, Ji, Jj);
+        const double rc = ed[6] * (ed[c3] - ed[3 + c3]);
+        // the block is (min, max): transpose the (i, j) product when i > j
+        const double *U = role == 0 ? Ji : (role == 1 ? Jj : (i <= j ? Ji : Jj));
+        const double *W = role == 0 ? Ji : (role == 1 ? Jj : (i <= j ? Jj : Ji));
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+          for (int q = 0; q < 6; ++q) h[6 * r + q] += lam * U[r] * W[q];
+        if (role < 2)
+#pragma unroll
+          for (int r = 0; r < 6; ++r) g[r] -= lam * U[r] * rc;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 36; ++q) h[q] = wave_sum(h[q]);
+  if (diag)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) g[q] = wave_sum(g[q]);
+  if (lane != 0) return;
+  if (diag)
+#pragma unroll
+    for (int r = 0; r < 6; ++r) h[7 * r] += F.damping;
+  float *Hb = H + 36 * (size_t)blk;
+#pragma unroll
+  for (int q = 0; q < 36; ++q) Hb[q] = (float)h[q];
+  if (!diag) return;
+#pragma unroll
+  for (int r = 0; r < 6; ++r) rhs[6 * a + r] = g[r];
+  double *M = Minv + 36 * (size_t)a;
+  for (int c = 0; c < 6; ++c) {
+    double e6[6] = {0, 0, 0, 0, 0, 0}, col[6];
+    e6[c] = 1.0;
+    if (ldlt6(h, e6, col) != 0)
+      for (int r = 0; r < 6; ++r) col[r] = 0.0;
+    for (int r = 0; r < 6; ++r) M[6 * r + c] = col[r];
+  }
+}
+
+// ---------------------------------------------------------------- PCG
+struct PcgState {
+  double *x, *r, *z, *p, *q;  // 6 N each
+  double *part_pq;            // per-workgroup partials of p.q (written by spmv, read by update)
+  double *part_rz;            // per-workgroup partials of r.z (written by init/update, read by dir)
+  double *rz;                 // rz[k]: r.z after iteration k (k = 0 .. max)
+  int *flag;                  // [0] converged, [1] iterations run
+  int nwg;
+};
+
+// fixed-order sum of the workgroup partials, computed by every workgroup
+__device__ double sum_partials(const double *part, int n, double *lds1) {
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += part[k];
+    lds1[0] = s;
+  }
+  __syncthreads();
+  const double s = lds1[0];
+  __syncthreads();
+  return s;
+}
+
+__device__ void wg_partial(double x, double *lds, double *part) {
+  const double w = wave_sum(x);
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[wid] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < kPcgThreads / 64; ++k) s += lds[k];
+    part[blockIdx.x] = s;
+  }
+}
+
+// z = Minv r for the node of this thread (all 6 rows of a node are in the workgroup)
+__device__ __forceinline__ double precond_row(const double *Minv, const double *rl, int node, int row) {
+  const double *M = Minv + 36 * (size_t)node + 6 * row;
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) s += M[c] * rl[c];
+  return s;
+}
+
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgState S, const double *__restrict__ rhs,
+                                                         const double *__restrict__ Minv, int nn) {
+  __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1];
+  const int t = blockIdx.x * kPcgThreads + threadIdx.x;
+  const int node = t / 6, row = t - node * 6;
+  const bool act = node < nn;
+  const double r = act ? rhs[t] : 0.0;
+  rl[threadIdx.x] = r;
+  __syncthreads();
+  double z = 0.0;
+  if (act) {
+    z = precond_row(Minv, rl + (threadIdx.x - row), node, row);
+    S.x[t] = 0.0; S.r[t] = r; S.z[t] = z; S.p[t] = z;
+  }
+  wg_partial(r * z, lds, S.part_rz);
+  if (t == 0) { S.flag[0] = 0; S.flag[1] = 0; }
+}
+
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_init_done(PcgState S) {
+  __shared__ double l1[1];
+  const double rz = sum_partials(S.part_rz, S.nwg, l1);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.rz[0] = rz;
+    if (!(rz > 0.0)) S.flag[0] = 1;
+  }
+}
+
+// q = H p (one thread per row), partial p.q
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(PcgState S, const float *__restrict__ H,
+                                                         const int *__restrict__ row_start,
+                                                         const uint32_t *__restrict__ row_ent,
+                                                         const int *__restrict__ blk_a, const int *__restrict__ blk_b,
+                                                         int nn) {
+  if (S.flag[0]) return;
+  __shared__ double lds[kPcgThreads / 64 + 1];
+  const int t = blockIdx.x * kPcgThreads + threadIdx.x;
+  const int node = t / 6, row = t - node * 6;
+  double qv = 0.0, pv = 0.0;
+  if (node < nn) {
+    for (int k = row_start[node]; k < row_start[node + 1]; ++k) {
+      const uint32_t ent = row_ent[k];
+      const int blk = (int)(ent & 0x7fffffffu);
+      const bool tr = ent >> 31;
+      const int col = tr ? blk_a[blk] : blk_b[blk];
+      const float *B = H + 36 * (size_t)blk;
+      const double *pc = S.p + 6 * (size_t)col;
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s += (double)(tr ? B[6 * c + row] : B[6 * row + c]) * pc[c];
+      qv += s;
+    }
+    S.q[t] = qv;
+    pv = S.p[t];
+  }
+  wg_partial(pv * qv, lds, S.part_pq);
+}
+
+// alpha = rz / pq; x += alpha p; r -= alpha q; z = Minv r; partial r.z
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const double *__restrict__ Minv, int nn,
+                                                           int it) {
+  if (S.flag[0]) return;
+  __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1], l1[1];
+  const double pq = sum_partials(S.part_pq, S.nwg, l1);
+  const double alpha = S.rz[it] / pq;
+  const int t = blockIdx.x * kPcgThreads + threadIdx.x;
+  const int node = t / 6, row = t - node * 6;
+  const bool act = node < nn;
+  double r = 0.0;
+  if (act) {
+    S.x[t] += alpha * S.p[t];
+    r = S.r[t] - alpha * S.q[t];
+    S.r[t] = r;
+  }
+  rl[threadIdx.x] = r;
+  __syncthreads();
+  double z = 0.0;
+  if (act) {
+    z = precond_row(Minv, rl + (threadIdx.x - row), node, row);
+    S.z[t] = z;
+  }
+  __syncthreads();
+  wg_partial(r * z, lds, S.part_rz);
+}
+
+// beta = rz_new / rz; p = z + beta p; convergence test
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(PcgState S, int nn, int it, double tol) {
+  if (S.flag[0]) return;
+  __shared__ double l1[1];
+  const double rzn = sum_partials(S.part_rz, S.nwg, l1);
+  const double beta = rzn / S.rz[it];
+  const int t = blockIdx.x * kPcgThreads + threadIdx.x;
+  if (t < 6 * nn) S.p[t] = S.z[t] + beta * S.p[t];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.rz[it + 1] = rzn;
+    S.flag[1] = it + 1;
+    if (!(rzn > tol * tol * S.rz[0]) || !(rzn > 0.0)) S.flag[0] = 1;
+  }
+}
+
+// T_i <- exp(x_i) T_i, then the node's dual quaternion
+__global__ __launch_bounds__(256) void k_nr_update(const double *__restrict__ x, double *__restrict__ T, int nn,
+                                                  dq8 *__restrict__ dq) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  double E[16], Tn[16];
+  se3_exp(x + 6 * (size_t)i, E);
+  mat4_mul(E, T + 16 * (size_t)i, Tn);
+  for (int k = 0; k < 16; ++k) T[16 * (size_t)i + k] = Tn[k];
+  dq[i] = dq_of_T(Tn);
+}
+
+// energies at the last linearisation: [0] residual count, [1] data, [2] ARAP
+__global__ __launch_bounds__(1024) void k_nr_stats(const float *__restrict__ vdata, int64_t nv,
+                                                  const double *__restrict__ edata, int ne, double lam,
+                                                  double *__restrict__ out) {
+  __shared__ double lds[3][1024 / 64];
+  double c = 0.0, ed = 0.0, er = 0.0;
+  for (int64_t v = threadIdx.x; v < nv; v += 1024) {
+    const float *d = vdata + 8 * v;
+    if (d[7] != 0.f) { c += 1.0; ed += (double)d[6] * (double)d[6]; }
+  }
+  for (int e = threadIdx.x; e < ne; e += 1024) {
+    const double *q = edata + 8 * (size_t)e;
+    for (int k = 0; k < 3; ++k) {
+      const double rc = q[6] * (q[k] - q[3 + k]);
+      er += lam * rc * rc;
+    }
+  }
+  c = wave_sum(c); ed = wave_sum(ed); er = wave_sum(er);
+  if ((threadIdx.x & 63) == 0) { lds[0][threadIdx.x >> 6] = c; lds[1][threadIdx.x >> 6] = ed; lds[2][threadIdx.x >> 6] = er; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s[3] = {0, 0, 0};
+    for (int k = 0; k < 1024 / 64; ++k)
+      for (int q = 0; q < 3; ++q) s[q] += lds[q][k];
+    out[0] = s[0]; out[1] = s[1]; out[2] = s[2];
+  }
+}
+
+__global__ void k_set_identity(double *T, int nn) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  for (int k = 0; k < 16; ++k) T[16 * (size_t)i + k] = (k % 5 == 0) ? 1.0 : 0.0;
+}
+
+}  // namespace dm
+
+// ---------------------------------------------------------------- host object
+using namespace dm;
+
+struct dm_graph {
+  hipStream_t stream = nullptr;
+  int64_t nv = 0;
+  int nn = 0, nb = 0;
+  long long ncontrib = 0, nrow = 0;
+  dm_graph_props props{};
+  std::vector<void *> allocs;
+  // canonical mesh and graph
+  float *verts = nullptr, *normals = nullptr;
+  int *node_vid = nullptr, *node_of_vertex = nullptr, *nbr = nullptr, *vnn = nullptr;
+  float *nbr_w = nullptr, *vnn_w = nullptr, *vnn_wn = nullptr;
+  double *T = nullptr;
+  dq8 *dq = nullptr;
+  // block pattern
+  int *blk_a = nullptr, *blk_b = nullptr, *cstart = nullptr, *row_start = nullptr;
+  uint32_t *codes = nullptr, *row_ent = nullptr;
+  // per-iteration buffers
+  float *pv = nullptr, *pn = nullptr, *vdata = nullptr, *H = nullptr;
+  double *edata = nullptr, *rhs = nullptr, *Minv = nullptr, *stats = nullptr;
+  PcgState pcg{};
+  int pcg_cap = 0;
+  // frame staging
+  float *depth = nullptr;
+  float4 *tv = nullptr, *tn = nullptr;
+  size_t pixels_cap = 0;
+
+  template <class T> int dalloc(T **p, size_t n) {
+    void *q = nullptr;
+    DM_HIP(hipMalloc(&q, std::max<size_t>(n * sizeof(T), 16)));
+    allocs.push_back(q);
+    *p = (T *)q;
+    return DM_OK;
+  }
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *p : allocs) (void)hipFree(p);
+    allocs.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+#define DM_TRY(x)                  \
+  do {                             \
+    int rc_ = (x);                 \
+    if (rc_ != DM_OK) return rc_;  \
+  } while (0)
+
+static unsigned grid_of(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+static int sort_pairs(const uint32_t *ki, uint32_t *ko, const uint32_t *vi, uint32_t *vo, long long n, int bits,
+                      hipStream_t st) {
+  size_t tb = 0;
+  DM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, ki, ko, vi, vo, (int)n, 0, bits, st));
+  DevBuf tmp;
+  DM_TRY(dev_alloc(tmp, tb));
+  DM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, ki, ko, vi, vo, (int)n, 0, bits, st));
+  DM_HIP(hipStreamSynchronize(st));  // tmp is released on return
+  return DM_OK;
+}
+
+static int graph_build(dm_graph *g, const float *h_verts, const float *h_normals) {
+  hipStream_t st = g->stream;
+  const int64_t nv = g->nv;
+  DM_TRY(g->dalloc(&g->verts, (size_t)nv * 3));
+  DM_TRY(g->dalloc(&g->normals, (size_t)nv * 3));
+  DM_HIP(hipMemcpyAsync(g->verts, h_verts, (size_t)nv * 12, hipMemcpyHostToDevice, st));
+  DM_HIP(hipMemcpyAsync(g->normals, h_normals, (size_t)nv * 12, hipMemcpyHostToDevice, st));
+  // node sampling
+  uint32_t tsize = 1;
+  while (tsize < 2 * (uint64_t)nv) tsize <<= 1;
+  DevBuf hkeys, hvals, isnode, rank, wsum;
+  DM_TRY(dev_alloc(hkeys, (size_t)tsize * 8));
+  DM_TRY(dev_alloc(hvals, (size_t)tsize * 8));
+  DM_TRY(dev_alloc(isnode, (size_t)nv * 4));
+  DM_TRY(dev_alloc(rank, (size_t)nv * 8));
+  DM_HIP(hipMemsetAsync(hkeys.p, 0, (size_t)tsize * 8, st));
+  DM_HIP(hipMemsetAsync(hvals.p, 0xff, (size_t)tsize * 8, st));
+  const float r = g->props.node_radius;
+  hipLaunchKernelGGL(k_nr_cells, dim3(grid_of(nv)), dim3(256), 0, st, g->verts, nv, r, hkeys.as<uint64_t>(),
+                     hvals.as<unsigned long long>(), tsize - 1);
+  hipLaunchKernelGGL(k_nr_mark, dim3(grid_of(nv)), dim3(256), 0, st, g->verts, nv, r, hkeys.as<uint64_t>(),
+                     hvals.as<unsigned long long>(), tsize - 1, isnode.as<int>());
+  DM_HIP(hipGetLastError());
+  long long *d_nn = nullptr, nn = 0;
+  DM_TRY(scan_ints(isnode.as<int>(), (int)nv, rank.as<long long>(), wsum, st, &d_nn));
+  DM_HIP(hipMemcpyAsync(&nn, d_nn, 8, hipMemcpyDeviceToHost, st));
+  DM_HIP(hipStreamSynchronize(st));
+  if (nn < kNK + 1) { set_error("deformation graph: fewer than 5 nodes"); return DM_E_INVALID; }
+  if (nn > g->props.max_nodes) {
+    set_error("deformation graph: " + std::to_string(nn) + " nodes exceed max_nodes");
+    return DM_E_CAPACITY;
+  }
+  g->nn = (int)nn;
+  DM_TRY(g->dalloc(&g->node_vid, nn));
+  DM_TRY(g->dalloc(&g->node_of_vertex, nv));
+  hipLaunchKernelGGL(k_nr_nodes, dim3(grid_of(nv)), dim3(256), 0, st, isnode.as<int>(), rank.as<long long>(), nv,
+                     g->node_vid, g->node_of_vertex);
+  // neighbours and weights
+  DM_TRY(g->dalloc(&g->nbr, nn * kNK));
+  DM_TRY(g->dalloc(&g->nbr_w, nn * kNK));
+  DM_TRY(g->dalloc(&g->vnn, (size_t)nv * kNK));
+  DM_TRY(g->dalloc(&g->vnn_w, (size_t)nv * kNK));
+  DM_TRY(g->dalloc(&g->vnn_wn, (size_t)nv * kNK));
+  hipLaunchKernelGGL(k_nr_knn, dim3(grid_of(nn)), dim3(256), 0, st, g->verts, g->node_vid, (int)nn, g->node_vid,
+                     (int64_t)nn, g->props.dgw, g->nbr, g->nbr_w, (float *)nullptr);
+  hipLaunchKernelGGL(k_nr_knn, dim3(grid_of(nv)), dim3(256), 0, st, g->verts, g->node_vid, (int)nn,
+                     (const int *)nullptr, nv, g->props.dgw, g->vnn, g->vnn_w, g->vnn_wn);
+  DM_HIP(hipGetLastError());
+  // block pattern: contributions sorted by block key (stable: vertex order, then edges)
+  DevBuf ccnt, coff, wsum2;
+  DM_TRY(dev_alloc(ccnt, (size_t)nv * 4));
+  DM_TRY(dev_alloc(coff, (size_t)nv * 8));
+  hipLaunchKernelGGL(k_nr_contrib_count, dim3(grid_of(nv)), dim3(256), 0, st, g->node_of_vertex, nv, ccnt.as<int>());
+  long long *d_nd = nullptr, ndata = 0;
+  DM_TRY(scan_ints(ccnt.as<int>(), (int)nv, coff.as<long long>(), wsum2, st, &d_nd));
+  DM_HIP(hipMemcpyAsync(&ndata, d_nd, 8, hipMemcpyDeviceToHost, st));
+  DM_HIP(hipStreamSynchronize(st));
+  const long long nc = ndata + 3LL * nn * kNK;
+  if (nc >= (1LL << 31)) { set_error("deformation graph: too many block contributions"); return DM_E_CAPACITY; }
+  g->ncontrib = nc;
+  DevBuf k0, c0, k1, flag, frank, wsum3;
+  DM_TRY(dev_alloc(k0, nc * 4));
+  DM_TRY(dev_alloc(c0, nc * 4));
+  DM_TRY(dev_alloc(k1, nc * 4));
+  DM_TRY(g->dalloc(&g->codes, nc));
+  hipLaunchKernelGGL(k_nr_contrib_emit, dim3(grid_of(nv + nn * kNK)), dim3(256), 0, st, g->node_of_vertex, g->vnn,
+                     coff.as<long long>(), nv, g->nbr, (int)nn, ndata, k0.as<uint32_t>(), c0.as<uint32_t>());
+  DM_HIP(hipGetLastError());
+  DM_TRY(sort_pairs(k0.as<uint32_t>(), k1.as<uint32_t>(), c0.as<uint32_t>(), g->codes, nc, 2 * kNodeBits, st));
+  DM_TRY(dev_alloc(flag, nc * 4));
+  DM_TRY(dev_alloc(frank, nc * 8));
+  hipLaunchKernelGGL(k_flag_heads, dim3(grid_of(nc)), dim3(256), 0, st, k1.as<uint32_t>(), nc, 0u, flag.as<int>());
+  long long *d_nb = nullptr, nb = 0;
+  DM_TRY(scan_ints(flag.as<int>(), (int)nc, frank.as<long long>(), wsum3, st, &d_nb));
+  DM_HIP(hipMemcpyAsync(&nb, d_nb, 8, hipMemcpyDeviceToHost, st));
+  DM_HIP(hipStreamSynchronize(st));
+  g->nb = (int)nb;
+  DM_TRY(g->dalloc(&g->blk_a, nb));
+  DM_TRY(g->dalloc(&g->blk_b, nb));
+  DM_TRY(g->dalloc(&g->cstart, nb + 1));
+  hipLaunchKernelGGL(k_nr_blocks, dim3(grid_of(nc)), dim3(256), 0, st, k1.as<uint32_t>(), flag.as<int>(),
+                     frank.as<long long>(), nc, g->blk_a, g->blk_b, g->cstart);
+  const int ncs = (int)nc;
+  DM_HIP(hipMemcpyAsync(g->cstart + nb, &ncs, 4, hipMemcpyHostToDevice, st));
+  // row index for the symmetric SpMV
+  std::vector<int> ha(nb), hb(nb);
+  DM_HIP(hipMemcpyAsync(ha.data(), g->blk_a, nb * 4, hipMemcpyDeviceToHost, st));
+  DM_HIP(hipMemcpyAsync(hb.data(), g->blk_b, nb * 4, hipMemcpyDeviceToHost, st));
+  DM_HIP(hipStreamSynchronize(st));
+  long long nrow = 0;
+  std::vector<long long> roff(nb);
+  for (long long k = 0; k < nb; ++k) { roff[k] = nrow; nrow += ha[k] == hb[k] ? 1 : 2; }
+  g->nrow = nrow;
+  DevBuf droff, rk0, rk1, rv0;
+  DM_TRY(dev_alloc(droff, nb * 8));
+  DM_TRY(dev_alloc(rk0, nrow * 4));
+  DM_TRY(dev_alloc(rk1, nrow * 4));
+  DM_TRY(dev_alloc(rv0, nrow * 4));
+  DM_TRY(g->dalloc(&g->row_ent, nrow));
+  DM_TRY(g->dalloc(&g->row_start, nn + 1));
+  DM_HIP(hipMemcpyAsync(droff.p, roff.data(), nb * 8, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_nr_row_emit, dim3(grid_of(nb)), dim3(256), 0, st, g->blk_a, g->blk_b, (int)nb,
+                     droff.as<long long>(), rk0.as<uint32_t>(), rv0.as<uint32_t>());
+  DM_TRY(sort_pairs(rk0.as<uint32_t>(), rk1.as<uint32_t>(), rv0.as<uint32_t>(), g->row_ent, nrow, 2 * kNodeBits, st));
+  hipLaunchKernelGGL(k_nr_row_start, dim3(grid_of(nrow + 1)), dim3(256), 0, st, rk1.as<uint32_t>(), nrow,
+                     g->row_start, (int)nn);
+  DM_HIP(hipGetLastError());
+  // state and per-iteration buffers
+  DM_TRY(g->dalloc(&g->T, nn * 16));
+  DM_TRY(g->dalloc(&g->dq, nn));
+  hipLaunchKernelGGL(k_set_identity, dim3(grid_of(nn)), dim3(256), 0, st, g->T, (int)nn);
+  hipLaunchKernelGGL(k_nr_node_dq, dim3(grid_of(nn)), dim3(256), 0, st, g->T, (int)nn, g->dq);
+  DM_TRY(g->dalloc(&g->pv, (size_t)nv * 3));
+  DM_TRY(g->dalloc(&g->pn, (size_t)nv * 3));
+  DM_TRY(g->dalloc(&g->vdata, (size_t)nv * 8));
+  DM_TRY(g->dalloc(&g->edata, (size_t)nn * kNK * 8));
+  DM_TRY(g->dalloc(&g->H, (size_t)nb * 36));
+  DM_TRY(g->dalloc(&g->rhs, (size_t)nn * 6));
+  DM_TRY(g->dalloc(&g->Minv, (size_t)nn * 36));
+  DM_TRY(g->dalloc(&g->stats, 4));
+  PcgState &S = g->pcg;
+  S.nwg = (int)grid_of(6 * nn, kPcgThreads);
+  DM_TRY(g->dalloc(&S.x, nn * 6));
+  DM_TRY(g->dalloc(&S.r, nn * 6));
+  DM_TRY(g->dalloc(&S.z, nn * 6));
+  DM_TRY(g->dalloc(&S.p, nn * 6));
+  DM_TRY(g->dalloc(&S.q, nn * 6));
+  DM_TRY(g->dalloc(&S.part_pq, S.nwg));
+  DM_TRY(g->dalloc(&S.part_rz, S.nwg));
+  DM_TRY(g->dalloc(&S.flag, 2));
+  DM_HIP(hipStreamSynchronize(st));
+  return DM_OK;
+}
+
+extern "C" {
+
+int dm_graph_create(const float *vertices, const float *normals, int64_t nv, const dm_graph_props *props,
+                    dm_graph **out) {
+  if (!vertices || !normals || !props || !out || nv <= 0 || nv >= (1LL << 27)) {
+    set_error("dm_graph_create: invalid argument (1 <= vertices < 2^27)");
+    return DM_E_INVALID;
+  }
+  if (!(props->node_radius > 0.f) || !(props->dgw > 0.f) || props->max_nodes < kNK + 1 ||
+      props->max_nodes > kMaxNodes) {
+    set_error("dm_graph_create: node_radius, dgw > 0 and 5 <= max_nodes <= 32768 required");
+    return DM_E_INVALID;
+  }
+  dm_graph *g = new dm_graph();
+  g->nv = nv;
+  g->props = *props;
+  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete g;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  int rc = graph_build(g, vertices, normals);
+  if (rc != DM_OK) {
+    g->release();
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return DM_OK;
+}
+
+int dm_graph_destroy(dm_graph *g) {
+  if (!g) return DM_OK;
+  g->release();
+  delete g;
+  return DM_OK;
+}
+
+int dm_graph_num_nodes(dm_graph *g, int32_t *n) {
+  if (!g || !n) { set_error("null argument"); return DM_E_INVALID; }
+  *n = g->nn;
+  return DM_OK;
+}
+
+int dm_graph_get_nodes(dm_graph *g, int32_t *vertex_ids, int32_t *nbr_ids, float *nbr_weights) {
+  if (!g) { set_error("null argument"); return DM_E_INVALID; }
+  if (vertex_ids) DM_HIP(hipMemcpyAsync(vertex_ids, g->node_vid, (size_t)g->nn * 4, hipMemcpyDeviceToHost, g->stream));
+  if (nbr_ids) DM_HIP(hipMemcpyAsync(nbr_ids, g->nbr, (size_t)g->nn * kNK * 4, hipMemcpyDeviceToHost, g->stream));
+  if (nbr_weights)
+    DM_HIP(hipMemcpyAsync(nbr_weights, g->nbr_w, (size_t)g->nn * kNK * 4, hipMemcpyDeviceToHost, g->stream));
+  DM_HIP(hipStreamSynchronize(g->stream));
+  return DM_OK;
+}
+
+int dm_graph_get_vertex_nn(dm_graph *g, int32_t *ids, float *weights) {
+  if (!g) { set_error("null argument"); return DM_E_INVALID; }
+  if (ids) DM_HIP(hipMemcpyAsync(ids, g->vnn, (size_t)g->nv * kNK * 4, hipMemcpyDeviceToHost, g->stream));
+  if (weights) DM_HIP(hipMemcpyAsync(weights, g->vnn_w, (size_t)g->nv * kNK * 4, hipMemcpyDeviceToHost, g->stream));
+  DM_HIP(hipStreamSynchronize(g->stream));
+  return DM_OK;
+}
+
+int dm_graph_num_blocks(dm_graph *g, int32_t *nb) {
+  if (!g || !nb) { set_error("null argument"); return DM_E_INVALID; }
+  *nb = g->nb;
+  return DM_OK;
+}
+
+int dm_graph_set_transforms(dm_graph *g, const double *T) {
+  if (!g || !T) { set_error("null argument"); return DM_E_INVALID; }
+  DM_HIP(hipMemcpyAsync(g->T, T, (size_t)g->nn * 128, hipMemcpyHostToDevice, g->stream));
+  hipLaunchKernelGGL(k_nr_node_dq, dim3(grid_of(g->nn)), dim3(256), 0, g->stream, g->T, g->nn, g->dq);
+  DM_HIP(hipGetLastError());
+  DM_HIP(hipStreamSynchronize(g->stream));
+  return DM_OK;
+}
+
+int dm_graph_get_transforms(dm_graph *g, double *T) {
+  if (!g || !T) { set_error("null argument"); return DM_E_INVALID; }
+  DM_HIP(hipMemcpyAsync(T, g->T, (size_t)g->nn * 128, hipMemcpyDeviceToHost, g->stream));
+  DM_HIP(hipStreamSynchronize(g->stream));
+  return DM_OK;
+}
+
+int dm_blend(dm_graph *g, float *out_vertices, float *out_normals) {
+  if (!g || !out_vertices) { set_error("null argument"); return DM_E_INVALID; }
+  hipLaunchKernelGGL(k_nr_blend, dim3(grid_of(g->nv)), dim3(256), 0, g->stream, g->verts, g->normals, g->nv,
+                     g->node_of_vertex, g->vnn, g->vnn_w, g->dq, g->pv, g->pn);
+  DM_HIP(hipGetLastError());
+  DM_HIP(hipMemcpyAsync(out_vertices, g->pv, (size_t)g->nv * 12, hipMemcpyDeviceToHost, g->stream));
+  if (out_normals) DM_HIP(hipMemcpyAsync(out_normals, g->pn, (size_t)g->nv * 12, hipMemcpyDeviceToHost, g->stream));
+  DM_HIP(hipStreamSynchronize(g->stream));
+  return DM_OK;
+}
+
+int dm_nonrigid_solve(dm_graph *g, const float *depth, const dm_sensor *sensor, const double pose[16],
+                      const dm_nonrigid_props *pr, double stats[4]) {
+  if (!g || !depth || !sensor || !pose || !pr || sensor->rows == 0 || sensor->cols == 0 || pr->gn_iterations < 0 ||
+      pr->pcg_iterations < 0 || pr->pcg_iterations > 1000) {
+    set_error("dm_nonrigid_solve: invalid argument");
+    return DM_E_INVALID;
+  }
+  hipStream_t st = g->stream;
+  const size_t P = (size_t)sensor->rows * sensor->cols;
+  if (P > g->pixels_cap) {
+    DM_TRY(g->dalloc(&g->depth, P));
+    DM_TRY(g->dalloc(&g->tv, P));
+    DM_TRY(g->dalloc(&g->tn, P));
+    g->pixels_cap = P;
+  }
+  if (pr->pcg_iterations + 1 > g->pcg_cap) {
+    DM_TRY(g->dalloc(&g->pcg.rz, pr->pcg_iterations + 1));
+    g->pcg_cap = pr->pcg_iterations + 1;
+  }
+  NrFrame F{};
+  F.rows = (int)sensor->rows; F.cols = (int)sensor->cols;
+  F.fx = sensor->fx; F.fy = sensor->fy; F.cx = sensor->cx; F.cy = sensor->cy;
+  for (int k = 0; k < 16; ++k) F.pose[k] = pose[k];
+  F.min_depth = pr->min_depth; F.max_depth = pr->max_depth; F.dist_max = pr->dist_max;
+  F.normal_cos = pr->normal_cos; F.lambda = pr->lambda_reg; F.damping = pr->damping; F.pcg_tol = pr->pcg_tol;
+  DM_HIP(hipMemcpyAsync(g->depth, depth, P * 4, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_nr_target_vert, dim3(grid_of(P)), dim3(256), 0, st, g->depth, F, g->tv);
+  hipLaunchKernelGGL(k_nr_target_norm, dim3(grid_of(P)), dim3(256), 0, st, g->tv, F, g->tn);
+  const int nn = g->nn, ne = nn * kNK;
+  PcgState S = g->pcg;
+  const unsigned pg = (unsigned)S.nwg;
+  for (int it = 0; it < pr->gn_iterations; ++it) {
+    hipLaunchKernelGGL(k_nr_blend, dim3(grid_of(g->nv)), dim3(256), 0, st, g->verts, g->normals, g->nv,
+                       g->node_of_vertex, g->vnn, g->vnn_w, g->dq, g->pv, g->pn);
+    hipLaunchKernelGGL(k_nr_data, dim3(grid_of(g->nv)), dim3(256), 0, st, g->pv, g->pn, g->nv, g->tv, g->tn,
+                       g->depth, F, g->vdata);
+    hipLaunchKernelGGL(k_nr_arap, dim3(grid_of(ne)), dim3(256), 0, st, g->T, g->verts, g->node_vid, g->nbr,
+                       g->nbr_w, nn, g->edata);
+    hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 4)), dim3(256), 0, st, g->blk_a, g->blk_b, g->cstart,
+                       g->codes, g->nb, g->vdata, g->vnn, g->vnn_wn, g->node_of_vertex, g->nbr, g->edata, F, g->H,
+                       g->rhs, g->Minv);
+    if (it + 1 == pr->gn_iterations)
+      hipLaunchKernelGGL(k_nr_stats, dim3(1), dim3(1024), 0, st, g->vdata, g->nv, g->edata, ne, pr->lambda_reg,
+                         g->stats);
+    hipLaunchKernelGGL(k_pcg_init, dim3(pg), dim3(kPcgThreads), 0, st, S, g->rhs, g->Minv, nn);
+    hipLaunchKernelGGL(k_pcg_init_done, dim3(1), dim3(kPcgThreads), 0, st, S);
+    for (int k = 0; k < pr->pcg_iterations; ++k) {
+      hipLaunchKernelGGL(k_pcg_spmv, dim3(pg), dim3(kPcgThreads), 0, st, S, g->H, g->row_start, g->row_ent,
+                         g->blk_a, g->blk_b, nn);
+      hipLaunchKernelGGL(k_pcg_update, dim3(pg), dim3(kPcgThreads), 0, st, S, g->Minv, nn, k);
+      hipLaunchKernelGGL(k_pcg_dir, dim3(pg), dim3(kPcgThreads), 0, st, S, nn, k, pr->pcg_tol);
+    }
+    hipLaunchKernelGGL(k_nr_update, dim3(grid_of(nn)), dim3(256), 0, st, S.x, g->T, nn, g->dq);
+    DM_HIP(hipGetLastError());
+  }
+  double hs[3] = {0, 0, 0};
+  int flags[2] = {0, 0};
+  if (pr->gn_iterations > 0) {
+    DM_HIP(hipMemcpyAsync(hs, g->stats, 24, hipMemcpyDeviceToHost, st));
+    DM_HIP(hipMemcpyAsync(flags, S.flag, 8, hipMemcpyDeviceToHost, st));
+  }
+  DM_HIP(hipStreamSynchronize(st));
+  if (stats) { stats[0] = hs[0]; stats[1] = hs[1]; stats[2] = hs[2]; stats[3] = flags[1]; }
+  return DM_OK;
+}
+
+}  // extern "C"

@@ -119,11 +119,217 @@ class TsdfVolume:
             return xyz[:n]
         return xyz[:n], sdf[:n], rgb[:n], w[:n]
 
+    # tsdfVolume::ExtractMesh (tsdf.cu:551-560): 200 000 triangles at most
+    def ExtractMesh(self, lower_corner=None, upper_corner=None, max_triangles=200000):
+        return MeshExtractor(max_triangles).ExtractMesh(self, lower_corner, upper_corner)
+
     def counters(self):
         out = np.zeros(5, np.int64)
         N.check(self._lib.dm_volume_counters(self._h, N.ptr(out)))
         return dict(allocated=int(out[0]), dropped_bucket=int(out[1]), dropped_pool=int(out[2]),
                     dropped_range=int(out[3]), dropped_keyset=int(out[4]))
+
+
+class Mesh:
+    """geometry::Mesh (include/geometry/mesh/mesh.h:25-66) as host arrays.
+
+    triangles (T, 3, 3) vertex positions, colors (T, 3, 3) rgb in [0, 1],
+    edge_ids (T, 3) uint64 grid-edge id of each vertex (equal id = same
+    vertex).  num_triangles_ is the total the extraction found; at most
+    max_triangles_ of them are held."""
+
+    def __init__(self, triangles, colors, edge_ids, num_triangles, max_triangles):
+        self.triangles = triangles
+        self.colors = colors
+        self.edge_ids = edge_ids
+        self.num_triangles_ = int(num_triangles)
+        self.max_triangles_ = int(max_triangles)
+
+    def __len__(self):
+        return len(self.triangles)
+
+    def weld(self):
+        """Indexed mesh: (vertices (V,3), colors (V,3), faces (T,3) int64),
+        vertices numbered in order of first appearance."""
+        flat = self.edge_ids.reshape(-1)
+        uniq, first, inv = np.unique(flat, return_index=True, return_inverse=True)
+        order = np.argsort(first, kind="stable")
+        rank = np.empty_like(order)
+        rank[order] = np.arange(len(order))
+        faces = rank[inv].reshape(-1, 3)
+        src = first[order]
+        return (self.triangles.reshape(-1, 3)[src], self.colors.reshape(-1, 3)[src], faces)
+
+    def SaveToFile(self, filename):
+        """Mesh::SaveToFile (mesh.cu:38-72): "v x y z r g b" per triangle
+        vertex, then "f 3i+1 3i+2 3i+3"; returns num_triangles_."""
+        n = len(self.triangles)
+        with open(filename, "w") as f:
+            P = self.triangles.reshape(-1, 3)
+            Cc = self.colors.reshape(-1, 3)
+            f.writelines(f"v {p[0]:g} {p[1]:g} {p[2]:g} {c[0]:g} {c[1]:g} {c[2]:g}\n" for p, c in zip(P, Cc))
+            f.writelines(f"f {i} {i + 1} {i + 2}\n" for i in range(1, 3 * n + 1, 3))
+        return self.num_triangles_
+
+    def SaveWelded(self, filename):
+        """OBJ with shared vertices (one "v" line per distinct edge id)."""
+        V, Cv, F = self.weld()
+        with open(filename, "w") as f:
+            f.writelines(f"v {p[0]:g} {p[1]:g} {p[2]:g} {c[0]:g} {c[1]:g} {c[2]:g}\n" for p, c in zip(V, Cv))
+            f.writelines(f"f {a + 1} {b + 1} {c + 1}\n" for a, b, c in F)
+        return len(F)
+
+
+class MeshExtractor:
+    """geometry::MeshExtractor (include/geometry/mesh/meshExtractor/meshExtractor.h:20-59):
+    marching cubes over the allocated blocks of a volume on the GPU."""
+
+    def __init__(self, max_triangles=200000):
+        self._lib = N.lib()
+        self.max_triangles = int(max_triangles)
+        self._mesh = None
+
+    # MeshExtractor::Init (meshExtractor.cu:11-15); the voxel size is the volume's
+    def Init(self, max_triangles, voxel_size=None):
+        self.max_triangles = int(max_triangles)
+
+    def ExtractMesh(self, volume, lower_corner=None, upper_corner=None):
+        """MeshExtractor::ExtractMesh (meshExtractor.cu:391-402).  Without a
+        box every allocated block is meshed."""
+        box = lower_corner is not None
+        lo = np.ascontiguousarray(lower_corner if box else [0, 0, 0], np.float64)
+        hi = np.ascontiguousarray(upper_corner if box else [0, 0, 0], np.float64)
+        n = C.c_int64()
+        h = volume._h
+        N.check(self._lib.dm_mesh_extract(h, N.ptr(lo) if box else None, N.ptr(hi) if box else None, 0,
+                                          None, None, None, C.byref(n)))
+        m = min(n.value, self.max_triangles)
+        pos = np.zeros((max(m, 1), 3, 3), np.float32)
+        col = np.zeros((max(m, 1), 3, 3), np.float32)
+        eid = np.zeros((max(m, 1), 3), np.uint64)
+        N.check(self._lib.dm_mesh_extract(h, N.ptr(lo) if box else None, N.ptr(hi) if box else None, m,
+                                          N.ptr(pos), N.ptr(col), N.ptr(eid), C.byref(n)))
+        self._mesh = Mesh(pos[:m], col[:m], eid[:m], n.value, self.max_triangles)
+        return self._mesh
+
+    def GetMesh(self):
+        return self._mesh
+
+
+# ---------------------------------------------------------------- non-rigid
+NONRIGID_DEFAULTS = dict(gn_iterations=5, pcg_iterations=40, pcg_tol=1e-6, lambda_reg=10.0, damping=1e-3,
+                         dist_max=0.05, normal_cos=0.5, min_depth=0.1, max_depth=5.0)
+
+
+class DefGraph:
+    """geometry::defGraph (include/geometry/defGraph/defGraph.h:45-87) on the GPU:
+    nodes sampled from a canonical mesh, 4 nearest nodes per node and per
+    vertex with weights exp(-d^2/dgw^2), node transforms (canonical ->
+    current).  `init` = defGraph::init + defGraphUpdateNodes."""
+
+    def __init__(self, vertices, normals, node_radius=0.025, dgw=0.05, max_nodes=4096):
+        self._lib = N.lib()
+        self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self.normals = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        pr = N.dm_graph_props(node_radius=node_radius, dgw=dgw, max_nodes=max_nodes)
+        h = C.c_void_p()
+        N.check(self._lib.dm_graph_create(N.ptr(self.vertices), N.ptr(self.normals), len(self.vertices),
+                                          C.byref(pr), C.byref(h)))
+        self._h = h
+        n = C.c_int32()
+        N.check(self._lib.dm_graph_num_nodes(h, C.byref(n)))
+        self.nodeNum = n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dm_graph_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def nodes(self):
+        """(vertex ids (N,), neighbour ids (N,4), neighbour weights (N,4))."""
+        vid = np.zeros(self.nodeNum, np.int32)
+        nbr = np.zeros((self.nodeNum, 4), np.int32)
+        w = np.zeros((self.nodeNum, 4), np.float32)
+        N.check(self._lib.dm_graph_get_nodes(self._h, N.ptr(vid), N.ptr(nbr), N.ptr(w)))
+        return vid, nbr, w
+
+    def vertex_neighbours(self):
+        ids = np.zeros((len(self.vertices), 4), np.int32)
+        w = np.zeros((len(self.vertices), 4), np.float32)
+        N.check(self._lib.dm_graph_get_vertex_nn(self._h, N.ptr(ids), N.ptr(w)))
+        return ids, w
+
+    def num_blocks(self):
+        n = C.c_int32()
+        N.check(self._lib.dm_graph_num_blocks(self._h, C.byref(n)))
+        return n.value
+
+    # defGraph::updateNodesDQ (defGraph.cu:704-730)
+    def set_transforms(self, T):
+        T = np.ascontiguousarray(T, np.float64).reshape(self.nodeNum, 16)
+        N.check(self._lib.dm_graph_set_transforms(self._h, N.ptr(T)))
+
+    def transforms(self):
+        T = np.zeros((self.nodeNum, 4, 4), np.float64)
+        N.check(self._lib.dm_graph_get_transforms(self._h, N.ptr(T)))
+        return T
+
+
+class DqBlender:
+    """blender::dqBlender (include/blender/dqBlender/dqBlender.h:19-83)."""
+
+    def __init__(self, graph=None):
+        self.graph = graph
+
+    def init(self, graph):
+        self.graph = graph
+
+    def blendMesh(self, graph=None):
+        """(warped vertices (V,3), rotated normals (V,3)) of the graph's canonical mesh."""
+        g = graph or self.graph
+        v = np.zeros_like(g.vertices)
+        n = np.zeros_like(g.normals)
+        N.check(g._lib.dm_blend(g._h, N.ptr(v), N.ptr(n)))
+        return v, n
+
+
+class NonRigidICP:
+    """solver::nonRigidICP (include/solver/nonRigidICP/nonRigidICP.h:59-183):
+    Gauss-Newton over the node transforms, point-to-plane data term against a
+    depth frame + ARAP regulariser, block-Jacobi PCG on the GPU."""
+
+    def __init__(self, **props):
+        self.props = dict(NONRIGID_DEFAULTS)
+        self.props.update(props)
+
+    def init(self, graph=None, **props):
+        self.props.update(props)
+
+    def solve(self, graph, depth, sensor, pose, **override):
+        p = dict(self.props)
+        p.update(override)
+        pr = N.dm_nonrigid_props(**p)
+        depth = _f32(depth)
+        pose = np.ascontiguousarray(pose, np.float64)
+        st = np.zeros(4, np.float64)
+        N.check(graph._lib.dm_nonrigid_solve(graph._h, N.ptr(depth), C.byref(_sensor(sensor)), N.ptr(pose),
+                                            C.byref(pr), N.ptr(st)))
+        return dict(residuals=int(st[0]), data_energy=float(st[1]), arap_energy=float(st[2]),
+                    pcg_iterations=int(st[3]))
+
+
+
+def mc_tri_table():
+    """The build's marching-cubes triangle table: 256 packed words."""
+    out = np.zeros(256, np.uint64)
+    N.check(N.lib().dm_mc_tri_table(N.ptr(out)))
+    return out
 
 
 class Session:

@@ -391,3 +391,168 @@ void dms_trajectory(const dms_scene *s, int nframes, uint64_t seed, double *pose
 }
 
 int dms_scene_size(void) { return (int)sizeof(dms_scene); }
+
+/* ------------------------------------------------------------------ */
+/* Deforming object: a bending cylinder                                 */
+/* ------------------------------------------------------------------ */
+/*
+ * Canonical shape: lateral surface of a cylinder of radius 0.15 m, axis
+ * along world y (down), y in [-0.3, 0.3], centred at the origin, plus the
+ * top cap at y = -0.3.  Deformation at frame f: every point moves along x
+ * by b(y, f) = A sin(2 pi f / 40) s(y)^2 with A = 0.05 m and
+ * s(y) = (0.3 - y) / 0.6 (0 at the bottom, which stays fixed, 1 at the top).
+ */
+#define CYL_R 0.15
+#define CYL_H2 0.3
+#define CYL_A 0.05
+#define CYL_PERIOD 40.0
+
+static double cyl_bend(double y, double f) {
+  double s = (CYL_H2 - y) / (2.0 * CYL_H2);
+  return CYL_A * sin(2.0 * 3.141592653589793 * f / CYL_PERIOD) * s * s;
+}
+
+/* Ground-truth deformation of n canonical points (xyz triples, FP64). */
+void dms_cylinder_warp(double frame, int64_t n, const double *in, double *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    out[3 * i] = in[3 * i] + cyl_bend(in[3 * i + 1], frame);
+    out[3 * i + 1] = in[3 * i + 1];
+    out[3 * i + 2] = in[3 * i + 2];
+  }
+}
+
+/* Canonical mesh: n_theta x n_y vertices of the lateral surface (index
+ * j * n_theta + i, theta_i = 2 pi i / n_theta, y_j evenly from -0.3 to 0.3)
+ * with outward normals, float32. */
+void dms_cylinder_mesh(int n_theta, int n_y, float *verts, float *normals) {
+  for (int j = 0; j < n_y; ++j) {
+    double y = -CYL_H2 + 2.0 * CYL_H2 * (double)j / (double)(n_y - 1);
+    for (int i = 0; i < n_theta; ++i) {
+      double th = 2.0 * 3.141592653589793 * (double)i / (double)n_theta;
+      size_t k = (size_t)j * n_theta + i;
+      verts[3 * k] = (float)(CYL_R * cos(th));
+      verts[3 * k + 1] = (float)y;
+      verts[3 * k + 2] = (float)(CYL_R * sin(th));
+      normals[3 * k] = (float)cos(th);
+      normals[3 * k + 1] = 0.f;
+      normals[3 * k + 2] = (float)sin(th);
+    }
+  }
+}
+
+/* first hit of the ray o + t d (t > 0) with the deformed cylinder, or -1 */
+static double cyl_cast(const double o[3], const double d[3], double f) {
+  double best = -1.0;
+  /* top cap: plane y = -0.3, disc around (b(-0.3), 0) */
+  if (fabs(d[1]) > 1e-12) {
+    double t = (-CYL_H2 - o[1]) / d[1];
+    if (t > 0) {
+      double x = o[0] + t * d[0] - cyl_bend(-CYL_H2, f), z = o[2] + t * d[2];
+      if (x * x + z * z <= CYL_R * CYL_R) best = t;
+    }
+  }
+  /* lateral surface: g(t) = (x - b(y))^2 + z^2 - r^2, marched in 2 mm steps
+   * over the slab |y| <= 0.3 and the bounding box, then bisected */
+  double t0 = 1e-6, t1 = 20.0;
+  const double lo[3] = {-CYL_R - CYL_A, -CYL_H2, -CYL_R}, hi[3] = {CYL_R + CYL_A, CYL_H2, CYL_R};
+  for (int k = 0; k < 3; ++k) {
+    if (fabs(d[k]) < 1e-12) {
+      if (o[k] < lo[k] || o[k] > hi[k]) return best;
+      continue;
+    }
+    double a = (lo[k] - o[k]) / d[k], b = (hi[k] - o[k]) / d[k];
+    if (a > b) { double tmp = a; a = b; b = tmp; }
+    if (a > t0) t0 = a;
+    if (b < t1) t1 = b;
+  }
+  if (t0 >= t1) return best;
+  const double step = 0.002;
+#define CYL_G(t) ({ double x_ = o[0] + (t) * d[0], y_ = o[1] + (t) * d[1], z_ = o[2] + (t) * d[2]; \
+                    double u_ = x_ - cyl_bend(y_, f); u_ * u_ + z_ * z_ - CYL_R * CYL_R; })
+  double ta = t0, ga = CYL_G(ta);
+  for (double tb = t0 + step;; tb += step) {
+    if (tb > t1) tb = t1;
+    double gb = CYL_G(tb);
+    if (ga > 0 && gb <= 0) {
+      double a = ta, b = tb;
+      for (int it = 0; it < 60; ++it) {
+        double m = 0.5 * (a + b);
+        if (CYL_G(m) > 0) a = m; else b = m;
+      }
+      double t = 0.5 * (a + b);
+      double y = o[1] + t * d[1];
+      if (y >= -CYL_H2 && y <= CYL_H2 && (best < 0 || t < best)) best = t;
+      break;
+    }
+    if (tb >= t1) break;
+    ta = tb;
+    ga = gb;
+  }
+#undef CYL_G
+  return best;
+}
+
+/* Depth frame of the cylinder deformed at frame f, seen from pose; the same
+ * camera model and noise as dms_render. */
+void dms_cylinder_render(double f, const double pose[16], int rows, int cols, double fx, double fy, double cx,
+                         double cy, uint64_t frame_seed, int noise, float *depth) {
+  const double o[3] = {pose[3], pose[7], pose[11]};
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int v = 0; v < rows; ++v) {
+    for (int u = 0; u < cols; ++u) {
+      int i = v * cols + u;
+      double dc[3] = {((double)u - cx) / fx, ((double)v - cy) / fy, 1.0};
+      double d[3];
+      for (int r = 0; r < 3; ++r) d[r] = pose[4 * r + 0] * dc[0] + pose[4 * r + 1] * dc[1] + pose[4 * r + 2] * dc[2];
+      double z = cyl_cast(o, d, f);
+      if (z < 0) z = 0.0;
+      if (noise && z > 0.0) {
+        dms_rng g = {hash3(frame_seed, 0xD5EULL, (uint64_t)i), 0};
+        if (rng_u(&g) < 0.01) {
+          z = 0.0;
+        } else {
+          double sig = 0.0012 + 0.0019 * (z - 0.4) * (z - 0.4);
+          z += sig * rng_normal(&g);
+          if (z < 0.0) z = 0.0;
+        }
+      }
+      depth[i] = (float)z;
+    }
+  }
+}
+
+/* Camera poses for the cylinder sequence: 0.9 m from the axis, 0.3 m above
+ * the centre, looking at the origin, with a seeded drift of at most 5 mm and
+ * 0.3 degrees per frame around that nominal pose (bounded to 2 cm). */
+void dms_cylinder_trajectory(int nframes, uint64_t seed, double *poses) {
+  dms_rng g = {seed ^ 0xC711DEULL, 0};
+  double base[3] = {0.0, -0.45, -0.85};
+  double p[3] = {base[0], base[1], base[2]};
+  double yaw = 0.0, pitch = 0.0;
+  for (int f = 0; f < nframes; ++f) {
+    if (f > 0) {
+      for (int k = 0; k < 3; ++k) {
+        double stepk = 0.0025 * rng_normal(&g);
+        if (stepk > 0.005) stepk = 0.005;
+        if (stepk < -0.005) stepk = -0.005;
+        p[k] += stepk - 0.05 * (p[k] - base[k]);
+        if (p[k] - base[k] > 0.02) p[k] = base[k] + 0.02;
+        if (p[k] - base[k] < -0.02) p[k] = base[k] - 0.02;
+      }
+      yaw += 0.003 * rng_normal(&g) - 0.05 * yaw;
+      pitch += 0.003 * rng_normal(&g) - 0.05 * pitch;
+    }
+    double tgt[3] = {0.0, 0.0, 0.0};
+    double fw0[3] = {tgt[0] - p[0], tgt[1] - p[1], tgt[2] - p[2]};
+    double hor = sqrt(fw0[0] * fw0[0] + fw0[2] * fw0[2]);
+    double y0 = atan2(fw0[0], fw0[2]) + yaw, p0 = atan2(fw0[1], hor) + pitch;
+    double fw[3] = {sin(y0) * cos(p0), sin(p0), cos(y0) * cos(p0)};
+    double R[9];
+    look_rotation(fw, R);
+    double *P = poses + 16 * (size_t)f;
+    P[0] = R[0]; P[1] = R[1]; P[2] = R[2];  P[3] = p[0];
+    P[4] = R[3]; P[5] = R[4]; P[6] = R[5];  P[7] = p[1];
+    P[8] = R[6]; P[9] = R[7]; P[10] = R[8]; P[11] = p[2];
+    P[12] = 0;   P[13] = 0;   P[14] = 0;    P[15] = 1;
+  }
+}

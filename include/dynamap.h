@@ -113,6 +113,28 @@ DM_API int dm_volume_get_blocks(dm_volume *vol, int32_t cap, int32_t *xyz, float
  * full per-frame key set. */
 DM_API int dm_volume_counters(dm_volume *vol, int64_t out[5]);
 
+/* ------------------------------------------------------------------ */
+/* Mesh: geometry::MeshExtractor / geometry::Mesh                      */
+/* ------------------------------------------------------------------ */
+/* The marching-cubes triangle table (meshExtractor/lookupTables.h triTable):
+ * one word per cube index, edge k of the triangle list in bits 4k..4k+3,
+ * 0xF ends the list. */
+DM_API int dm_mc_tri_table(uint64_t out[256]);
+/* MeshExtractor::ExtractMesh, meshExtractor.cu:364-402, tsdf.cu:551-560.
+ * Every voxel of every allocated block is a cell (cell centre = voxel
+ * centre, corners at +-voxel/2); with box_lo/box_hi (both or neither) only
+ * cells whose centre lies in the box.  Triangles are ordered by hash entry
+ * (bucket, slot), then cell (x-major within the block), then table order.
+ * *ntri receives the total; the first min(total, cap) triangles are written:
+ * pos and col 9 floats per triangle (3 vertices; col = rgb / 255 of the cell's
+ * centre voxel, meshExtractor.cu:193), edge_ids 3 per triangle (the grid edge
+ * of each vertex: axis << 60 | (x + 2^19) << 40 | (y + 2^19) << 20 | (z + 2^19)
+ * of its lower end, in corner units where corner c lies at (c - 1/2) * voxel).
+ * Equal ids are the same vertex, so welding is a merge by id.  Any output
+ * may be NULL; cap 0 counts only. */
+DM_API int dm_mesh_extract(dm_volume *vol, const double box_lo[3], const double box_hi[3], int64_t cap, float *pos,
+                           float *col, uint64_t *edge_ids, int64_t *ntri);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,7 +7,7 @@
 //   DynaMap::tsdf::HashTable       include/dataStructure/hashTable/hashTable.h:32-141
 //   DynaMap::solver::gnSolver      include/solver/gnSolver/gnSolver.h:57-100
 //   DynaMap::builder               include/builder/builder.h:49-79
-//   DynaMap::geometry::MeshExtractor / Mesh
+//   DynaMap::geometry::MeshExtractor / Mesh (extraction and OBJ output)
 //                                  include/geometry/mesh/meshExtractor/meshExtractor.h:20-59,
 //                                  include/geometry/mesh/mesh.h:25-66
 // Differences: device memory is owned by the objects (the reference uses
@@ -15,7 +15,9 @@
 // Eigen matrices are replaced by row-major double[16] poses; failures throw
 // DynaMap::Error carrying the DM_E_* code.
 #pragma once
+#include <algorithm>
 #include <array>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -58,6 +60,41 @@ struct rgbdImage {
   const uint8_t *rgb = nullptr;   // rows*cols*3, may be null
   rgbdSensor sensor{};
 };
+
+namespace geometry {
+
+// Vertex / Triangle (include/geometry/geom_Types.h): position and colour
+struct Vertex {
+  float3 position;
+  float3 color;
+};
+struct Triangle {
+  Vertex v0, v1, v2;
+};
+
+// Mesh (mesh.h:25-66) as a host triangle soup.  edge_ids[3 * i + j] names
+// the grid edge vertex j of triangle i lies on; equal ids are one vertex.
+class Mesh {
+ public:
+  std::vector<Triangle> triangles_;
+  std::vector<uint64_t> edge_ids_;
+  unsigned int num_triangles_ = 0;  // found by the extraction
+  unsigned int max_triangles_ = 0;  // held at most
+  // Mesh::SaveToFile (mesh.cu:38-72): "v x y z r g b" per triangle vertex,
+  // then one face "f 3i+1 3i+2 3i+3" per triangle; returns num_triangles_
+  int SaveToFile(const std::string &filename) const {
+    std::ofstream f(filename);
+    if (!f) throw Error(DM_E_INVALID, "cannot open " + filename);
+    for (const Triangle &t : triangles_)
+      for (const Vertex *v : {&t.v0, &t.v1, &t.v2})
+        f << "v " << v->position.x << " " << v->position.y << " " << v->position.z << " " << v->color.x << " "
+          << v->color.y << " " << v->color.z << "\n";
+    for (size_t i = 1; i <= triangles_.size() * 3; i += 3) f << "f " << i << " " << i + 1 << " " << i + 2 << "\n";
+    return (int)num_triangles_;
+  }
+};
+
+}  // namespace geometry
 
 namespace tsdf {
 
@@ -123,6 +160,10 @@ class tsdfVolume {
   }
   tsdfProperties GetProperties() const { return Properties; }
   dm_volume *handle() { return v_; }
+  // tsdfVolume::ExtractMesh (tsdf.cu:551-560): at most 200 000 triangles
+  geometry::Mesh ExtractMesh(const float3 &lower_corner, const float3 &upper_corner);
+  // every allocated block
+  geometry::Mesh ExtractMesh();
 
   tsdfProperties Properties{};
 
@@ -131,6 +172,69 @@ class tsdfVolume {
   bool owned_ = false;
 };
 
+}  // namespace tsdf
+
+namespace geometry {
+
+// MeshExtractor (meshExtractor.h:20-59): marching cubes on the GPU over the
+// allocated blocks; with a box, over the cells whose centre lies in it.
+class MeshExtractor {
+ public:
+  // MeshExtractor::Init (meshExtractor.cu:11-15); the voxel size is the volume's
+  void Init(unsigned int max_triangles, float voxel_size = 0.f) {
+    (void)voxel_size;
+    max_triangles_ = max_triangles;
+  }
+  void Free() { mesh_ = Mesh(); }
+  // MeshExtractor::ExtractMesh (meshExtractor.cu:391-402)
+  void ExtractMesh(tsdf::tsdfVolume *volume, float3 lower_corner, float3 upper_corner) {
+    const double lo[3] = {lower_corner.x, lower_corner.y, lower_corner.z};
+    const double hi[3] = {upper_corner.x, upper_corner.y, upper_corner.z};
+    run(volume, lo, hi);
+  }
+  void ExtractMesh(tsdf::tsdfVolume *volume) { run(volume, nullptr, nullptr); }
+  Mesh GetMesh() const { return mesh_; }
+
+ private:
+  void run(tsdf::tsdfVolume *volume, const double *lo, const double *hi) {
+    int64_t total = 0;
+    check(dm_mesh_extract(volume->handle(), lo, hi, 0, nullptr, nullptr, nullptr, &total));
+    const int64_t m = std::min<int64_t>(total, max_triangles_);
+    std::vector<float> pos((size_t)m * 9), col((size_t)m * 9);
+    mesh_ = Mesh();
+    mesh_.edge_ids_.resize((size_t)m * 3);
+    check(dm_mesh_extract(volume->handle(), lo, hi, m, pos.data(), col.data(), mesh_.edge_ids_.data(), &total));
+    mesh_.triangles_.resize((size_t)m);
+    for (int64_t i = 0; i < m; ++i) {
+      Vertex *v[3] = {&mesh_.triangles_[i].v0, &mesh_.triangles_[i].v1, &mesh_.triangles_[i].v2};
+      for (int j = 0; j < 3; ++j) {
+        const float *p = &pos[(size_t)i * 9 + 3 * j], *c = &col[(size_t)i * 9 + 3 * j];
+        v[j]->position = float3{p[0], p[1], p[2]};
+        v[j]->color = float3{c[0], c[1], c[2]};
+      }
+    }
+    mesh_.num_triangles_ = (unsigned int)total;
+    mesh_.max_triangles_ = max_triangles_;
+  }
+  unsigned int max_triangles_ = 200000;
+  Mesh mesh_;
+};
+
+}  // namespace geometry
+
+namespace tsdf {
+inline geometry::Mesh tsdfVolume::ExtractMesh(const float3 &lower_corner, const float3 &upper_corner) {
+  geometry::MeshExtractor x;
+  x.Init(200000, Properties.voxel_size);
+  x.ExtractMesh(this, lower_corner, upper_corner);
+  return x.GetMesh();
+}
+inline geometry::Mesh tsdfVolume::ExtractMesh() {
+  geometry::MeshExtractor x;
+  x.Init(200000, Properties.voxel_size);
+  x.ExtractMesh(this);
+  return x.GetMesh();
+}
 }  // namespace tsdf
 
 namespace solver {

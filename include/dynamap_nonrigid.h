@@ -1,0 +1,83 @@
+/*
+ * dynamap_nonrigid.h -- C ABI of the non-rigid path: deformation graph
+ * (geometry::defGraph, include/geometry/defGraph/defGraph.h:45-87),
+ * dual-quaternion blending (blender::dqBlender,
+ * include/blender/dqBlender/dqBlender.h:19-83) and the non-rigid
+ * Gauss-Newton solver (solver::nonRigidICP,
+ * include/solver/nonRigidICP/nonRigidICP.h:59-183).  Paths are relative to
+ * the reference repository root.
+ *
+ * Poses and node transforms are row-major 4x4 doubles; a node transform maps
+ * canonical coordinates to the current frame (world).
+ */
+#ifndef DYNAMAP_NONRIGID_H
+#define DYNAMAP_NONRIGID_H
+#include "dynamap.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dm_graph dm_graph;
+
+/* Graph construction parameters.  The reference fixes NODE_NUM = 273,
+ * KNN = 4 and dgw = 0.015 at compile time (include/DynaMap.h:20-25); here
+ * the node count follows from the sampling radius. */
+typedef struct dm_graph_props {
+  float node_radius; /* grid cell edge of the node sampling, metres */
+  float dgw;         /* weight scale: w = exp(-d^2 / dgw^2) (defGraph.cu:131,410) */
+  int32_t max_nodes; /* capacity, 5 .. 32768; more sampled nodes is an error */
+} dm_graph_props;
+
+/* nonRigidICP solver parameters (the reference's solver::Properties,
+ * gnSolver.h:47-55, plus the terms of the build's formulation). */
+typedef struct dm_nonrigid_props {
+  int32_t gn_iterations;  /* Gauss-Newton iterations per call */
+  int32_t pcg_iterations; /* PCG iteration cap per Gauss-Newton iteration */
+  double pcg_tol;         /* PCG stops when r.z <= tol^2 r0.z0 */
+  double lambda_reg;      /* ARAP weight */
+  double damping;         /* added to every diagonal entry of the normal equations */
+  double dist_max;        /* data association distance gate, metres */
+  double normal_cos;      /* data association gate on |n_src . n_tgt| */
+  double min_depth, max_depth;
+} dm_nonrigid_props;
+
+/* defGraph::init + defGraphUpdateNodes (defGraph.cu:15-69, :483-545): nodes
+ * sampled from the canonical mesh (per grid cell of edge node_radius, the
+ * vertex closest to the cell centre), 4 nearest nodes of every node and of
+ * every vertex with weights exp(-d^2/dgw^2), node transforms = identity.
+ * vertices, normals: nv x 3 floats (host). */
+DM_API int dm_graph_create(const float *vertices, const float *normals, int64_t nv, const dm_graph_props *props,
+                           dm_graph **out);
+/* defGraph::Free (defGraph.cu:86-101) */
+DM_API int dm_graph_destroy(dm_graph *g);
+/* defGraph::nodeNum */
+DM_API int dm_graph_num_nodes(dm_graph *g, int32_t *n);
+/* defGraphNode::vertexID / nIds / nWeights (defGraph.h:35-43): per node its
+ * vertex index, its 4 nearest nodes (nearest first) and their weights.  Any
+ * output may be NULL. */
+DM_API int dm_graph_get_nodes(dm_graph *g, int32_t *vertex_ids, int32_t *nbr_ids, float *nbr_weights);
+/* defGraph::visibleNodeIds / visibleNWeights (defGraph.h:71-73): the 4
+ * nearest nodes of every vertex and their weights (nv x 4 each). */
+DM_API int dm_graph_get_vertex_nn(dm_graph *g, int32_t *ids, float *weights);
+/* Number of 6x6 blocks of the normal equations' upper triangle. */
+DM_API int dm_graph_num_blocks(dm_graph *g, int32_t *nb);
+/* defGraph::updateNodesDQ (defGraph.cu:704-730): node transforms (n x 16). */
+DM_API int dm_graph_set_transforms(dm_graph *g, const double *T);
+DM_API int dm_graph_get_transforms(dm_graph *g, double *T);
+/* dqBlender::blendMesh (dqBlender.cu:187-218): the canonical mesh warped by
+ * the blended node dual quaternions; out_normals may be NULL. */
+DM_API int dm_blend(dm_graph *g, float *out_vertices, float *out_normals);
+/* nonRigidICP::solve (nonRigidICP.cu:557-704): gn_iterations Gauss-Newton
+ * steps of the point-to-plane data term against one depth frame (host,
+ * rows*cols metres) seen from pose (camera-to-world) plus the ARAP term,
+ * updating the node transforms.  stats (may be NULL): [0] data residuals,
+ * [1] data energy and [2] ARAP energy at the last linearisation, [3] PCG
+ * iterations of the last step. */
+DM_API int dm_nonrigid_solve(dm_graph *g, const float *depth, const dm_sensor *sensor, const double pose[16],
+                             const dm_nonrigid_props *props, double stats[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DYNAMAP_NONRIGID_H */

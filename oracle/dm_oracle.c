@@ -893,3 +893,4 @@ OR_EXPORT void or_icp_track(const float *const *sv, const float *const *sn, cons
   }
   memcpy(Tinc, T, sizeof T);
 }
+#include "dm_oracle_mc.inc"

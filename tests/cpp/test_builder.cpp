@@ -22,6 +22,17 @@ int main() {
   const float centre = d[60 * 160 + 80];
   std::printf("blocks %d centre depth %.5f\n", nb, centre);
   if (nb <= 0 || std::fabs(centre - 1.0f) > 2e-3f) return 1;
+  // mesh of the wall in a box (tsdfVolume::ExtractMesh): vertices at z = 1
+  geometry::Mesh mesh = vol.ExtractMesh(float3{-0.2f, -0.2f, 0.5f}, float3{0.2f, 0.2f, 1.5f});
+  std::printf("triangles %u\n", mesh.num_triangles_);
+  if (mesh.num_triangles_ < 100 || mesh.triangles_.size() != mesh.num_triangles_) return 5;
+  for (const auto &t : mesh.triangles_)
+    if (std::fabs(t.v0.position.z - 1.0f) > 2e-3f || std::fabs(t.v2.position.z - 1.0f) > 2e-3f) return 6;
+  geometry::MeshExtractor mx;
+  mx.Init(10);
+  mx.ExtractMesh(&vol);
+  if (mx.GetMesh().triangles_.size() != 10 || mx.GetMesh().num_triangles_ < mesh.num_triangles_) return 7;
+  if (mesh.SaveToFile("/tmp/dm_wall_mesh.obj") != (int)mesh.num_triangles_) return 8;
   // builder: two identical frames of a single plane: the motion along the
   // plane normal (z) is constrained and must be zero; in-plane sliding is not
   // observable by point-to-plane ICP and is only kept small by the damping

@@ -233,3 +233,157 @@ class OraclePipeline:
         d, v, n, c = self.vol.raycast(Tf if self.bits == 32 else self.T, self.sensor)
         self.model = model_maps(v, n, intensity(c) if c is not None else np.zeros(d.shape, np.float32))
         return self.T.copy()
+
+
+def mc_extract(vol, box=None):
+    """Oracle marching cubes: (positions (T,3,3), colours (T,3,3), edge ids (T,3))."""
+    L = vol.L
+    L.or_mc_extract.restype = C.c_int64
+    L.or_mc_extract.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, C.c_int64]
+    lo = np.asarray(box[0] if box else [0, 0, 0], np.float64)
+    hi = np.asarray(box[1] if box else [0, 0, 0], np.float64)
+    n = L.or_mc_extract(vol.h, 1 if box else 0, P(lo), P(hi), None, None, None, 0)
+    pos = np.zeros((max(n, 1), 3, 3), np.float32)
+    col = np.zeros((max(n, 1), 3, 3), np.float32)
+    eid = np.zeros((max(n, 1), 3), np.uint64)
+    L.or_mc_extract(vol.h, 1 if box else 0, P(lo), P(hi), P(pos), P(col), P(eid), n)
+    return pos[:n], col[:n], eid[:n]
+
+
+def mc_table(bits=32):
+    out = np.zeros(256, np.uint64)
+    load(bits).or_mc_table(P(out))
+    return out
+
+
+def weld(eid):
+    """Indexed mesh from edge ids: vertices numbered by first appearance."""
+    flat = eid.reshape(-1)
+    uniq, first, inv = np.unique(flat, return_index=True, return_inverse=True)
+    order = np.argsort(first)
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    return rank[inv].reshape(eid.shape), len(uniq)
+
+
+def _set_blocks(self, xyz, sdf, w, rgb=None):
+    """Write whole blocks in the given order (test hook): xyz (n,3) int32,
+    sdf (n,512), w (n,512) uint8, rgb (n,512,3) uint8 or None."""
+    L = self.L
+    L.or_set_blocks.restype = _i
+    L.or_set_blocks.argtypes = [_vp, _i, _vp, _vp, _vp, _vp]
+    xyz = np.ascontiguousarray(xyz, np.int32)
+    sdf = np.ascontiguousarray(sdf, np.float64)
+    w = np.ascontiguousarray(w, np.uint8)
+    rgb = None if rgb is None else np.ascontiguousarray(rgb, np.uint8)
+    return L.or_set_blocks(self.h, len(xyz), P(xyz), P(sdf), P(w), P(rgb))
+
+
+OracleVolume.set_blocks = _set_blocks
+
+
+# ---------------------------------------------------------------- non-rigid
+class or_nr_props(C.Structure):
+    _fields_ = [("gn_iterations", C.c_int), ("pcg_iterations", C.c_int), ("pcg_tol", C.c_double),
+                ("lambda_reg", C.c_double), ("damping", C.c_double), ("dist_max", C.c_double),
+                ("normal_cos", C.c_double), ("min_depth", C.c_double), ("max_depth", C.c_double)]
+
+
+NR_DEFAULTS = dict(gn_iterations=5, pcg_iterations=40, pcg_tol=1e-6, lambda_reg=10.0, damping=1e-3,
+                   dist_max=0.05, normal_cos=0.5, min_depth=0.1, max_depth=5.0)
+
+
+def nr_props(**kw):
+    p = dict(NR_DEFAULTS)
+    p.update(kw)
+    return or_nr_props(**p)
+
+
+class OracleGraph:
+    """Deformation graph + blender + non-rigid solver of the CPU oracle."""
+
+    def __init__(self, verts, normals, node_radius=0.025, dgw=0.05, max_nodes=4096, bits=64):
+        L = self.L = load(bits)
+        L.or_graph_create.restype = _vp
+        L.or_graph_create.argtypes = [_vp, _vp, C.c_int64, _d, _d, _i, _vp]
+        self.verts = np.ascontiguousarray(verts, np.float32)
+        self.normals = np.ascontiguousarray(normals, np.float32)
+        st = np.zeros(1, np.int32)
+        self.h = L.or_graph_create(P(self.verts), P(self.normals), len(self.verts), node_radius, dgw, max_nodes,
+                                   P(st))
+        self.status = int(st[0])
+        L.or_graph_num_nodes.restype = _i
+        L.or_graph_num_nodes.argtypes = [_vp]
+        self.nn = L.or_graph_num_nodes(self.h)
+        for n, a in (("or_graph_nodes", [_vp, _vp, _vp, _vp]), ("or_graph_vertex_nn", [_vp, _vp, _vp]),
+                     ("or_graph_set_transforms", [_vp, _vp]), ("or_graph_get_transforms", [_vp, _vp]),
+                     ("or_blend", [_vp, _vp, _vp]), ("or_graph_destroy", [_vp]),
+                     ("or_dq_of_transform", [_vp, _vp])):
+            f = getattr(L, n)
+            f.restype, f.argtypes = None, a
+        L.or_nonrigid_solve.restype = _i
+        L.or_nonrigid_solve.argtypes = [_vp, _vp, _i, _i, _d, _d, _d, _d, _vp, C.POINTER(or_nr_props), _vp]
+        L.or_nonrigid_system.restype = _i
+        L.or_nonrigid_system.argtypes = [_vp, _vp, _i, _i, _d, _d, _d, _d, _vp, C.POINTER(or_nr_props), _vp, _vp,
+                                         _vp, _vp]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.or_graph_destroy(self.h)
+            self.h = None
+
+    def nodes(self):
+        vid = np.zeros(self.nn, np.int32)
+        nbr = np.zeros((self.nn, 4), np.int32)
+        w = np.zeros((self.nn, 4), np.float64)
+        self.L.or_graph_nodes(self.h, P(vid), P(nbr), P(w))
+        return vid, nbr, w
+
+    def vertex_nn(self):
+        ids = np.zeros((len(self.verts), 4), np.int32)
+        w = np.zeros((len(self.verts), 4), np.float64)
+        self.L.or_graph_vertex_nn(self.h, P(ids), P(w))
+        return ids, w
+
+    def set_transforms(self, T):
+        T = np.ascontiguousarray(T, np.float64).reshape(self.nn, 16)
+        self.L.or_graph_set_transforms(self.h, P(T))
+
+    def transforms(self):
+        T = np.zeros((self.nn, 4, 4))
+        self.L.or_graph_get_transforms(self.h, P(T))
+        return T
+
+    def blend(self):
+        v = np.zeros_like(self.verts)
+        n = np.zeros_like(self.normals)
+        self.L.or_blend(self.h, P(v), P(n))
+        return v, n
+
+    def solve(self, depth, sensor, pose, **kw):
+        depth = np.ascontiguousarray(depth, np.float32)
+        pose = np.ascontiguousarray(pose, np.float64)
+        st = np.zeros(4)
+        pr = nr_props(**kw)
+        self.L.or_nonrigid_solve(self.h, P(depth), *_intr(sensor), P(pose), C.byref(pr), P(st))
+        return st
+
+    def system(self, depth, sensor, pose, **kw):
+        n = 6 * self.nn
+        H = np.zeros((n, n))
+        b = np.zeros(n)
+        x = np.zeros(n)
+        e = np.zeros(4)
+        depth = np.ascontiguousarray(depth, np.float32)
+        pose = np.ascontiguousarray(pose, np.float64)
+        pr = nr_props(**kw)
+        self.L.or_nonrigid_system(self.h, P(depth), *_intr(sensor), P(pose), C.byref(pr), P(H), P(b), P(x), P(e))
+        return H, b, x, e
+
+
+def dq_of_transform(T, bits=64):
+    out = np.zeros(8)
+    L = load(bits)
+    L.or_dq_of_transform.restype, L.or_dq_of_transform.argtypes = None, [_vp, _vp]
+    L.or_dq_of_transform(P(np.ascontiguousarray(T, np.float64)), P(out))
+    return out
