@@ -26,6 +26,7 @@ Reported:
   cpu_baseline  the FP64 CPU oracle pipeline on a bounded sample
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -50,9 +51,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-frame", action="store_true")
     ap.add_argument("--no-poisson", action="store_true")
-    ap.add_argument("--copy-calibration-mib", type=int, default=0,
-                    help="after the run, fill then copy a buffer of this size (known HBM bytes for "
-                         "scaling the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-nonrigid", action="store_true")
+    ap.add_argument("--nonrigid-frames", type=int, default=20)
     return ap.parse_args()
 
 
@@ -62,6 +62,61 @@ def log(*a):
 
 def ate(A, B):
     return float(np.sqrt(np.mean(np.sum((A[..., :3, 3] - B[..., :3, 3]) ** 2, -1))))
+
+
+def bench_nonrigid(dm, nframes, with_cpu):
+    """Deforming-cylinder workload: 200 000-vertex canonical mesh, 2.5 cm
+    nodes, 640x480 depth of a sinusoidally bending cylinder; one solve
+    (5 Gauss-Newton iterations, block-Jacobi PCG) per frame."""
+    N = dm._native
+    S = N.synth()
+    nt, ny = 500, 400
+    V = np.zeros((nt * ny, 3), np.float32)
+    Nn = np.zeros_like(V)
+    S.dms_cylinder_mesh(nt, ny, N.ptr(V), N.ptr(Nn))
+    sens = dict(rows=480, cols=640, fx=525.0, fy=525.0, cx=319.5, cy=239.5)
+    poses = np.zeros((nframes, 16))
+    S.dms_cylinder_trajectory(nframes, 20261202 + 3000, N.ptr(poses))
+    frames = []
+    for f in range(nframes):
+        d = np.zeros((480, 640), np.float32)
+        S.dms_cylinder_render(float(f), N.ptr(np.ascontiguousarray(poses[f])), 480, 640, 525.0, 525.0, 319.5,
+                              239.5, 20261202 + 3000 + f, 1, N.ptr(d))
+        frames.append(d)
+    t0 = time.time()
+    g = dm.DefGraph(V, Nn, 0.025, 0.05)
+    t_graph = time.time() - t0
+    solver = dm.NonRigidICP()
+    solver.solve(g, frames[0], sens, poses[0].reshape(4, 4))  # capture
+    g.set_transforms(np.tile(np.eye(4), (g.nodeNum, 1, 1)))
+    t0 = time.time()
+    for f in range(nframes):
+        st = solver.solve(g, frames[f], sens, poses[f].reshape(4, 4))
+    el = time.time() - t0
+    vb, _ = dm.DqBlender(g).blendMesh()
+    f = nframes - 1
+    b = 0.05 * np.sin(2 * np.pi * f / 40) * ((0.3 - vb[:, 1]) / 0.6) ** 2
+    err = np.abs(np.sqrt((vb[:, 0] - b) ** 2 + vb[:, 2] ** 2) - 0.15)[V[:, 2] < -0.07]
+    res = dict(workload="bending cylinder, 200k-vertex mesh, 640x480, 5 GN iterations x 40 PCG per frame",
+               frames=nframes, fps=nframes / el, ms_per_frame=1000 * el / nframes, nodes=g.nodeNum,
+               blocks=g.num_blocks(), graph_build_ms=1000 * t_graph, surface_err_mm=1000 * float(err.mean()),
+               last_stats=st)
+    log(f"non-rigid: {res['fps']:.1f} frames/s ({res['ms_per_frame']:.2f} ms), nodes {g.nodeNum}, "
+        f"surface err {res['surface_err_mm']:.4f} mm")
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        o = O.OracleGraph(V, Nn, 0.025, 0.05, bits=64)
+        t0 = time.time()
+        k = 0
+        while k < nframes and (k < 2 or time.time() - t0 < 5.0):
+            o.solve(frames[k], sens, poses[k].reshape(4, 4))
+            k += 1
+        res["cpu_baseline"] = dict(value=k / (time.time() - t0), unit="frames/s", kind="port",
+                                   cores=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                                   sample=f"FP64 oracle non-rigid solve, frames 0..{k - 1}")
+    g.close()
+    return res
 
 
 def main():
@@ -171,24 +226,38 @@ def main():
         s_idx = ns // 2
         name, depth, rgb, gt = traces[s_idx]
         w = configs.WORKLOADS[name]
-        pl = O.OraclePipeline(w["tsdf"], w["icp"], w["sensor"], gt[0], bits=64)
-        t1 = time.time()
-        poses_o = []
-        f = 0
-        while f < nf and (time.time() - t1 < args.cpu_seconds or f < 3):
-            poses_o.append(pl.process_frame(depth[f], rgb[f] if rgb is not None else None))
-            f += 1
-        el = time.time() - t1
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        L64 = O.load(64)
+        L64.or_get_threads.restype = C.c_int
+        cores = int(L64.or_get_threads())
+
+        def oracle_run(threads, budget_s, min_frames):
+            L64.or_set_threads(threads)
+            pl = O.OraclePipeline(w["tsdf"], w["icp"], w["sensor"], gt[0], bits=64)
+            t1 = time.time()
+            ps = []
+            while len(ps) < nf and (time.time() - t1 < budget_s or len(ps) < min_frames):
+                k = len(ps)
+                ps.append(pl.process_frame(depth[k], rgb[k] if rgb is not None else None))
+            return ps, time.time() - t1
+
+        poses_o, el = oracle_run(cores, args.cpu_seconds, 3)
+        f = len(poses_o)
+        p1, el1 = oracle_run(1, min(10.0, args.cpu_seconds), 2)
+        L64.or_set_threads(cores)
         cpu = dict(value=f / el, unit="frames/s", cores=cores, kind="port",
                    sample=f"FP64 oracle pipeline (track, integrate, raycast), session {s_idx} "
-                          f"(640x480, 8 mm), frames 0..{f - 1}, {el:.1f} s")
+                          f"(640x480, 8 mm), frames 0..{f - 1}, {el:.1f} s",
+                   value_1_thread=len(p1) / el1, sample_1_thread=f"frames 0..{len(p1) - 1}, {el1:.1f} s")
         # GPU (per-frame device path) poses for the same frames vs the oracle
         s1 = dm.Session(w["tsdf"], w["icp"], w["sensor"], gt[0])
         g = np.stack([s1.process_frame(depth[k], rgb[k]) for k in range(f)])
         s1.close()
         out["ate_mm"]["gpu_vs_fp64_oracle"] = 1000 * ate(g, np.stack(poses_o))
         out["ate_mm"]["fp64_oracle_frames"] = f
+
+    # ---- non-rigid workload: bending cylinder, 200k-vertex mesh, 640x480
+    if not args.no_nonrigid:
+        out["nonrigid"] = bench_nonrigid(dm, args.nonrigid_frames, not args.no_cpu)
 
     res = {
         "metric": "aggregate frames/s + p50/p99 frame latency, batched vs per-frame, ATE≤1mm",
@@ -206,15 +275,6 @@ def main():
         "config": {"workload": "serve-16: 8 x 320x240 @ 2 cm + 8 x 640x480 @ 8 mm", "sessions": ns,
                    "frames_per_session": nf, "icp_iterations": [10, 5, 4], "batch": "one frame per session"},
     }
-    if args.copy_calibration_mib:
-        import torch
-        nb = args.copy_calibration_mib << 20
-        a = torch.empty(nb, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        a.fill_(7)       # writes nb bytes
-        b.copy_(a)       # reads nb bytes, writes nb bytes
-        torch.cuda.synchronize()
-        log(f"copy calibration: fill + copy of {nb} bytes")
     res.update(out)
     res["roofline"] = roofline
     res["cpu_baseline"] = cpu

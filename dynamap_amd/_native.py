@@ -73,7 +73,11 @@ PROTOTYPES = {
     "dm_volume_get_entries": (C.c_int, [_vp, _vp]),
     "dm_volume_get_blocks": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp, _P(C.c_int32)]),
     "dm_volume_counters": (C.c_int, [_vp, _vp]),
+    "dm_volume_snapshot": (C.c_int, [_vp, _vp, C.c_uint64, _P(C.c_uint64)]),
+    "dm_volume_restore": (C.c_int, [_vp, _vp, C.c_uint64]),
     "dm_mc_tri_table": (C.c_int, [_vp]),
+    "dm_depth_bilateral": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, C.c_int32, C.c_float, C.c_float]),
+    "dm_depth_pyramid": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_int32, _vp]),
     "dm_graph_create": (C.c_int, [_vp, _vp, C.c_int64, _P(dm_graph_props), _P(_vp)]),
     "dm_graph_destroy": (C.c_int, [_vp]),
     "dm_graph_num_nodes": (C.c_int, [_vp, _P(C.c_int32)]),
@@ -91,6 +95,7 @@ PROTOTYPES = {
     "dm_session_process_frame": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp]),
     "dm_session_process_frame_dev": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp]),
     "dm_session_get_pose": (C.c_int, [_vp, _vp]),
+    "dm_session_set_depth_filter": (C.c_int, [_vp, C.c_int32, C.c_float, C.c_float]),
     "dm_session_source_maps": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp]),
     "dm_session_set_model_maps": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp]),
     "dm_session_icp_system": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp]),

@@ -9,6 +9,8 @@ Seeds are 20261202 + 1000 * workload + session.
                       geometric + photometric ICP
   serve-16          : 8 x rigid-320x240-2cm + 8 x rigid-640x480-8mm, 300 frames each,
                       Poisson arrivals at 30 Hz per session
+  large-640x480-4mm : 8 x 8 x 3 m multi-room (12 boxes, 20 spheres), 4 mm voxels,
+                      2^22 buckets, 2^21 blocks, trunc 2 cm, geometric + photometric ICP
 """
 import numpy as np
 
@@ -25,6 +27,8 @@ TSDF_2CM = dict(voxel_size=0.02, num_buckets=1 << 16, bucket_size=10, num_blocks
                 max_sdf_weight=64, truncation_distance=0.06, max_sensor_depth=5.0, min_sensor_depth=0.3)
 TSDF_8MM = dict(voxel_size=0.008, num_buckets=1 << 20, bucket_size=10, num_blocks=1 << 18, block_size=8,
                 max_sdf_weight=64, truncation_distance=0.03, max_sensor_depth=5.0, min_sensor_depth=0.3)
+TSDF_4MM = dict(voxel_size=0.004, num_buckets=1 << 22, bucket_size=10, num_blocks=1 << 21, block_size=8,
+                max_sdf_weight=64, truncation_distance=0.02, max_sensor_depth=5.0, min_sensor_depth=0.3)
 ICP_DEPTH = dict(lvl_iterations=(10, 5, 4), min_depth=0.3, max_depth=5.0, dist_max=0.1, photo_weight=0.0,
                  regularization=0.02, max_increment=0.3)
 ICP_RGBD = dict(ICP_DEPTH, photo_weight=1e-3)
@@ -86,6 +90,7 @@ class Scene:
 WORKLOADS = {
     "rigid-320x240-2cm": dict(sensor=SENSOR_320, tsdf=TSDF_2CM, icp=ICP_DEPTH, scene=1, rgb=False, wid=1),
     "rigid-640x480-8mm": dict(sensor=SENSOR_640, tsdf=TSDF_8MM, icp=ICP_RGBD, scene=1, rgb=True, wid=2),
+    "large-640x480-4mm": dict(sensor=SENSOR_640, tsdf=TSDF_4MM, icp=ICP_RGBD, scene=5, rgb=True, wid=5),
 }
 
 

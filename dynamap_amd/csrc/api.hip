@@ -46,7 +46,7 @@ size_t session_bytes(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   const size_t P = (size_t)rows * cols;
   const size_t nc = (size_t)new_cap(p.num_blocks);
   size_t b = (size_t)p.num_buckets * bucket_stride(p.bucket_size);
-  b += (size_t)p.num_blocks * (kBlockVox * 8 + 16 + 2);
+  b += (size_t)p.num_blocks * (kBlockVox * 8 + 16 + 2) + (size_t)kCoarseWords * 4;
   b += (size_t)pow2_at_least((int)(2 * nc)) * 8 + nc * (8 + 4 * 4) + (size_t)p.num_buckets * 4;
   size_t maps = 0;
   for (int l = 0; l < kLevels; ++l) maps += (size_t)(rows >> l) * (cols >> l) * 16 * 4;
@@ -102,6 +102,7 @@ int Sess::init(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   DM_TRY(dalloc(&h.block_key, (size_t)p.num_blocks));
   DM_TRY(dalloc(&h.blk_near, (size_t)p.num_blocks));
   DM_TRY(dalloc(&h.blk_ndil, (size_t)p.num_blocks));
+  DM_TRY(dalloc(&h.coarse, (size_t)kCoarseWords));
   DM_TRY(dalloc(&h.dset, (size_t)dsz));
   DM_TRY(dalloc(&h.newkeys, (size_t)h.newcap, false));
   DM_TRY(dalloc(&h.newdslot, (size_t)h.newcap, false));
@@ -219,21 +220,6 @@ int Sess::stage_frame(const float *host_depth, const uint8_t *host_rgb) {
 // ---------------------------------------------------------------------------
 // tracking orchestration shared by the session and the tracking test hooks
 
-// Sum the per-workgroup records in the same fixed order as k_icp_solve.
-static void host_sum_partials(const std::vector<float> &part, int nwg, double *sums) {
-  for (int k = 0; k < 30; ++k) {
-    double lanes[8];
-    for (int q = 0; q < 8; ++q) {
-      double a = 0.0;
-      for (int w = q; w < nwg; w += 8) a += (double)part[(size_t)w * kIcpVals + k];
-      lanes[q] = a;
-    }
-    double a = 0.0;
-    for (int q = 0; q < 8; ++q) a += lanes[q];
-    sums[k] = a;
-  }
-}
-
 static int total_iters(const DmView &h) { return h.iters[0] + h.iters[1] + h.iters[2]; }
 
 // Run the level loop.  Device solve: back-to-back kernels.  Host solve:
@@ -246,8 +232,7 @@ static int run_icp(Sess &S, bool host_solve) {
     for (int l = kLevels - 1; l >= 0; --l)
       for (int it = 1; it <= S.h.iters[l]; ++it) {
         ++done;
-        launch_icp_reduce(L, l, it);
-        launch_icp_solve(L, l, it);
+        launch_icp_reduce(L, l, it, true);
       }
     DM_HIP(hipGetLastError());
     return DM_OK;
@@ -262,16 +247,16 @@ static int run_icp(Sess &S, bool host_solve) {
   std::vector<float> part;
   for (int l = kLevels - 1; l >= 0; --l) {
     const int P = (S.h.rows >> l) * (S.h.cols >> l);
-    const int nwg = (P + 255) / 256;
+    const int nwg = icp_wg_count(P);
     part.resize((size_t)nwg * kIcpVals);
     for (int it = 1; it <= S.h.iters[l]; ++it) {
       ++done;
-      launch_icp_reduce(L, l, it);
+      launch_icp_reduce(L, l, it, false);
       DM_HIP(hipMemcpyAsync(part.data(), S.h.partials, part.size() * sizeof(float), hipMemcpyDeviceToHost,
                             S.stream));
       DM_HIP(hipStreamSynchronize(S.stream));
       double sums[30], A[36], b[6], xi[6], E[16];
-      host_sum_partials(part, nwg, sums);
+      icp_sum_records(part.data(), nwg, sums);
       sym6_from_upper(sums, A);
       for (int k = 0; k < 6; ++k) b[k] = sums[21 + k];
       icp_step(A, b, it, (double)S.h.lambda, xi);
@@ -303,6 +288,8 @@ static int run_icp(Sess &S, bool host_solve) {
 
 static int process_frame(Sess &S, bool host_solve, double *pose_out) {
   const Launch L = S.launch();
+  if (S.filt_r > 0)
+    launch_bilateral(S.h.depth, S.d_depth_f, S.h.rows, S.h.cols, S.filt_r, S.filt_sd, S.filt_ss, S.stream);
   launch_pose_begin(L);
   launch_source_maps(L);
   DM_TRY(run_icp(S, host_solve));
@@ -316,6 +303,36 @@ static int process_frame(Sess &S, bool host_solve, double *pose_out) {
   if (pose_out) memcpy(pose_out, T, sizeof T);
   S.frames++;
   return DM_OK;
+}
+
+// ---------------------------------------------------------------- snapshot
+struct SnapHeader {
+  char magic[8];
+  uint32_t version, hdr_bytes;
+  dm_tsdf_props props;
+  uint32_t max_rows, max_cols, rows, cols;
+  float fx, fy, cx, cy;
+  int32_t num_alloc, iters[3];
+  float icp_f[6];  // icp dmin, dmax, dist_max, w_photo, lambda, max_inc
+  int32_t ctr[C_COUNT];
+  uint64_t payload;
+};
+static const char kSnapMagic[8] = {'D', 'M', 'S', 'N', 'A', 'P', '0', '1'};
+
+// device sections of a volume / session state, in file order
+static std::vector<std::pair<void *, size_t>> snap_sections(const Sess &S, int nalloc) {
+  const DmView &h = S.h;
+  std::vector<std::pair<void *, size_t>> v = {
+      {h.buckets, (size_t)h.num_buckets * h.bstride}, {h.block_key, (size_t)nalloc * sizeof(int4)},
+      {h.vox, (size_t)nalloc * kBlockVox * 8},        {h.blk_near, (size_t)nalloc},
+      {h.blk_ndil, (size_t)nalloc},                  {h.coarse, (size_t)kCoarseWords * 4},
+      {h.posed, (size_t)PD_COUNT * 8},               {h.posef, (size_t)PF_COUNT * 4}};
+  for (int l = 0; l < kLevels; ++l) {
+    const size_t Pl = (size_t)(h.rows >> l) * (h.cols >> l);
+    v.push_back({h.tvert[l], Pl * 16});
+    v.push_back({h.tnorm[l], Pl * 16});
+  }
+  return v;
 }
 
 }  // namespace dm
@@ -484,6 +501,90 @@ int dm_volume_counters(dm_volume *vol, int64_t out[5]) {
   return DM_OK;
 }
 
+// Snapshot of the whole reconstruction state: hash table, allocated blocks,
+// block flags, pose state and the model maps.  buf NULL: *size only.
+int dm_volume_snapshot(dm_volume *vol, void *buf, uint64_t cap, uint64_t *size) {
+  if (!vol || !size) { set_error("null argument"); return DM_E_INVALID; }
+  Sess &S = *vol;
+  SnapHeader H{};
+  memcpy(H.magic, kSnapMagic, 8);
+  H.version = 1;
+  H.hdr_bytes = sizeof(SnapHeader);
+  H.props = S.props;
+  H.max_rows = S.max_rows; H.max_cols = S.max_cols; H.rows = S.h.rows; H.cols = S.h.cols;
+  H.fx = S.h.fx; H.fy = S.h.fy; H.cx = S.h.cx; H.cy = S.h.cy;
+  DM_HIP(hipMemcpyAsync(H.ctr, S.h.ctr, sizeof H.ctr, hipMemcpyDeviceToHost, S.stream));
+  DM_HIP(hipStreamSynchronize(S.stream));
+  H.num_alloc = H.ctr[C_NUM_ALLOC];
+  for (int l = 0; l < 3; ++l) H.iters[l] = S.h.iters[l];
+  const float f6[6] = {S.h.icp_dmin, S.h.icp_dmax, S.h.dist_max, S.h.w_photo, S.h.lambda, S.h.max_inc};
+  memcpy(H.icp_f, f6, sizeof f6);
+  const auto secs = snap_sections(S, H.num_alloc);
+  uint64_t total = sizeof(SnapHeader);
+  for (const auto &x : secs) total += x.second;
+  H.payload = total - sizeof(SnapHeader);
+  *size = total;
+  if (!buf) return DM_OK;
+  if (cap < total) { set_error("dm_volume_snapshot: buffer too small"); return DM_E_CAPACITY; }
+  uint8_t *o = (uint8_t *)buf;
+  memcpy(o, &H, sizeof H);
+  o += sizeof H;
+  for (const auto &x : secs) {
+    DM_HIP(hipMemcpyAsync(o, x.first, x.second, hipMemcpyDeviceToHost, S.stream));
+    o += x.second;
+  }
+  DM_HIP(hipStreamSynchronize(S.stream));
+  return DM_OK;
+}
+
+// Restore a snapshot into a volume / session created with the same
+// properties and frame size; the per-frame scratch is reset.
+int dm_volume_restore(dm_volume *vol, const void *buf, uint64_t size) {
+  if (!vol || !buf || size < sizeof(SnapHeader)) { set_error("invalid argument"); return DM_E_INVALID; }
+  Sess &S = *vol;
+  SnapHeader H;
+  memcpy(&H, buf, sizeof H);
+  if (memcmp(H.magic, kSnapMagic, 8) != 0 || H.version != 1 || H.hdr_bytes != sizeof(SnapHeader)) {
+    set_error("dm_volume_restore: not a snapshot of this format");
+    return DM_E_INVALID;
+  }
+  if (memcmp(&H.props, &S.props, sizeof(dm_tsdf_props)) != 0 || H.max_rows != S.max_rows ||
+      H.max_cols != S.max_cols || H.num_alloc < 0 || H.num_alloc > S.props.num_blocks ||
+      H.rows > S.max_rows || H.cols > S.max_cols) {
+    set_error("dm_volume_restore: properties or frame size differ from the snapshot's");
+    return DM_E_INVALID;
+  }
+  S.h.rows = (int)H.rows; S.h.cols = (int)H.cols;
+  S.h.fx = H.fx; S.h.fy = H.fy; S.h.cx = H.cx; S.h.cy = H.cy;
+  const auto secs = snap_sections(S, H.num_alloc);
+  uint64_t total = sizeof(SnapHeader);
+  for (const auto &x : secs) total += x.second;
+  if (size != total || H.payload != total - sizeof(SnapHeader)) {
+    set_error("dm_volume_restore: size does not match the snapshot header");
+    return DM_E_INVALID;
+  }
+  const uint8_t *p = (const uint8_t *)buf + sizeof H;
+  for (const auto &x : secs) {
+    DM_HIP(hipMemcpyAsync(x.first, p, x.second, hipMemcpyHostToDevice, S.stream));
+    p += x.second;
+  }
+  DM_HIP(hipMemcpyAsync(S.h.ctr, H.ctr, sizeof H.ctr, hipMemcpyHostToDevice, S.stream));
+  // blocks beyond the snapshot's are unused again
+  const size_t rest = (size_t)(S.props.num_blocks - H.num_alloc);
+  if (rest) {
+    DM_HIP(hipMemsetAsync(S.h.vox + (size_t)H.num_alloc * kBlockVox, 0, rest * kBlockVox * 8, S.stream));
+    DM_HIP(hipMemsetAsync(S.h.blk_near + H.num_alloc, 0, rest, S.stream));
+    DM_HIP(hipMemsetAsync(S.h.blk_ndil + H.num_alloc, 0, rest, S.stream));
+  }
+  DM_HIP(hipMemsetAsync(S.h.bucket_head, 0xff, (size_t)S.props.num_buckets * 4, S.stream));
+  for (int l = 0; l < 3; ++l) S.h.iters[l] = H.iters[l];
+  S.h.icp_dmin = H.icp_f[0]; S.h.icp_dmax = H.icp_f[1]; S.h.dist_max = H.icp_f[2];
+  S.h.w_photo = H.icp_f[3]; S.h.lambda = H.icp_f[4]; S.h.max_inc = H.icp_f[5];
+  DM_TRY(S.push_view());
+  DM_HIP(hipStreamSynchronize(S.stream));
+  return DM_OK;
+}
+
 // ---------------------------------------------------------------- session
 static int apply_icp(Sess &S, const dm_icp_props *icp) {
   if (!icp) return DM_OK;
@@ -548,6 +649,21 @@ int dm_session_process_frame_dev(dm_session *s, const float *dev_depth, const ui
   return process_frame(*s, solve_on_host != 0, pose_out);
 }
 
+int dm_session_set_depth_filter(dm_session *s, int32_t radius, float sigma_depth, float sigma_space) {
+  if (!s || radius < 0 || radius > 7 || (radius > 0 && (!(sigma_depth > 0.f) || !(sigma_space > 0.f)))) {
+    set_error("dm_session_set_depth_filter: invalid argument (0 <= radius <= 7, sigmas > 0)");
+    return DM_E_INVALID;
+  }
+  if (radius > 0 && !s->d_depth_f) DM_TRY(s->dalloc(&s->d_depth_f, (size_t)s->max_rows * s->max_cols));
+  s->filt_r = radius;
+  s->filt_sd = sigma_depth;
+  s->filt_ss = sigma_space;
+  s->h.depth_track = radius > 0 ? s->d_depth_f : nullptr;
+  DM_TRY(s->push_view());
+  DM_HIP(hipStreamSynchronize(s->stream));
+  return DM_OK;
+}
+
 int dm_session_get_pose(dm_session *s, double pose[16]) {
   if (!s || !pose) { set_error("null argument"); return DM_E_INVALID; }
   DM_HIP(hipMemcpyAsync(pose, s->h.posed + PD_T, 16 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -605,15 +721,15 @@ int dm_session_icp_system(dm_session *s, int32_t level, const double Tinc[16], d
   DM_HIP(hipMemcpyAsync(s->h.posed + PD_TINC, Tinc, 16 * sizeof(double), hipMemcpyHostToDevice, s->stream));
   int it1 = 1;
   if (s->h.iters[level] < 1) { set_error("level has no iterations"); return DM_E_STATE; }
-  launch_icp_reduce(s->launch(), level, it1);
+  launch_icp_reduce(s->launch(), level, it1, false);
   DM_HIP(hipGetLastError());
   const int P = (s->h.rows >> level) * (s->h.cols >> level);
-  const int nwg = (P + 255) / 256;
+  const int nwg = icp_wg_count(P);
   std::vector<float> part((size_t)nwg * kIcpVals);
   DM_HIP(hipMemcpyAsync(part.data(), s->h.partials, part.size() * 4, hipMemcpyDeviceToHost, s->stream));
   DM_HIP(hipStreamSynchronize(s->stream));
   double sm[30];
-  host_sum_partials(part, nwg, sm);
+  icp_sum_records(part.data(), nwg, sm);
   if (A) sym6_from_upper(sm, A);
   if (b) for (int k = 0; k < 6; ++k) b[k] = sm[21 + k];
   if (sums) { sums[0] = sm[27]; sums[1] = sm[28]; sums[2] = sm[29]; }

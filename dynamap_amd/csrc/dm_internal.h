@@ -16,6 +16,10 @@ constexpr int kBlockVox = 512;             // voxels per block
 constexpr int kKeyRange = 1 << 20;         // |block coordinate| limit of the packed key
 constexpr int kLevels = 3;                 // ICP pyramid levels
 constexpr int kIcpVals = 32;               // per-workgroup ICP partial record (30 used)
+// Coarse empty-space bitmap of the raycast: cells of 8^3 blocks, a window of
+// kCoarse^3 cells centred on block (0, 0, 0); cells outside it count as near.
+constexpr int kCoarse = 64;
+constexpr int kCoarseWords = kCoarse * kCoarse * kCoarse / 32;
 
 // per-volume counters, int32 each
 enum Ctr : int {
@@ -61,6 +65,7 @@ struct DmView {
   int4 *block_key;      // num_blocks block coordinates
   uint8_t *blk_near;    // num_blocks: 1 when an observed voxel lies below the raycast far threshold
   uint8_t *blk_ndil;    // num_blocks: blk_near dilated over the 26 neighbouring blocks
+  uint32_t *coarse;     // kCoarse^3 bits: cell of 8^3 blocks holds a block with blk_ndil set
   uint64_t *dset;       // per-frame new-key set, dset_mask + 1 slots, 0 = empty
   uint64_t *newkeys;    // newcap
   int32_t *newdslot;    // newcap
@@ -71,6 +76,7 @@ struct DmView {
   int32_t *ctr;         // C_COUNT
   // current frame input (device)
   const float *depth;
+  const float *depth_track;  // depth the tracking maps are built from (filtered), or null = depth
   const uint8_t *rgb;   // interleaved r,g,b (may be null)
   // maps: vertex+intensity and normal per level, source (live frame) and
   // target (model raycast); level l is (rows >> l) x (cols >> l)
@@ -105,11 +111,20 @@ void launch_alloc_rays(const Launch &L);  // traversal kernel only (profiling)
 void launch_integrate(const Launch &L);
 void launch_raycast(const Launch &L);
 void launch_source_maps(const Launch &L);
-void launch_icp_reduce(const Launch &L, int level, int it);
-void launch_icp_solve(const Launch &L, int level, int it);
+// ICP linearisation at one level (one FP32 record per workgroup); with solve,
+// a second kernel sums the records and takes the Gauss-Newton step on the
+// device.
+void launch_icp_reduce(const Launch &L, int level, int it, bool solve);
 void launch_pose_begin(const Launch &L);
 void launch_pose_log(const Launch &L, const int *dest, double *log);
+void launch_bilateral(const float *in, float *out, int rows, int cols, int r, float sigma_d, float sigma_s,
+                      hipStream_t st);
 
-int icp_wg_count(int pixels);   // workgroups used by the level-0 ICP reduction
+constexpr int kIcpPix = 1024;   // pixels per ICP reduction workgroup (256 threads x 4)
+__host__ __device__ inline int icp_wg_count(int pixels) { return (pixels + kIcpPix - 1) / kIcpPix; }
+// The FP64 sum of the per-workgroup records in the order the device uses
+// (thread t of 256 takes records t, t+256, ...; xor tree within each
+// wavefront; the four wavefronts as (w0 + w1) + (w2 + w3)).
+void icp_sum_records(const float *part, int nwg, double *sums);
 
 }  // namespace dm

@@ -35,6 +35,9 @@ struct Sess {
   size_t pinned_bytes = 0;
   float *d_ray_depth = nullptr;  // raycast outputs for host readback
   uint8_t *d_ray_rgb = nullptr;
+  float *d_depth_f = nullptr;    // bilateral-filtered depth for tracking (when enabled)
+  int filt_r = 0;                // filter radius, 0 = off
+  float filt_sd = 0.f, filt_ss = 0.f;
   size_t hbm_bytes = 0;          // device bytes owned
   int frames = 0;
 

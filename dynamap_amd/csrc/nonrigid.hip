@@ -381,8 +381,9 @@ struct NrFrame {
 };
 
 // target vertex (camera) per pixel, components rounded to float (w = 1 valid)
-__global__ __launch_bounds__(256) void k_nr_target_vert(const float *__restrict__ depth, NrFrame F,
+__global__ __launch_bounds__(256) void k_nr_target_vert(const float *__restrict__ depth, const NrFrame *Fp,
                                                        float4 *__restrict__ tv) {
+  const NrFrame &F = *Fp;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= F.rows * F.cols) return;
   const float d = depth[i];
@@ -394,8 +395,9 @@ __global__ __launch_bounds__(256) void k_nr_target_vert(const float *__restrict_
 
 // target normal (camera): cross(V[i+2]-V[i-2], V[i+W]-V[i-W]) in FP64 with
 // out-of-range neighbours replaced by the pixel itself; w = 1 valid
-__global__ __launch_bounds__(256) void k_nr_target_norm(const float4 *__restrict__ tv, NrFrame F,
+__global__ __launch_bounds__(256) void k_nr_target_norm(const float4 *__restrict__ tv, const NrFrame *Fp,
                                                        float4 *__restrict__ tn) {
+  const NrFrame &F = *Fp;
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int size = F.rows * F.cols;
   if (i >= size) return;
@@ -422,7 +424,8 @@ __global__ __launch_bounds__(256) void k_nr_target_norm(const float4 *__restrict
 __global__ __launch_bounds__(256) void k_nr_data(const float *__restrict__ pv, const float *__restrict__ pn,
                                                 int64_t nv, const float4 *__restrict__ tv,
                                                 const float4 *__restrict__ tn, const float *__restrict__ depth,
-                                                NrFrame F, float *__restrict__ out) {
+                                                const NrFrame *Fp, float *__restrict__ out) {
+  const NrFrame &F = *Fp;
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (v >= nv) return;
   float *o = out + 8 * v;
@@ -519,12 +522,13 @@ __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk
                                                     int nb, const float *__restrict__ vdata,
                                                     const int *__restrict__ vnn, const float *__restrict__ vnn_wn,
                                                     const int *__restrict__ node_of_vertex, const int *__restrict__ nbr,
-                                                    const double *__restrict__ edata, NrFrame F,
+                                                    const double *__restrict__ edata, const NrFrame *Fp,
                                                     float *__restrict__ H, double *__restrict__ rhs,
                                                     double *__restrict__ Minv) {
   const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (blk >= nb) return;  // uniform per wavefront
+  const NrFrame &F = *Fp;
   const int a = blk_a[blk], b = blk_b[blk];
   const bool diag = a == b;
   double h[36], g[6];
@@ -734,8 +738,9 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const do
 }
 
 // beta = rz_new / rz; p = z + beta p; convergence test
-__global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(PcgState S, int nn, int it, double tol) {
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(PcgState S, int nn, int it, const NrFrame *Fp) {
   if (S.flag[0]) return;
+  const double tol = Fp->pcg_tol;
   __shared__ double l1[1];
   const double rzn = sum_partials(S.part_rz, S.nwg, l1);
   const double beta = rzn / S.rz[it];
@@ -762,8 +767,9 @@ __global__ __launch_bounds__(256) void k_nr_update(const double *__restrict__ x,
 
 // energies at the last linearisation: [0] residual count, [1] data, [2] ARAP
 __global__ __launch_bounds__(1024) void k_nr_stats(const float *__restrict__ vdata, int64_t nv,
-                                                  const double *__restrict__ edata, int ne, double lam,
+                                                  const double *__restrict__ edata, int ne, const NrFrame *Fp,
                                                   double *__restrict__ out) {
+  const double lam = Fp->lambda;
   __shared__ double lds[3][1024 / 64];
   double c = 0.0, ed = 0.0, er = 0.0;
   for (int64_t v = threadIdx.x; v < nv; v += 1024) {
@@ -824,6 +830,10 @@ struct dm_graph {
   float *depth = nullptr;
   float4 *tv = nullptr, *tn = nullptr;
   size_t pixels_cap = 0;
+  NrFrame *d_frame = nullptr, *h_frame = nullptr;  // device record and its pinned host copy
+  // the captured solve and what it was captured for
+  hipGraphExec_t exec = nullptr;
+  int key_gn = -1, key_pcg = -1, key_rows = -1, key_cols = -1;
 
   template <class T> int dalloc(T **p, size_t n) {
     void *q = nullptr;
@@ -834,6 +844,10 @@ struct dm_graph {
   }
   void release() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    if (h_frame) (void)hipHostFree(h_frame);
+    h_frame = nullptr;
     for (void *p : allocs) (void)hipFree(p);
     allocs.clear();
     if (stream) (void)hipStreamDestroy(stream);
@@ -990,8 +1004,47 @@ static int graph_build(dm_graph *g, const float *h_verts, const float *h_normals
   DM_TRY(g->dalloc(&S.part_pq, S.nwg));
   DM_TRY(g->dalloc(&S.part_rz, S.nwg));
   DM_TRY(g->dalloc(&S.flag, 2));
+  DM_TRY(g->dalloc(&g->d_frame, 1));
+  DM_HIP(hipHostMalloc((void **)&g->h_frame, sizeof(NrFrame), hipHostMallocDefault));
   DM_HIP(hipStreamSynchronize(st));
   return DM_OK;
+}
+
+// The kernels of one solve call: target maps, then gn Gauss-Newton
+// iterations of (blend, data, ARAP, assembly, PCG, update).  Everything
+// that changes between calls (pose, gates, depth) is read from device
+// memory, so the sequence is captured once and replayed.
+static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its) {
+  hipStream_t st = g->stream;
+  const size_t P = (size_t)rows * cols;
+  const int nn = g->nn, ne = nn * kNK;
+  const PcgState S = g->pcg;
+  const unsigned pg = (unsigned)S.nwg;
+  const NrFrame *F = g->d_frame;
+  hipLaunchKernelGGL(k_nr_target_vert, dim3(grid_of(P)), dim3(256), 0, st, g->depth, F, g->tv);
+  hipLaunchKernelGGL(k_nr_target_norm, dim3(grid_of(P)), dim3(256), 0, st, g->tv, F, g->tn);
+  for (int it = 0; it < gn; ++it) {
+    hipLaunchKernelGGL(k_nr_blend, dim3(grid_of(g->nv)), dim3(256), 0, st, g->verts, g->normals, g->nv,
+                       g->node_of_vertex, g->vnn, g->vnn_w, g->dq, g->pv, g->pn);
+    hipLaunchKernelGGL(k_nr_data, dim3(grid_of(g->nv)), dim3(256), 0, st, g->pv, g->pn, g->nv, g->tv, g->tn,
+                       g->depth, F, g->vdata);
+    hipLaunchKernelGGL(k_nr_arap, dim3(grid_of(ne)), dim3(256), 0, st, g->T, g->verts, g->node_vid, g->nbr,
+                       g->nbr_w, nn, g->edata);
+    hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 4)), dim3(256), 0, st, g->blk_a, g->blk_b, g->cstart,
+                       g->codes, g->nb, g->vdata, g->vnn, g->vnn_wn, g->node_of_vertex, g->nbr, g->edata, F, g->H,
+                       g->rhs, g->Minv);
+    if (it + 1 == gn)
+      hipLaunchKernelGGL(k_nr_stats, dim3(1), dim3(1024), 0, st, g->vdata, g->nv, g->edata, ne, F, g->stats);
+    hipLaunchKernelGGL(k_pcg_init, dim3(pg), dim3(kPcgThreads), 0, st, S, g->rhs, g->Minv, nn);
+    hipLaunchKernelGGL(k_pcg_init_done, dim3(1), dim3(kPcgThreads), 0, st, S);
+    for (int k = 0; k < pcg_its; ++k) {
+      hipLaunchKernelGGL(k_pcg_spmv, dim3(pg), dim3(kPcgThreads), 0, st, S, g->H, g->row_start, g->row_ent,
+                         g->blk_a, g->blk_b, nn);
+      hipLaunchKernelGGL(k_pcg_update, dim3(pg), dim3(kPcgThreads), 0, st, S, g->Minv, nn, k);
+      hipLaunchKernelGGL(k_pcg_dir, dim3(pg), dim3(kPcgThreads), 0, st, S, nn, k, F);
+    }
+    hipLaunchKernelGGL(k_nr_update, dim3(grid_of(nn)), dim3(256), 0, st, S.x, g->T, nn, g->dq);
+  }
 }
 
 extern "C" {
@@ -1103,47 +1156,39 @@ int dm_nonrigid_solve(dm_graph *g, const float *depth, const dm_sensor *sensor, 
     DM_TRY(g->dalloc(&g->tv, P));
     DM_TRY(g->dalloc(&g->tn, P));
     g->pixels_cap = P;
+    g->key_gn = -1;  // buffers moved: capture again
   }
   if (pr->pcg_iterations + 1 > g->pcg_cap) {
     DM_TRY(g->dalloc(&g->pcg.rz, pr->pcg_iterations + 1));
     g->pcg_cap = pr->pcg_iterations + 1;
+    g->key_gn = -1;
   }
-  NrFrame F{};
+  DM_HIP(hipStreamSynchronize(st));  // the pinned frame record may still be read by the last call
+  NrFrame &F = *g->h_frame;
+  F = NrFrame{};
   F.rows = (int)sensor->rows; F.cols = (int)sensor->cols;
   F.fx = sensor->fx; F.fy = sensor->fy; F.cx = sensor->cx; F.cy = sensor->cy;
   for (int k = 0; k < 16; ++k) F.pose[k] = pose[k];
   F.min_depth = pr->min_depth; F.max_depth = pr->max_depth; F.dist_max = pr->dist_max;
   F.normal_cos = pr->normal_cos; F.lambda = pr->lambda_reg; F.damping = pr->damping; F.pcg_tol = pr->pcg_tol;
+  DM_HIP(hipMemcpyAsync(g->d_frame, g->h_frame, sizeof(NrFrame), hipMemcpyHostToDevice, st));
   DM_HIP(hipMemcpyAsync(g->depth, depth, P * 4, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_nr_target_vert, dim3(grid_of(P)), dim3(256), 0, st, g->depth, F, g->tv);
-  hipLaunchKernelGGL(k_nr_target_norm, dim3(grid_of(P)), dim3(256), 0, st, g->tv, F, g->tn);
-  const int nn = g->nn, ne = nn * kNK;
-  PcgState S = g->pcg;
-  const unsigned pg = (unsigned)S.nwg;
-  for (int it = 0; it < pr->gn_iterations; ++it) {
-    hipLaunchKernelGGL(k_nr_blend, dim3(grid_of(g->nv)), dim3(256), 0, st, g->verts, g->normals, g->nv,
-                       g->node_of_vertex, g->vnn, g->vnn_w, g->dq, g->pv, g->pn);
-    hipLaunchKernelGGL(k_nr_data, dim3(grid_of(g->nv)), dim3(256), 0, st, g->pv, g->pn, g->nv, g->tv, g->tn,
-                       g->depth, F, g->vdata);
-    hipLaunchKernelGGL(k_nr_arap, dim3(grid_of(ne)), dim3(256), 0, st, g->T, g->verts, g->node_vid, g->nbr,
-                       g->nbr_w, nn, g->edata);
-    hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 4)), dim3(256), 0, st, g->blk_a, g->blk_b, g->cstart,
-                       g->codes, g->nb, g->vdata, g->vnn, g->vnn_wn, g->node_of_vertex, g->nbr, g->edata, F, g->H,
-                       g->rhs, g->Minv);
-    if (it + 1 == pr->gn_iterations)
-      hipLaunchKernelGGL(k_nr_stats, dim3(1), dim3(1024), 0, st, g->vdata, g->nv, g->edata, ne, pr->lambda_reg,
-                         g->stats);
-    hipLaunchKernelGGL(k_pcg_init, dim3(pg), dim3(kPcgThreads), 0, st, S, g->rhs, g->Minv, nn);
-    hipLaunchKernelGGL(k_pcg_init_done, dim3(1), dim3(kPcgThreads), 0, st, S);
-    for (int k = 0; k < pr->pcg_iterations; ++k) {
-      hipLaunchKernelGGL(k_pcg_spmv, dim3(pg), dim3(kPcgThreads), 0, st, S, g->H, g->row_start, g->row_ent,
-                         g->blk_a, g->blk_b, nn);
-      hipLaunchKernelGGL(k_pcg_update, dim3(pg), dim3(kPcgThreads), 0, st, S, g->Minv, nn, k);
-      hipLaunchKernelGGL(k_pcg_dir, dim3(pg), dim3(kPcgThreads), 0, st, S, nn, k, pr->pcg_tol);
-    }
-    hipLaunchKernelGGL(k_nr_update, dim3(grid_of(nn)), dim3(256), 0, st, S.x, g->T, nn, g->dq);
-    DM_HIP(hipGetLastError());
+  if (!g->exec || g->key_gn != pr->gn_iterations || g->key_pcg != pr->pcg_iterations || g->key_rows != F.rows ||
+      g->key_cols != F.cols) {
+    if (g->exec) { (void)hipGraphExecDestroy(g->exec); g->exec = nullptr; }
+    hipGraph_t graph = nullptr;
+    DM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    nr_enqueue(g, F.rows, F.cols, pr->gn_iterations, pr->pcg_iterations);
+    hipError_t ce = hipStreamEndCapture(st, &graph);
+    if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+    ce = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ce != hipSuccess) { g->exec = nullptr; return hip_fail(ce, "hipGraphInstantiate"); }
+    g->key_gn = pr->gn_iterations; g->key_pcg = pr->pcg_iterations;
+    g->key_rows = F.rows; g->key_cols = F.cols;
   }
+  DM_HIP(hipGraphLaunch(g->exec, st));
+  const PcgState S = g->pcg;
   double hs[3] = {0, 0, 0};
   int flags[2] = {0, 0};
   if (pr->gn_iterations > 0) {

@@ -67,6 +67,7 @@ int sess_reset(Sess &S, const double init_pose[16]) {
   DM_HIP(hipMemsetAsync(S.h.vox, 0, (size_t)n * kBlockVox * 8, S.stream));
   DM_HIP(hipMemsetAsync(S.h.blk_near, 0, (size_t)n, S.stream));
   DM_HIP(hipMemsetAsync(S.h.blk_ndil, 0, (size_t)n, S.stream));
+  DM_HIP(hipMemsetAsync(S.h.coarse, 0, (size_t)kCoarseWords * 4, S.stream));
   DM_HIP(hipMemsetAsync(S.h.ctr, 0, C_COUNT * 4, S.stream));
   double pd[PD_COUNT] = {0};
   for (int k = 0; k < 16; ++k) {
@@ -160,8 +161,7 @@ static void enqueue_pipeline(const dm_server *srv, const int iters[3]) {
   launch_source_maps(L);
   for (int l = kLevels - 1; l >= 0; --l)
     for (int it = 1; it <= iters[l]; ++it) {
-      launch_icp_reduce(L, l, it);
-      launch_icp_solve(L, l, it);
+      launch_icp_reduce(L, l, it, true);
     }
   launch_alloc(L);
   launch_integrate(L);
@@ -558,8 +558,8 @@ int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage
     for (int k = 0; k < reps; ++k) {
       switch (st) {
         case 0: launch_source_maps(L); break;
-        case 1: launch_icp_reduce(L, 0, 1); break;
-        case 2: launch_icp_solve(L, 0, 1); break;
+        case 1: launch_icp_reduce(L, 0, 1, true); break;
+        case 2: launch_icp_reduce(L, 0, 1, false); break;
         case 3: launch_alloc(L); break;
         case 4: launch_integrate(L); break;
         case 5: launch_raycast(L); break;

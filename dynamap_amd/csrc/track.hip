@@ -30,7 +30,8 @@ __device__ __forceinline__ float intensity_rgb(const uint8_t *rgb, int idx) {
 // depth of level-l pixel j, taken from the level-0 frame at (2^l u, 2^l v)
 __device__ __forceinline__ float level_depth(const DmView &v, int l, int j, int cols_l) {
   const int vv = j / cols_l, uu = j - vv * cols_l;
-  return v.depth[(size_t)(vv << l) * v.cols + (uu << l)];
+  const float *d = v.depth_track ? v.depth_track : v.depth;
+  return d[(size_t)(vv << l) * v.cols + (uu << l)];
 }
 
 // Vertex of level-l pixel j.  D: evaluated in FP64 and rounded to float, so
@@ -96,11 +97,123 @@ __device__ __forceinline__ bool bad3(float4 n) {
 
 constexpr int kVals = 30;
 
-// One Gauss-Newton linearisation at pyramid level `level`
-// (buildNLSKernel, gnSolver.cu:55-195).  K: Project() association, the
-// |cos| > cos(pi/10) normal gate, r = (p - q).n, J = [p x n; n].
-// D: source normal rotated by Tinc, pixel bounds check, |p - q| <= dist_max,
-// depth gate [icp_dmin, icp_dmax], photometric term with weight w_photo.
+// Per-pixel terms of one Gauss-Newton linearisation at pyramid level `level`
+// (buildNLSKernel, gnSolver.cu:55-195) accumulated into vals (upper JtJ 21,
+// Jtr 6, geometric count, r^2, photometric count).  K: Project()
+// association, the |cos| > cos(pi/10) normal gate, r = (p - q).n,
+// J = [p x n; n].  D: source normal rotated by Tinc, pixel bounds check,
+// |p - q| <= dist_max, depth gate [icp_dmin, icp_dmax], photometric term with
+// weight w_photo.
+__device__ __forceinline__ void icp_pixel(const DmView &v, int level, const Intr &s, int i, float *vals) {
+  const float4 V = v.svert[level][i];
+  const float4 N = v.snorm[level][i];
+  const float4 *tv = v.tvert[level];
+  const float4 *tn = v.tnorm[level];
+  bool ok = V.z >= v.icp_dmin && V.z <= v.icp_dmax && !bad3(N);
+  // D: transform, projection and gates in FP64 (the pixel choice and the
+  // inclusion decisions follow the FP64 geometry); residual and Jacobian
+  // in FP32 below
+  const double *Td = v.posed + PD_TINC;
+  double px = 0.0, py = 0.0, pz = 0.0;
+  int pu = -1, pv = -1, j = 0;
+  float4 Q = make_float4(0.f, 0.f, 0.f, 0.f), NT = Q;
+  if (ok) {
+    const double X = V.x, Y = V.y, Z = V.z;
+    px = Td[0] * X + Td[1] * Y + Td[2] * Z + Td[3];
+    py = Td[4] * X + Td[5] * Y + Td[6] * Z + Td[7];
+    pz = Td[8] * X + Td[9] * Y + Td[10] * Z + Td[11];
+    pu = to_i32d(round(((double)s.fx * px) / pz + (double)s.cx));
+    pv = to_i32d(round(((double)s.fy * py) / pz + (double)s.cy));
+    ok = !(pu < 0 || pv < 0 || pu >= s.cols || pv >= s.rows);
+  }
+  if (ok) {
+    j = pv * s.cols + pu;
+    Q = tv[j];
+    NT = tn[j];
+    ok = Q.z >= v.icp_dmin && Q.z <= v.icp_dmax && !bad3(NT);
+  }
+  f3 p = mk(0.f, 0.f, 0.f), pq = p;
+  if (ok) {
+    const double X = N.x, Y = N.y, Z = N.z;
+    const double nrx = Td[0] * X + Td[1] * Y + Td[2] * Z;
+    const double nry = Td[4] * X + Td[5] * Y + Td[6] * Z;
+    const double nrz = Td[8] * X + Td[9] * Y + Td[10] * Z;
+    const double ct = (double)NT.x * nrx + (double)NT.y * nry + (double)NT.z * nrz;
+    const double dqx = px - (double)Q.x, dqy = py - (double)Q.y, dqz = pz - (double)Q.z;
+    ok = fabs(ct) > 0.95105651629515357 && sqrt(dqx * dqx + dqy * dqy + dqz * dqz) <= (double)v.dist_max;
+    p = mk((float)px, (float)py, (float)pz);
+    pq = mk((float)dqx, (float)dqy, (float)dqz);
+  }
+  if (!ok) return;
+  float pvl[kVals];
+  const f3 n = mk(NT.x, NT.y, NT.z);
+  const float r = dot(pq, n);
+  const f3 c = cross(p, n);
+  const float J[6] = {c.x, c.y, c.z, n.x, n.y, n.z};
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = a; b < 6; ++b) pvl[k++] = J[a] * J[b];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) pvl[21 + a] = J[a] * r;
+  pvl[27] = 1.f;
+  pvl[28] = r * r;
+  pvl[29] = 0.f;
+  if (v.w_photo > 0.f && pu >= 1 && pv >= 1 && pu < s.cols - 1 && pv < s.rows - 1) {
+    const float gx = (tv[j + 1].w - tv[j - 1].w) * 0.5f;
+    const float gy = (tv[j + s.cols].w - tv[j - s.cols].w) * 0.5f;
+    const float rp = Q.w - V.w;
+    const float iz = 1.0f / p.z;
+    const f3 G = mk(gx * s.fx * iz, gy * s.fy * iz, -(gx * s.fx * p.x + gy * s.fy * p.y) * iz * iz);
+    const f3 cg = cross(p, G);
+    const float Jp[6] = {cg.x, cg.y, cg.z, G.x, G.y, G.z};
+    const float wp = v.w_photo;
+    int kk = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = a; b < 6; ++b) { pvl[kk] = pvl[kk] + wp * Jp[a] * Jp[b]; ++kk; }
+#pragma unroll
+    for (int a = 0; a < 6; ++a) pvl[21 + a] = pvl[21 + a] + wp * Jp[a] * rp;
+    pvl[29] = 1.f;
+  }
+#pragma unroll
+  for (int q = 0; q < kVals; ++q) vals[q] = vals[q] + pvl[q];
+}
+
+// The damped Gauss-Newton step from the summed records: Tinc <- exp(xi)
+// Tinc.  On the session's last iteration of the frame apply the reference's
+// guard (gnSolver.cu:426-428) and set the frame pose T = Tprev * Tinc.
+__device__ void icp_step_device(const DmView &v, int level, int it, const double *sums) {
+  bool last = it == v.iters[level];
+  for (int l = 0; l < level; ++l) last = last && v.iters[l] == 0;
+  double A[36], b[6], xi[6], E[16];
+  sym6_from_upper(sums, A);
+  for (int k = 0; k < 6; ++k) b[k] = sums[21 + k];
+  icp_step(A, b, it, (double)v.lambda, xi);
+  se3_exp(xi, E);
+  double *Tinc = v.posed + PD_TINC;
+  mat4_mul(E, Tinc, Tinc);
+  for (int k = 0; k < 6; ++k) v.posed[PD_ACC + k] += xi[k];
+  if (last) {
+    const double *acc = v.posed + PD_ACC;
+    const double ssum = acc[0] + acc[1] + acc[2] + acc[3] + acc[4] + acc[5];
+    if (ssum != ssum || isinf(ssum) || ssum >= (double)v.max_inc) {
+      for (int k = 0; k < 16; ++k) Tinc[k] = (k % 5 == 0) ? 1.0 : 0.0;
+      atomicAdd(&v.ctr[C_ICP_FAIL], 1);
+    }
+    double *T = v.posed + PD_T;
+    mat4_mul(v.posed + PD_TPREV, Tinc, T);
+    for (int k = 0; k < 16; ++k) v.posef[PF_T + k] = (float)T[k];
+    rigid_inverse_ff(v.posef + PF_T, v.posef + PF_TINV);
+  }
+  for (int k = 0; k < 16; ++k) v.posef[PF_TINC + k] = (float)Tinc[k];
+}
+
+// One linearisation: every thread accumulates 4 pixels (strided by 256 for
+// coalescing) in FP32; a wavefront xor tree and an LDS stage reduce them to
+// one record per workgroup.
 __global__ __launch_bounds__(256) void k_icp_reduce(const DmView *__restrict__ views, const int *__restrict__ active,
                                                      int level, int it) {
   const int slot_ = active[blockIdx.y];
@@ -109,87 +222,15 @@ __global__ __launch_bounds__(256) void k_icp_reduce(const DmView *__restrict__ v
   if (v.ctr[C_HAS_MODEL] == 0 || it > v.iters[level]) return;
   const Intr s = level_intr(v, level);
   const int P = s.rows * s.cols;
-  const int nwg = (P + 255) >> 8;
+  const int nwg = icp_wg_count(P);
   if ((int)blockIdx.x >= nwg) return;
-  const int i = blockIdx.x * 256 + threadIdx.x;
   float vals[kVals];
 #pragma unroll
   for (int k = 0; k < kVals; ++k) vals[k] = 0.f;
-  if (i < P) {
-    const float4 V = v.svert[level][i];
-    const float4 N = v.snorm[level][i];
-    const float4 *tv = v.tvert[level];
-    const float4 *tn = v.tnorm[level];
-    bool ok = V.z >= v.icp_dmin && V.z <= v.icp_dmax && !bad3(N);
-    // D: transform, projection and gates in FP64 (the pixel choice and the
-    // inclusion decisions follow the FP64 geometry); residual and Jacobian
-    // in FP32 below
-    const double *Td = v.posed + PD_TINC;
-    double px = 0.0, py = 0.0, pz = 0.0;
-    int pu = -1, pv = -1, j = 0;
-    float4 Q = make_float4(0.f, 0.f, 0.f, 0.f), NT = Q;
-    if (ok) {
-      const double X = V.x, Y = V.y, Z = V.z;
-      px = Td[0] * X + Td[1] * Y + Td[2] * Z + Td[3];
-      py = Td[4] * X + Td[5] * Y + Td[6] * Z + Td[7];
-      pz = Td[8] * X + Td[9] * Y + Td[10] * Z + Td[11];
-      pu = to_i32d(round(((double)s.fx * px) / pz + (double)s.cx));
-      pv = to_i32d(round(((double)s.fy * py) / pz + (double)s.cy));
-      ok = !(pu < 0 || pv < 0 || pu >= s.cols || pv >= s.rows);
-    }
-    if (ok) {
-      j = pv * s.cols + pu;
-      Q = tv[j];
-      NT = tn[j];
-      ok = Q.z >= v.icp_dmin && Q.z <= v.icp_dmax && !bad3(NT);
-    }
-    f3 p = mk(0.f, 0.f, 0.f), pq = p;
-    if (ok) {
-      const double X = N.x, Y = N.y, Z = N.z;
-      const double nrx = Td[0] * X + Td[1] * Y + Td[2] * Z;
-      const double nry = Td[4] * X + Td[5] * Y + Td[6] * Z;
-      const double nrz = Td[8] * X + Td[9] * Y + Td[10] * Z;
-      const double ct = (double)NT.x * nrx + (double)NT.y * nry + (double)NT.z * nrz;
-      const double dqx = px - (double)Q.x, dqy = py - (double)Q.y, dqz = pz - (double)Q.z;
-      ok = fabs(ct) > 0.95105651629515357 && sqrt(dqx * dqx + dqy * dqy + dqz * dqz) <= (double)v.dist_max;
-      p = mk((float)px, (float)py, (float)pz);
-      pq = mk((float)dqx, (float)dqy, (float)dqz);
-    }
-    {
-      const f3 n = mk(NT.x, NT.y, NT.z);
-      if (ok) {
-        const float r = dot(pq, n);
-        const f3 c = cross(p, n);
-        const float J[6] = {c.x, c.y, c.z, n.x, n.y, n.z};
-        int k = 0;
 #pragma unroll
-        for (int a = 0; a < 6; ++a)
-#pragma unroll
-          for (int b = a; b < 6; ++b) vals[k++] = J[a] * J[b];
-#pragma unroll
-        for (int a = 0; a < 6; ++a) vals[21 + a] = J[a] * r;
-        vals[27] = 1.f;
-        vals[28] = r * r;
-        if (v.w_photo > 0.f && pu >= 1 && pv >= 1 && pu < s.cols - 1 && pv < s.rows - 1) {
-          const float gx = (tv[j + 1].w - tv[j - 1].w) * 0.5f;
-          const float gy = (tv[j + s.cols].w - tv[j - s.cols].w) * 0.5f;
-          const float rp = Q.w - V.w;
-          const float iz = 1.0f / p.z;
-          const f3 G = mk(gx * s.fx * iz, gy * s.fy * iz, -(gx * s.fx * p.x + gy * s.fy * p.y) * iz * iz);
-          const f3 cg = cross(p, G);
-          const float Jp[6] = {cg.x, cg.y, cg.z, G.x, G.y, G.z};
-          const float wp = v.w_photo;
-          int kk = 0;
-#pragma unroll
-          for (int a = 0; a < 6; ++a)
-#pragma unroll
-            for (int b = a; b < 6; ++b) { vals[kk] = vals[kk] + wp * Jp[a] * Jp[b]; ++kk; }
-#pragma unroll
-          for (int a = 0; a < 6; ++a) vals[21 + a] = vals[21 + a] + wp * Jp[a] * rp;
-          vals[29] = 1.f;
-        }
-      }
-    }
+  for (int q = 0; q < kIcpPix / 256; ++q) {
+    const int i = blockIdx.x * kIcpPix + q * 256 + threadIdx.x;
+    if (i < P) icp_pixel(v, level, s, i, vals);
   }
   // wavefront tree (pairwise) then LDS across the four waves
 #pragma unroll
@@ -212,59 +253,42 @@ __global__ __launch_bounds__(256) void k_icp_reduce(const DmView *__restrict__ v
   }
 }
 
-// Sum the per-workgroup records in FP64 (fixed order), take the damped
-// Gauss-Newton step, compose Tinc <- exp(xi) Tinc.  On the last iteration of
-// the frame apply the reference's guard (gnSolver.cu:426-428) and set the
-// frame pose T = Tprev * Tinc.
+// Sum the records in FP64 in a fixed order (thread t takes records t, t+256,
+// ...; xor tree per wavefront; (w0 + w1) + (w2 + w3); host replica:
+// icp_sum_records) and take the Gauss-Newton step (one workgroup per session).
 __global__ __launch_bounds__(256) void k_icp_solve(const DmView *__restrict__ views, const int *__restrict__ active,
                                                     int level, int it) {
   const int slot_ = active[blockIdx.x];
   if (slot_ < 0) return;
   const DmView &v = views[slot_];
   if (v.ctr[C_HAS_MODEL] == 0 || it > v.iters[level]) return;
-  // the session's final iteration: last one of this level, no finer level
-  bool last = it == v.iters[level];
-  for (int l = 0; l < level; ++l) last = last && v.iters[l] == 0;
-  const int P = (v.rows >> level) * (v.cols >> level);
-  const int nwg = (P + 255) >> 8;
-  __shared__ double part[256];
-  __shared__ double sums[kVals];
-  const int t = threadIdx.x;
-  if (t < 8 * kVals) {
-    const int k = t >> 3, lane = t & 7;
-    double acc = 0.0;
-    for (int w = lane; w < nwg; w += 8) acc += (double)v.partials[(size_t)w * kIcpVals + k];
-    part[t] = acc;
+  const int nwg = icp_wg_count((v.rows >> level) * (v.cols >> level));
+  double acc[kVals];
+#pragma unroll
+  for (int k = 0; k < kVals; ++k) acc[k] = 0.0;
+  for (int w = threadIdx.x; w < nwg; w += 256) {
+    const float *rec = v.partials + (size_t)w * kIcpVals;
+#pragma unroll
+    for (int k = 0; k < kVals; ++k) acc[k] += (double)rec[k];
+  }
+#pragma unroll
+  for (int k = 0; k < kVals; ++k) {
+    double x = acc[k];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    acc[k] = x;
+  }
+  __shared__ double dred[4][kVals];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kVals; ++k) dred[wave][k] = acc[k];
   }
   __syncthreads();
-  if (t < kVals) {
-    double a = 0.0;
-    for (int q = 0; q < 8; ++q) a += part[8 * t + q];
-    sums[t] = a;
-  }
-  __syncthreads();
-  if (t != 0) return;
-  double A[36], b[6], xi[6], E[16];
-  sym6_from_upper(sums, A);
-  for (int k = 0; k < 6; ++k) b[k] = sums[21 + k];
-  icp_step(A, b, it, (double)v.lambda, xi);
-  se3_exp(xi, E);
-  double *Tinc = v.posed + PD_TINC;
-  mat4_mul(E, Tinc, Tinc);
-  for (int k = 0; k < 6; ++k) v.posed[PD_ACC + k] += xi[k];
-  if (last) {
-    const double *acc = v.posed + PD_ACC;
-    const double ssum = acc[0] + acc[1] + acc[2] + acc[3] + acc[4] + acc[5];
-    if (ssum != ssum || isinf(ssum) || ssum >= (double)v.max_inc) {
-      for (int k = 0; k < 16; ++k) Tinc[k] = (k % 5 == 0) ? 1.0 : 0.0;
-      atomicAdd(&v.ctr[C_ICP_FAIL], 1);
-    }
-    double *T = v.posed + PD_T;
-    mat4_mul(v.posed + PD_TPREV, Tinc, T);
-    for (int k = 0; k < 16; ++k) v.posef[PF_T + k] = (float)T[k];
-    rigid_inverse_ff(v.posef + PF_T, v.posef + PF_TINV);
-  }
-  for (int k = 0; k < 16; ++k) v.posef[PF_TINC + k] = (float)Tinc[k];
+  if (threadIdx.x != 0) return;
+  double sums[kVals];
+  for (int k = 0; k < kVals; ++k) sums[k] = (dred[0][k] + dred[1][k]) + (dred[2][k] + dred[3][k]);
+  icp_step_device(v, level, it, sums);
 }
 
 // Start of a frame: Tprev <- T when a model exists, otherwise T <- Tinit;
@@ -299,16 +323,36 @@ void launch_source_maps(const Launch &L) {
                      L.views, L.active);
 }
 
-void launch_icp_reduce(const Launch &L, int level, int it) {
+void launch_icp_reduce(const Launch &L, int level, int it, bool solve) {
   if (L.nact <= 0) return;
   const int P = L.max_pixels >> (2 * level);
-  hipLaunchKernelGGL(k_icp_reduce, dim3((unsigned)((P + 255) / 256 + 1), (unsigned)L.nact), dim3(256), 0, L.stream,
+  hipLaunchKernelGGL(k_icp_reduce, dim3((unsigned)icp_wg_count(P), (unsigned)L.nact), dim3(256), 0, L.stream,
                      L.views, L.active, level, it);
+  if (solve)
+    hipLaunchKernelGGL(k_icp_solve, dim3((unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active, level, it);
 }
 
-void launch_icp_solve(const Launch &L, int level, int it) {
-  if (L.nact <= 0) return;
-  hipLaunchKernelGGL(k_icp_solve, dim3((unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active, level, it);
+void icp_sum_records(const float *part, int nwg, double *sums) {
+  for (int k = 0; k < kVals; ++k) {
+    double lane[256];
+    for (int t = 0; t < 256; ++t) {
+      double a = 0.0;
+      for (int w = t; w < nwg; w += 256) a += (double)part[(size_t)w * kIcpVals + k];
+      lane[t] = a;
+    }
+    double wv[4];
+    for (int q = 0; q < 4; ++q) {
+      double x[64];
+      for (int i = 0; i < 64; ++i) x[i] = lane[64 * q + i];
+      for (int off = 32; off >= 1; off >>= 1) {
+        double y[64];
+        for (int i = 0; i < 64; ++i) y[i] = x[i] + x[i ^ off];
+        for (int i = 0; i < 64; ++i) x[i] = y[i];
+      }
+      wv[q] = x[0];
+    }
+    sums[k] = (wv[0] + wv[1]) + (wv[2] + wv[3]);
+  }
 }
 
 void launch_pose_begin(const Launch &L) {
@@ -331,6 +375,5 @@ void launch_pose_log(const Launch &L, const int *dest, double *log) {
   hipLaunchKernelGGL(k_pose_log, dim3((unsigned)L.nact), dim3(64), 0, L.stream, L.views, L.active, dest, log);
 }
 
-int icp_wg_count(int pixels) { return (pixels + 255) / 256; }
 
 }  // namespace dm

@@ -325,6 +325,18 @@ __device__ __forceinline__ float far_threshold(float trunc) {
   return big + big * (1.0f / 256.0f);
 }
 
+// Coarse cell (8^3 blocks) of a block coordinate, and its bit in the window.
+__device__ __forceinline__ int coarse_of(int b) { return b < 0 ? (b - 7) / 8 : b / 8; }
+__device__ __forceinline__ bool coarse_bit(int cx, int cy, int cz, int &word, int &bit) {
+  const int x = cx + kCoarse / 2, y = cy + kCoarse / 2, z = cz + kCoarse / 2;
+  if ((unsigned)x >= (unsigned)kCoarse || (unsigned)y >= (unsigned)kCoarse || (unsigned)z >= (unsigned)kCoarse)
+    return false;
+  const int idx = (z * kCoarse + y) * kCoarse + x;
+  word = idx >> 5;
+  bit = idx & 31;
+  return true;
+}
+
 __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ views, const int *__restrict__ active) {
   const int slot_ = active[blockIdx.y];
   if (slot_ < 0) return;
@@ -335,6 +347,8 @@ __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ vi
   const float vs = v.voxel_size, tau = v.trunc;
   const double half = 0.5 * (double)vs * 8.0;
   const float far_thr = far_threshold(tau);
+  // the coarse bitmap is rebuilt by k_dilate_near
+  for (int w = blockIdx.x * 256 + threadIdx.x; w < kCoarseWords; w += gridDim.x * 256) v.coarse[w] = 0u;
   int nvis = 0;
   for (int blk = blockIdx.x; blk < nalloc; blk += gridDim.x) {
     if (threadIdx.x == 0) v.blk_ndil[blk] = 0;  // rebuilt by k_dilate_near
@@ -381,6 +395,16 @@ __global__ __launch_bounds__(256) void k_dilate_near(const DmView *__restrict__ 
         for (int dx = -1; dx <= 1; ++dx) {
           const int ptr = (dx | dy | dz) == 0 ? blk : find_block(v, key.x + dx, key.y + dy, key.z + dz);
           if (ptr >= 0) v.blk_ndil[ptr] = 1;
+        }
+    // coarse cells that hold a block within one block of this one (at most 8)
+    const int cx0 = coarse_of(key.x - 1), cx1 = coarse_of(key.x + 1), cy0 = coarse_of(key.y - 1),
+              cy1 = coarse_of(key.y + 1), cz0 = coarse_of(key.z - 1), cz1 = coarse_of(key.z + 1);
+    for (int cz = cz0; cz <= cz1; ++cz)
+      for (int cy = cy0; cy <= cy1; ++cy)
+        for (int cx = cx0; cx <= cx1; ++cx) {
+          int w, b;
+          if (!coarse_bit(cx, cy, cz, w, b)) continue;
+          if (!(v.coarse[w] & (1u << b))) atomicOr(&v.coarse[w], 1u << b);
         }
   }
 }
@@ -431,8 +455,10 @@ __device__ __forceinline__ bool block_far(const DmView &v, int bx, int by, int b
 // p/vs -+ 0.5 half away from zero).  When the block holding the (approximate)
 // centre voxel is allocated and its dilated flag is clear, all those voxels
 // are far; the run continues while they stay within that block's 27
-// neighbours: returns true and zmax, the last z of the run.  An unallocated
-// centre block falls back to checking the blocks of the voxel range.
+// neighbours: returns true and zmax, the last z of the run.  A coarse cell of
+// 8^3 blocks none of which has its dilated flag set extends the run to the
+// cell and its one-block margin.  An unallocated centre block falls back to
+// checking the blocks of the voxel range.
 __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, float z, FarCache &c,
                                         float &zmax) {
   const float e = 0.05f;
@@ -441,8 +467,15 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, f
   const int bx = floordiv8((int)floorf(px + 0.5f)), by = floordiv8((int)floorf(py + 0.5f)),
             bz = floordiv8((int)floorf(pz + 0.5f));
   int x0, x1, y0, y1, z0, z1;
-  const int ptr = find_block(v, bx, by, bz);
-  if (ptr >= 0 && v.blk_ndil[ptr] == 0) {
+  // a coarse cell without dilated-near blocks: every block of the cell and its
+  // one-block margin is far, and the run extends over the whole cell
+  const int cx = coarse_of(bx), cy = coarse_of(by), cz = coarse_of(bz);
+  int cw, cb;
+  const bool cfar = coarse_bit(cx, cy, cz, cw, cb) && !(v.coarse[cw] & (1u << cb));
+  const int ptr = cfar ? -1 : find_block(v, bx, by, bz);
+  if (cfar) {
+    x0 = 8 * cx - 1; x1 = 8 * cx + 8; y0 = 8 * cy - 1; y1 = 8 * cy + 8; z0 = 8 * cz - 1; z1 = 8 * cz + 8;
+  } else if (ptr >= 0 && v.blk_ndil[ptr] == 0) {
     x0 = bx - 1; x1 = bx + 1; y0 = by - 1; y1 = by + 1; z0 = bz - 1; z1 = bz + 1;
   } else {
     x0 = floordiv8((int)floorf(px - e) - 1); x1 = floordiv8((int)floorf(px + 1.0f + e));
@@ -466,7 +499,7 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, f
   return true;
 }
 
-__global__ __launch_bounds__(256) void k_raycast(const DmView *__restrict__ views, const int *__restrict__ active) {
+__global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ views, const int *__restrict__ active) {
   const int slot_ = active[blockIdx.y];
   if (slot_ < 0) return;
   const DmView &v = views[slot_];

@@ -119,6 +119,18 @@ class TsdfVolume:
             return xyz[:n]
         return xyz[:n], sdf[:n], rgb[:n], w[:n]
 
+    def snapshot(self):
+        """The reconstruction state as bytes (hash table, blocks, flags, pose, model maps)."""
+        n = C.c_uint64()
+        N.check(self._lib.dm_volume_snapshot(self._h, None, 0, C.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        N.check(self._lib.dm_volume_snapshot(self._h, N.ptr(buf), n.value, C.byref(n)))
+        return buf.tobytes()
+
+    def restore(self, data):
+        buf = np.frombuffer(data, np.uint8)
+        N.check(self._lib.dm_volume_restore(self._h, N.ptr(np.ascontiguousarray(buf)), len(buf)))
+
     # tsdfVolume::ExtractMesh (tsdf.cu:551-560): 200 000 triangles at most
     def ExtractMesh(self, lower_corner=None, upper_corner=None, max_triangles=200000):
         return MeshExtractor(max_triangles).ExtractMesh(self, lower_corner, upper_corner)
@@ -170,6 +182,13 @@ class Mesh:
             f.writelines(f"v {p[0]:g} {p[1]:g} {p[2]:g} {c[0]:g} {c[1]:g} {c[2]:g}\n" for p, c in zip(P, Cc))
             f.writelines(f"f {i} {i + 1} {i + 2}\n" for i in range(1, 3 * n + 1, 3))
         return self.num_triangles_
+
+    def SavePly(self, filename):
+        """Binary PLY of the welded mesh (shared vertices, uchar colours)."""
+        from .io import save_ply
+        V, Cv, F = self.weld()
+        save_ply(filename, V, F, Cv)
+        return len(F)
 
     def SaveWelded(self, filename):
         """OBJ with shared vertices (one "v" line per distinct edge id)."""
@@ -325,6 +344,30 @@ class NonRigidICP:
 
 
 
+def bilateral_filter(depth, radius=3, sigma_depth=0.03, sigma_space=2.0):
+    """pyramid::bilateralFilter (pyramid.cu:200-282) on the GPU."""
+    d = _f32(depth)
+    out = np.zeros_like(d)
+    N.check(N.lib().dm_depth_bilateral(N.ptr(d), N.ptr(out), d.shape[0], d.shape[1], int(radius),
+                                       float(sigma_depth), float(sigma_space)))
+    return out
+
+
+def depth_pyramid(depth, sigma_depth=0.03, levels=3):
+    """Gaussian depth pyramid (pyramid::downPyramidDepth's role) on the GPU:
+    list of levels 0..levels-1."""
+    d = _f32(depth)
+    r, c = d.shape
+    sizes = [(r >> l, c >> l) for l in range(levels)]
+    out = np.zeros(sum(a * b for a, b in sizes), np.float32)
+    N.check(N.lib().dm_depth_pyramid(N.ptr(d), r, c, float(sigma_depth), int(levels), N.ptr(out)))
+    res, o = [], 0
+    for a, b in sizes:
+        res.append(out[o:o + a * b].reshape(a, b))
+        o += a * b
+    return res
+
+
 def mc_tri_table():
     """The build's marching-cubes triangle table: 256 packed words."""
     out = np.zeros(256, np.uint64)
@@ -359,6 +402,10 @@ class Session:
     @property
     def volume(self):
         return TsdfVolume._borrow(self._lib.dm_session_volume(self._h), self.tsdf)
+
+    def set_depth_filter(self, radius=3, sigma_depth=0.03, sigma_space=2.0):
+        """Bilateral-filter each frame before tracking (radius 0 = off)."""
+        N.check(self._lib.dm_session_set_depth_filter(self._h, int(radius), float(sigma_depth), float(sigma_space)))
 
     def process_frame(self, depth, rgb=None, solve_on_host=False):
         depth, rgb = _f32(depth), _u8(rgb)
@@ -450,7 +497,7 @@ class Server:
     def reset(self):
         N.check(self._lib.dm_server_reset(self._h))
 
-    STAGES = ("source_maps", "icp_reduce_l0", "icp_solve", "alloc_all", "integrate", "raycast", "alloc_rays")
+    STAGES = ("source_maps", "icp_l0", "icp_l0_reduce_only", "alloc_all", "integrate", "raycast", "alloc_rays")
 
     def profile(self, frame, reps=5):
         ms = np.zeros(7)

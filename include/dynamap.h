@@ -112,6 +112,31 @@ DM_API int dm_volume_get_blocks(dm_volume *vol, int32_t cap, int32_t *xyz, float
  * exhausted), [3] block visits outside the key range, [4] new keys lost to a
  * full per-frame key set. */
 DM_API int dm_volume_counters(dm_volume *vol, int64_t out[5]);
+/* Snapshot / restore of the reconstruction state (no reference counterpart;
+ * resumable sessions): the hash table, the allocated blocks and their flags,
+ * and for a session its pose state and model maps, in one host buffer.
+ * dm_volume_snapshot with buf NULL reports the size.  dm_volume_restore
+ * needs a volume (or dm_session_volume of a session) created with the same
+ * properties and frame size; processing then continues exactly as it would
+ * have in the snapshotted one. */
+DM_API int dm_volume_snapshot(dm_volume *vol, void *buf, uint64_t cap, uint64_t *size);
+DM_API int dm_volume_restore(dm_volume *vol, const void *buf, uint64_t size);
+
+/* ------------------------------------------------------------------ */
+/* Depth preprocessing: pyramid::bilateralFilter / downPyramidDepth    */
+/* ------------------------------------------------------------------ */
+/* pyramid::bilateralFilter (pyramid.cu:200-282): weight
+ * exp(-dd^2/(2 sigma_depth^2)) * exp(-|q-p|^2/(2 sigma_space^2)) over the
+ * (2 radius + 1)^2 window, pixels without depth skipped and kept at 0.
+ * radius 0..7; host buffers of rows*cols metres. */
+DM_API int dm_depth_bilateral(const float *host_in, float *host_out, uint32_t rows, uint32_t cols, int32_t radius,
+                              float sigma_depth, float sigma_space);
+/* pyramid::downPyramidDepth (pyramid.cu:119-143), as a Gaussian pyramid:
+ * level l+1 pixel (u, v) = binomial 5x5 average around (2u, 2v) of the
+ * pixels within 3 sigma_depth of the centre's depth.  host_out receives
+ * levels 0..levels-1 back to back ((rows >> l) x (cols >> l) each). */
+DM_API int dm_depth_pyramid(const float *host_in, uint32_t rows, uint32_t cols, float sigma_depth, int32_t levels,
+                            float *host_out);
 
 /* ------------------------------------------------------------------ */
 /* Mesh: geometry::MeshExtractor / geometry::Mesh                      */

@@ -7,6 +7,9 @@
 //   DynaMap::tsdf::HashTable       include/dataStructure/hashTable/hashTable.h:32-141
 //   DynaMap::solver::gnSolver      include/solver/gnSolver/gnSolver.h:57-100
 //   DynaMap::builder               include/builder/builder.h:49-79
+//   DynaMap::geometry::defGraph    include/geometry/defGraph/defGraph.h:45-87
+//   DynaMap::blender::dqBlender    include/blender/dqBlender/dqBlender.h:19-83
+//   DynaMap::solver::nonRigidICP   include/solver/nonRigidICP/nonRigidICP.h:59-183
 //   DynaMap::geometry::MeshExtractor / Mesh (extraction and OBJ output)
 //                                  include/geometry/mesh/meshExtractor/meshExtractor.h:20-59,
 //                                  include/geometry/mesh/mesh.h:25-66
@@ -23,6 +26,7 @@
 #include <vector>
 
 #include "../dynamap.h"
+#include "../dynamap_nonrigid.h"
 #include "../dynamap_session.h"
 
 namespace DynaMap {
@@ -160,6 +164,15 @@ class tsdfVolume {
   }
   tsdfProperties GetProperties() const { return Properties; }
   dm_volume *handle() { return v_; }
+  // whole reconstruction state (no reference counterpart: resumable sessions)
+  std::vector<uint8_t> Snapshot() {
+    uint64_t n = 0;
+    check(dm_volume_snapshot(v_, nullptr, 0, &n));
+    std::vector<uint8_t> buf(n);
+    check(dm_volume_snapshot(v_, buf.data(), n, &n));
+    return buf;
+  }
+  void Restore(const std::vector<uint8_t> &buf) { check(dm_volume_restore(v_, buf.data(), buf.size())); }
   // tsdfVolume::ExtractMesh (tsdf.cu:551-560): at most 200 000 triangles
   geometry::Mesh ExtractMesh(const float3 &lower_corner, const float3 &upper_corner);
   // every allocated block
@@ -254,6 +267,148 @@ struct Properties {
     p.photo_weight = photoWeight; p.regularization = regularization; p.max_increment = maxIncrement;
     return p;
   }
+};
+
+// gnSolver (gnSolver.h:57-100): multi-level projective ICP of a source frame
+// against target maps, on the GPU.  The target is given as a depth frame
+// (its vertex/normal maps become the model) or left as the last raycast of
+// a session.
+class gnSolver {
+ public:
+  gnSolver() = default;
+  gnSolver(const gnSolver &) = delete;
+  gnSolver &operator=(const gnSolver &) = delete;
+  ~gnSolver() {
+    if (s_) dm_session_destroy(s_);
+  }
+  // gnSolver::init (gnSolver.cu:287-315)
+  void init(const Properties &prop, const rgbdSensor &sensor, const tsdf::tsdfProperties &tp) {
+    const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    dm_icp_props ip = prop.c();
+    check(dm_session_create(&tp, &ip, &sensor, I, &s_));
+    rows_ = sensor.rows;
+    cols_ = sensor.cols;
+  }
+  // gnSolver::solve (gnSolver.cu:317-438): increment source camera -> target
+  // camera; solve_on_host keeps the reference's host 6x6 steps
+  std::array<double, 16> solve(const rgbdImage &target, const rgbdImage &source, bool solve_on_host = false) {
+    for (int l = 0; l < 3; ++l) {
+      const size_t P = (size_t)(rows_ >> l) * (cols_ >> l);
+      std::vector<float> v(P * 3), n(P * 3), in(P);
+      check(dm_session_source_maps(s_, target.depth, target.rgb, l, v.data(), n.data(), in.data()));
+      check(dm_session_set_model_maps(s_, l, v.data(), n.data(), in.data()));
+    }
+    std::vector<float> tmp((size_t)rows_ * cols_ * 3);
+    check(dm_session_source_maps(s_, source.depth, source.rgb, 0, tmp.data(), nullptr, nullptr));
+    std::array<double, 16> T{};
+    check(dm_session_icp_track(s_, solve_on_host ? 1 : 0, T.data()));
+    return T;
+  }
+
+ private:
+  dm_session *s_ = nullptr;
+  uint32_t rows_ = 0, cols_ = 0;
+};
+
+// nonRigidICP properties (the build's formulation; the reference's
+// solver::Properties carries the iteration counts and regularisation)
+struct NonRigidProperties {
+  int gnIterations = 5, pcgIterations = 40;
+  double pcgTol = 1e-6, lambdaReg = 10.0, damping = 1e-3, distMax = 0.05, normalCos = 0.5;
+  double minDepth = 0.1, maxDepth = 5.0;
+  dm_nonrigid_props c() const {
+    return dm_nonrigid_props{gnIterations, pcgIterations, pcgTol, lambdaReg, damping, distMax, normalCos, minDepth,
+                             maxDepth};
+  }
+};
+
+}  // namespace solver
+
+namespace geometry {
+
+// MeshSTD (meshSTD.h): vertices with normals
+struct MeshSTD {
+  std::vector<float3> vertices, normals;
+};
+
+// defGraph (defGraph.h:45-87): nodes sampled from the canonical mesh, their
+// neighbours and weights, node transforms; on the GPU.
+class defGraph {
+ public:
+  defGraph() = default;
+  defGraph(const defGraph &) = delete;
+  defGraph &operator=(const defGraph &) = delete;
+  ~defGraph() { Free(); }
+  // defGraph::init + defGraphUpdateNodes (defGraph.cu:15-69, :483-545)
+  void init(const MeshSTD &srcMesh, float nodeRadius = 0.025f, float dgw = 0.05f, int maxNodes = 4096) {
+    Free();
+    dm_graph_props p{nodeRadius, dgw, maxNodes};
+    check(dm_graph_create(&srcMesh.vertices[0].x, &srcMesh.normals[0].x, (int64_t)srcMesh.vertices.size(), &p, &g_));
+    int32_t n = 0;
+    check(dm_graph_num_nodes(g_, &n));
+    nodeNum = n;
+    verticesNum = srcMesh.vertices.size();
+  }
+  void defGraphUpdateNodes() {}  // neighbours and weights are built by init
+  void Free() {
+    if (g_) dm_graph_destroy(g_);
+    g_ = nullptr;
+  }
+  // defGraph::updateNodesDQ (defGraph.cu:704-730): node transforms, nodeNum x 16
+  void updateNodesDQ(const std::vector<double> &T) { check(dm_graph_set_transforms(g_, T.data())); }
+  std::vector<double> nodeTransforms() {
+    std::vector<double> T((size_t)nodeNum * 16);
+    check(dm_graph_get_transforms(g_, T.data()));
+    return T;
+  }
+  dm_graph *handle() { return g_; }
+  int nodeNum = 0;
+  size_t verticesNum = 0;
+
+ private:
+  dm_graph *g_ = nullptr;
+};
+
+}  // namespace geometry
+
+namespace blender {
+
+// dqBlender (dqBlender.h:19-83)
+class dqBlender {
+ public:
+  void init(geometry::defGraph &graph) { graph_ = &graph; }
+  // dqBlender::blendMesh (dqBlender.cu:187-218)
+  void blendMesh(geometry::MeshSTD &dst, geometry::defGraph &graph) {
+    dst.vertices.resize(graph.verticesNum);
+    dst.normals.resize(graph.verticesNum);
+    check(dm_blend(graph.handle(), &dst.vertices[0].x, &dst.normals[0].x));
+  }
+  void blendMesh(geometry::MeshSTD &dst) { blendMesh(dst, *graph_); }
+
+ private:
+  geometry::defGraph *graph_ = nullptr;
+};
+
+}  // namespace blender
+
+namespace solver {
+
+// nonRigidICP (nonRigidICP.h:59-183)
+class nonRigidICP {
+ public:
+  void init(geometry::defGraph &, const NonRigidProperties &prop, int = 0) { prop_ = prop; }
+  // nonRigidICP::solve (nonRigidICP.cu:557-704): Gauss-Newton on the node
+  // transforms against one depth frame seen from pose (camera-to-world)
+  std::array<double, 4> solve(geometry::defGraph &graph, blender::dqBlender &, const rgbdImage &target,
+                              const double pose[16]) {
+    std::array<double, 4> st{};
+    dm_nonrigid_props p = prop_.c();
+    check(dm_nonrigid_solve(graph.handle(), target.depth, &target.sensor, pose, &p, st.data()));
+    return st;
+  }
+
+ private:
+  NonRigidProperties prop_;
 };
 
 }  // namespace solver

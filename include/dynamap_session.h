@@ -34,6 +34,11 @@ DM_API int dm_session_process_frame(dm_session *s, const float *host_depth, cons
 DM_API int dm_session_process_frame_dev(dm_session *s, const float *dev_depth, const uint8_t *dev_rgb,
                                         int32_t solve_on_host, double pose_out[16]);
 DM_API int dm_session_get_pose(dm_session *s, double pose[16]);
+/* Bilateral filter of the frame before tracking (pyramid::bilateralFilter,
+ * pyramid.cu:259-282, as dm_depth_bilateral): the tracking maps are built
+ * from the filtered depth, integration uses the raw depth.  radius 0 turns
+ * it off (the default). */
+DM_API int dm_session_set_depth_filter(dm_session *s, int32_t radius, float sigma_depth, float sigma_space);
 
 /* rgbdImage::getPointCloudXYZ + getNormalsfromVertices (rgbdImage.cu:39-49,
  * 158-167) on pyramid level `level` (pyramid::downPyramidDepth,

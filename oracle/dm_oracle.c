@@ -41,6 +41,11 @@ typedef OR_REAL real;
 
 OR_EXPORT int or_real_size(void) { return (int)sizeof(real); }
 
+/* OpenMP threads of the oracle's parallel loops (CPU-baseline timing). */
+#include <omp.h>
+OR_EXPORT void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+OR_EXPORT int or_get_threads(void) { return omp_get_max_threads(); }
+
 /* ------------------------------------------------------------------ */
 /* small vector helpers (evaluation order spelled out)                 */
 /* ------------------------------------------------------------------ */

@@ -1,6 +1,7 @@
 // Drives the reference-shaped C++ classes (tsdfVolume, builder) through the
 // C ABI, as the reference's own test programs do (src/test/testDynaMap.cpp):
 // a synthetic wall is integrated, raycast and tracked.  Exit code 0 = pass.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -52,6 +53,51 @@ int main() {
     return 3;
   } catch (const Error &e) {
     if (e.code != DM_E_INVALID) return 4;
+  }
+  // gnSolver: a wall at 1.02 m registered to a wall at 1.00 m (the motion
+  // along the normal is observable: tz = -0.02)
+  {
+    std::vector<float> near(120 * 160, 1.02f);
+    solver::gnSolver g;
+    solver::Properties gp;
+    gp.minDepth = 0.1f;
+    g.init(gp, s, p);
+    auto Tinc = g.solve(img, rgbdImage{near.data(), nullptr, s});
+    std::printf("gnSolver tz %.5f\n", Tinc[11]);
+    if (std::fabs(Tinc[11] + 0.02) > 1e-3) return 9;
+  }
+  // defGraph + dqBlender: a common node translation moves every vertex
+  {
+    geometry::MeshSTD m;
+    for (int j = 0; j < 30; ++j)
+      for (int i = 0; i < 40; ++i) {
+        m.vertices.push_back(float3{0.01f * i, 0.01f * j, 1.0f});
+        m.normals.push_back(float3{0.f, 0.f, -1.f});
+      }
+    geometry::defGraph graph;
+    graph.init(m, 0.05f, 0.08f, 1024);
+    std::vector<double> T((size_t)graph.nodeNum * 16, 0.0);
+    for (int n = 0; n < graph.nodeNum; ++n) {
+      for (int k = 0; k < 4; ++k) T[16 * n + 5 * k] = 1.0;
+      T[16 * n + 3] = 0.01;
+    }
+    graph.updateNodesDQ(T);
+    blender::dqBlender b;
+    b.init(graph);
+    geometry::MeshSTD out;
+    b.blendMesh(out, graph);
+    double worst = 0;
+    for (size_t k = 0; k < m.vertices.size(); ++k)
+      worst = std::max(worst, (double)std::fabs(out.vertices[k].x - m.vertices[k].x - 0.01f));
+    std::printf("defGraph nodes %d max blend error %.2e\n", graph.nodeNum, worst);
+    if (graph.nodeNum < 5 || worst > 1e-6) return 10;
+  }
+  // snapshot / restore of the volume
+  {
+    const int before = vol.GetNumAllocatedBlocks();
+    std::vector<uint8_t> snap = vol.Snapshot();
+    vol.Restore(snap);
+    if (vol.GetNumAllocatedBlocks() != before || snap.size() < 4096) return 11;
   }
   std::printf("ok\n");
   return 0;

@@ -387,3 +387,24 @@ def dq_of_transform(T, bits=64):
     L.or_dq_of_transform.restype, L.or_dq_of_transform.argtypes = None, [_vp, _vp]
     L.or_dq_of_transform(P(np.ascontiguousarray(T, np.float64)), P(out))
     return out
+
+
+# ---------------------------------------------------------------- preprocessing
+def bilateral(depth, radius, sigma_depth, sigma_space, bits=64):
+    L = load(bits)
+    L.or_bilateral.restype = None
+    L.or_bilateral.argtypes = [_vp, _vp, _i, _i, _i, _d, _d]
+    d = np.ascontiguousarray(depth, np.float32)
+    out = np.zeros_like(d)
+    L.or_bilateral(P(d), P(out), d.shape[0], d.shape[1], radius, sigma_depth, sigma_space)
+    return out
+
+
+def pyr_down(depth, sigma_depth, bits=64):
+    L = load(bits)
+    L.or_pyr_down.restype = None
+    L.or_pyr_down.argtypes = [_vp, _i, _i, _d, _vp]
+    d = np.ascontiguousarray(depth, np.float32)
+    out = np.zeros((d.shape[0] >> 1, d.shape[1] >> 1), np.float32)
+    L.or_pyr_down(P(d), d.shape[0], d.shape[1], sigma_depth, P(out))
+    return out
