@@ -148,6 +148,10 @@ def main():
         return srv
 
     srv = make_server()
+    # ---- launch shapes chosen by the autotuner on this workload's state
+    srv.run(N.DM_RUN_BATCHED, 0, W)
+    tuning = srv.autotune(frame=W, reps=3)
+    log("autotune", tuning)
     # ---- batched closed loop
     srv.run(N.DM_RUN_BATCHED, 0, W)
     r = srv.run(N.DM_RUN_BATCHED, W, K, want_poses=True)
@@ -273,7 +277,8 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded analytic RGB-D room scenes, noise + dropout)",
         "config": {"workload": "serve-16: 8 x 320x240 @ 2 cm + 8 x 640x480 @ 8 mm", "sessions": ns,
-                   "frames_per_session": nf, "icp_iterations": [10, 5, 4], "batch": "one frame per session"},
+                   "frames_per_session": nf, "icp_iterations": [10, 5, 4], "batch": "one frame per session",
+                   "tuning": tuning},
     }
     res.update(out)
     res["roofline"] = roofline

@@ -108,6 +108,7 @@ PROTOTYPES = {
     "dm_server_run": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp, _vp, _P(C.c_double),
                                 _P(C.c_int64)]),
     "dm_server_reset": (C.c_int, [_vp]),
+    "dm_server_autotune": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp, _vp]),
     "dm_server_profile": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp, _vp]),
 }
 

@@ -65,11 +65,6 @@ template <class T> int Sess::dalloc(T **ptr, size_t count, bool zero) {
   return DM_OK;
 }
 
-#define DM_TRY(x)              \
-  do {                         \
-    int r_ = (x);              \
-    if (r_ != DM_OK) return r_; \
-  } while (0)
 
 int Sess::init(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   DM_TRY(validate_props(p));
@@ -174,6 +169,8 @@ Launch Sess::launch() const {
   L.nact = 1;
   L.max_pixels = h.rows * h.cols;
   L.max_tiles = ((h.cols + 15) / 16) * ((h.rows + 15) / 16);
+  L.max_rows = h.rows;
+  L.max_cols = h.cols;
   L.stream = stream;
   return L;
 }

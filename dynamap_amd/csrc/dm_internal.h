@@ -103,6 +103,9 @@ struct Launch {
   int nact;             // entries in the list
   int max_pixels;       // largest rows*cols among active sessions
   int max_tiles;        // largest number of 16x16 pixel tiles among active sessions
+  int max_rows, max_cols;  // largest frame among active sessions
+  int ray_tw_log2 = 4;  // raycast tile: 2^ray_tw_log2 x (256 >> ray_tw_log2) pixels
+  int integ_wg = 2048;  // integration workgroups per session
   hipStream_t stream;
 };
 

@@ -20,6 +20,13 @@ int hip_fail(hipError_t e, const char *what);
     if (e_ != hipSuccess) return hip_fail(e_, #call); \
   } while (0)
 
+// propagate a DM_* status
+#define DM_TRY(x)                  \
+  do {                             \
+    int rc_ = (x);                 \
+    if (rc_ != DM_OK) return rc_;  \
+  } while (0)
+
 struct Sess {
   dm_tsdf_props props{};
   dm_icp_props icp{};

@@ -855,11 +855,6 @@ struct dm_graph {
   }
 };
 
-#define DM_TRY(x)                  \
-  do {                             \
-    int rc_ = (x);                 \
-    if (rc_ != DM_OK) return rc_;  \
-  } while (0)
 
 static unsigned grid_of(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 

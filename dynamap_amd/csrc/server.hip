@@ -54,6 +54,8 @@ struct dm_server {
   uint64_t used_bytes = 0;
   hipGraphExec_t graph = nullptr;
   int graph_iters[3] = {0, 0, 0};
+  int ray_tw_log2 = 4;               // launch shapes (dm_server_autotune)
+  int integ_wg = 2048;
 };
 
 namespace dm {
@@ -150,6 +152,10 @@ static Launch server_launch(const dm_server *srv) {
   L.nact = srv->cfg.max_sessions;
   L.max_pixels = (int)(srv->cfg.max_rows * srv->cfg.max_cols);
   L.max_tiles = (int)(((srv->cfg.max_cols + 15) / 16) * ((srv->cfg.max_rows + 15) / 16));
+  L.max_rows = (int)srv->cfg.max_rows;
+  L.max_cols = (int)srv->cfg.max_cols;
+  L.ray_tw_log2 = srv->ray_tw_log2;
+  L.integ_wg = srv->integ_wg;
   L.stream = srv->stream;
   return L;
 }
@@ -511,17 +517,18 @@ int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int32_t nfr
 // 6 ray traversal kernel alone.  work (summed over sessions, per launch):
 // [0] pixels, [1] visible blocks, [2] block visits of the traversal, [3] new
 // blocks, [4] sessions, [5] allocated blocks.
-int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage_ms, int64_t *work) {
-  if (!srv || reps <= 0 || !stage_ms) { set_error("invalid profile arguments"); return DM_E_INVALID; }
-  std::vector<int> act;
+// Put every admitted session that has frame `frame` into the batch table at
+// that frame and run one full frame, so stages can be timed on a real state.
+static int stage_setup(dm_server *srv, int32_t frame, std::vector<int> &act, int64_t &pixels) {
+  act.clear();
   for (int t = 0; t < (int)srv->traces.size(); ++t)
     if (srv->traces[t].sess && frame < srv->traces[t].nframes) act.push_back(t);
-  if (act.empty()) { set_error("no admitted session to profile"); return DM_E_STATE; }
+  if (act.empty()) { set_error("no admitted session at this frame"); return DM_E_STATE; }
   DM_HIP(hipStreamSynchronize(srv->stream));
   DmView *tab = srv->h_table[0];
   int *lst = srv->h_list[0];
   for (int s = 0; s < srv->cfg.max_sessions; ++s) lst[s] = -1;
-  int64_t pixels = 0;
+  pixels = 0;
   for (size_t q = 0; q < act.size(); ++q) {
     Trace &tr = srv->traces[act[q]];
     const size_t P = (size_t)tr.sensor.rows * tr.sensor.cols;
@@ -534,13 +541,21 @@ int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage
   DM_HIP(hipMemcpyAsync(srv->d_table, tab, sizeof(DmView) * srv->cfg.max_sessions, hipMemcpyHostToDevice, srv->stream));
   DM_HIP(hipMemcpyAsync(srv->d_list, lst, sizeof(int) * srv->cfg.max_sessions, hipMemcpyHostToDevice, srv->stream));
   Launch L = server_launch(srv);
-  // a full frame first so every session has a model and a pose
   launch_pose_begin(L);
   launch_source_maps(L);
   launch_alloc(L);
   launch_integrate(L);
   launch_raycast(L);
   DM_HIP(hipStreamSynchronize(srv->stream));
+  return DM_OK;
+}
+
+int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage_ms, int64_t *work) {
+  if (!srv || reps <= 0 || !stage_ms) { set_error("invalid profile arguments"); return DM_E_INVALID; }
+  std::vector<int> act;
+  int64_t pixels = 0;
+  DM_TRY(stage_setup(srv, frame, act, pixels));
+  Launch L = server_launch(srv);
   hipEvent_t e0, e1;
   DM_HIP(hipEventCreate(&e0));
   DM_HIP(hipEventCreate(&e1));
@@ -585,6 +600,62 @@ int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage
   if (work) { work[0] = pixels; work[3] = 0; work[4] = (int64_t)act.size(); }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  DM_HIP(hipGetLastError());
+  return DM_OK;
+}
+
+// Launch-shape autotuner: times the raycast with tiles of 8x32, 16x16, 32x8
+// and 64x4 pixels and the integration with 1024, 2048 and 4096 workgroups per
+// session on the state of `frame`, keeps the fastest of each, and captures
+// the serving graph again with them.  The sessions' volumes are advanced by
+// the timing runs: call dm_server_reset before serving.
+int dm_server_autotune(dm_server *srv, int32_t frame, int32_t reps, int32_t choice[2], double *ms) {
+  if (!srv || reps <= 0) { set_error("invalid autotune arguments"); return DM_E_INVALID; }
+  std::vector<int> act;
+  int64_t pixels = 0;
+  DM_TRY(stage_setup(srv, frame, act, pixels));
+  hipEvent_t e0, e1;
+  DM_HIP(hipEventCreate(&e0));
+  DM_HIP(hipEventCreate(&e1));
+  auto timed = [&](const Launch &L, bool ray, float &out) -> int {
+    ray ? launch_raycast(L) : launch_integrate(L);  // warm-up
+    DM_HIP(hipEventRecord(e0, srv->stream));
+    for (int k = 0; k < reps; ++k) ray ? launch_raycast(L) : launch_integrate(L);
+    DM_HIP(hipEventRecord(e1, srv->stream));
+    DM_HIP(hipEventSynchronize(e1));
+    DM_HIP(hipEventElapsedTime(&out, e0, e1));
+    out /= (float)reps;
+    return DM_OK;
+  };
+  const int tws[4] = {3, 4, 5, 6};
+  const int wgs[3] = {1024, 2048, 4096};
+  int best_tw = srv->ray_tw_log2, best_wg = srv->integ_wg;
+  float best = 3.0e38f;
+  for (int k = 0; k < 4; ++k) {
+    Launch L = server_launch(srv);
+    L.ray_tw_log2 = tws[k];
+    float t = 0.f;
+    int rc = timed(L, true, t);
+    if (rc != DM_OK) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
+    if (ms) ms[k] = t;
+    if (t < best) { best = t; best_tw = tws[k]; }
+  }
+  best = 3.0e38f;
+  for (int k = 0; k < 3; ++k) {
+    Launch L = server_launch(srv);
+    L.integ_wg = wgs[k];
+    float t = 0.f;
+    int rc = timed(L, false, t);
+    if (rc != DM_OK) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
+    if (ms) ms[4 + k] = t;
+    if (t < best) { best = t; best_wg = wgs[k]; }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  srv->ray_tw_log2 = best_tw;
+  srv->integ_wg = best_wg;
+  if (srv->graph) { hipGraphExecDestroy(srv->graph); srv->graph = nullptr; }
+  if (choice) { choice[0] = 1 << best_tw; choice[1] = best_wg; }
   DM_HIP(hipGetLastError());
   return DM_OK;
 }

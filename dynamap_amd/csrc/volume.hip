@@ -499,15 +499,18 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, f
   return true;
 }
 
-__global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ views, const int *__restrict__ active) {
+__global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ views, const int *__restrict__ active,
+                                                        int tw_log2) {
   const int slot_ = active[blockIdx.y];
   if (slot_ < 0) return;
   const DmView &v = views[slot_];
-  const int tiles_x = (v.cols + 15) >> 4;
+  // tile of tw x th pixels (tw * th = 256), shape chosen by the autotuner
+  const int tw = 1 << tw_log2, th = 256 >> tw_log2;
+  const int tiles_x = (v.cols + tw - 1) >> tw_log2;
   const int tile = blockIdx.x;
-  if (tile >= tiles_x * ((v.rows + 15) >> 4)) return;
-  const int u = (tile % tiles_x) * 16 + (threadIdx.x & 15);
-  const int r = (tile / tiles_x) * 16 + (threadIdx.x >> 4);
+  if (tile >= tiles_x * ((v.rows + th - 1) / th)) return;
+  const int u = (tile % tiles_x) * tw + (threadIdx.x & (tw - 1));
+  const int r = (tile / tiles_x) * th + (threadIdx.x >> tw_log2);
   if (u >= v.cols || r >= v.rows) return;
   const int i = r * v.cols + u;
   const Intr s = intr_of(v);
@@ -629,14 +632,17 @@ void launch_alloc_rays(const Launch &L) {
 
 void launch_integrate(const Launch &L) {
   if (L.nact <= 0) return;
-  hipLaunchKernelGGL(k_integrate, dim3(2048, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
+  hipLaunchKernelGGL(k_integrate, dim3((unsigned)L.integ_wg, (unsigned)L.nact), dim3(256), 0, L.stream, L.views,
+                     L.active);
   hipLaunchKernelGGL(k_dilate_near, dim3(256, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
 }
 
 void launch_raycast(const Launch &L) {
   if (L.nact <= 0) return;
-  hipLaunchKernelGGL(k_raycast, dim3((unsigned)L.max_tiles, (unsigned)L.nact), dim3(256), 0, L.stream, L.views,
-                     L.active);
+  const int tw = 1 << L.ray_tw_log2, th = 256 >> L.ray_tw_log2;
+  const unsigned tiles = (unsigned)(((L.max_cols + tw - 1) / tw) * ((L.max_rows + th - 1) / th));
+  hipLaunchKernelGGL(k_raycast, dim3(tiles, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active,
+                     L.ray_tw_log2);
 }
 
 }  // namespace dm

@@ -499,6 +499,17 @@ class Server:
 
     STAGES = ("source_maps", "icp_l0", "icp_l0_reduce_only", "alloc_all", "integrate", "raycast", "alloc_rays")
 
+    def autotune(self, frame=0, reps=3):
+        """Pick the raycast tile shape and integration grid by timing them on
+        the sessions' state at `frame`; the sessions are reset afterwards."""
+        choice = np.zeros(2, np.int32)
+        ms = np.zeros(7)
+        N.check(self._lib.dm_server_autotune(self._h, int(frame), int(reps), N.ptr(choice), N.ptr(ms)))
+        N.check(self._lib.dm_server_reset(self._h))
+        return dict(ray_tile=f"{int(choice[0])}x{256 // int(choice[0])}", integrate_workgroups=int(choice[1]),
+                    raycast_ms={f"{w}x{256 // w}": float(t) for w, t in zip((8, 16, 32, 64), ms[:4])},
+                    integrate_ms={str(n): float(t) for n, t in zip((1024, 2048, 4096), ms[4:])})
+
     def profile(self, frame, reps=5):
         ms = np.zeros(7)
         work = np.zeros(6, np.int64)

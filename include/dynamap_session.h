@@ -110,6 +110,13 @@ DM_API int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int3
 DM_API int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage_ms, int64_t *work);
 /* Retire every session (free their state). */
 DM_API int dm_server_reset(dm_server *srv);
+/* Launch-shape autotuner (no reference counterpart): times the raycast tile
+ * shapes (8x32, 16x16, 32x8, 64x4 pixels) and the integration grid (1024,
+ * 2048, 4096 workgroups per session) on the admitted sessions' state at
+ * `frame`, keeps the fastest of each for the serving graph.  choice (may be
+ * NULL) = {tile width, integration workgroups}; ms (may be NULL) = the 7
+ * timings.  Advances the sessions' volumes: dm_server_reset before serving. */
+DM_API int dm_server_autotune(dm_server *srv, int32_t frame, int32_t reps, int32_t choice[2], double *ms);
 
 #ifdef __cplusplus
 }
