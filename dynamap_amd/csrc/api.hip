@@ -46,7 +46,7 @@ size_t session_bytes(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   const size_t P = (size_t)rows * cols;
   const size_t nc = (size_t)new_cap(p.num_blocks);
   size_t b = (size_t)p.num_buckets * bucket_stride(p.bucket_size);
-  b += (size_t)p.num_blocks * (kBlockVox * 8 + 16 + 2) + (size_t)kCoarseWords * 4;
+  b += (size_t)p.num_blocks * (kBlockVox * 8 + 16 + 1) + ((size_t)kCoarseWords + 2 * kWinWords) * 4;
   b += (size_t)pow2_at_least((int)(2 * nc)) * 8 + nc * (8 + 4 * 4) + (size_t)p.num_buckets * 4;
   size_t maps = 0;
   for (int l = 0; l < kLevels; ++l) maps += (size_t)(rows >> l) * (cols >> l) * 16 * 4;
@@ -96,7 +96,8 @@ int Sess::init(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   DM_TRY(dalloc(&h.vox, (size_t)p.num_blocks * kBlockVox));
   DM_TRY(dalloc(&h.block_key, (size_t)p.num_blocks));
   DM_TRY(dalloc(&h.blk_near, (size_t)p.num_blocks));
-  DM_TRY(dalloc(&h.blk_ndil, (size_t)p.num_blocks));
+  DM_TRY(dalloc(&h.nearmap, kWinWords));
+  DM_TRY(dalloc(&h.dilmap, kWinWords));
   DM_TRY(dalloc(&h.coarse, (size_t)kCoarseWords));
   DM_TRY(dalloc(&h.dset, (size_t)dsz));
   DM_TRY(dalloc(&h.newkeys, (size_t)h.newcap, false));
@@ -314,15 +315,15 @@ struct SnapHeader {
   int32_t ctr[C_COUNT];
   uint64_t payload;
 };
-static const char kSnapMagic[8] = {'D', 'M', 'S', 'N', 'A', 'P', '0', '1'};
+static const char kSnapMagic[8] = {'D', 'M', 'S', 'N', 'A', 'P', '0', '2'};
 
-// device sections of a volume / session state, in file order
+// device sections of a volume / session state, in file order; the raycast's
+// bitmaps are not stored (restore rebuilds them from blk_near and the keys)
 static std::vector<std::pair<void *, size_t>> snap_sections(const Sess &S, int nalloc) {
   const DmView &h = S.h;
   std::vector<std::pair<void *, size_t>> v = {
       {h.buckets, (size_t)h.num_buckets * h.bstride}, {h.block_key, (size_t)nalloc * sizeof(int4)},
       {h.vox, (size_t)nalloc * kBlockVox * 8},        {h.blk_near, (size_t)nalloc},
-      {h.blk_ndil, (size_t)nalloc},                  {h.coarse, (size_t)kCoarseWords * 4},
       {h.posed, (size_t)PD_COUNT * 8},               {h.posef, (size_t)PF_COUNT * 4}};
   for (int l = 0; l < kLevels; ++l) {
     const size_t Pl = (size_t)(h.rows >> l) * (h.cols >> l);
@@ -505,7 +506,7 @@ int dm_volume_snapshot(dm_volume *vol, void *buf, uint64_t cap, uint64_t *size) 
   Sess &S = *vol;
   SnapHeader H{};
   memcpy(H.magic, kSnapMagic, 8);
-  H.version = 1;
+  H.version = 2;
   H.hdr_bytes = sizeof(SnapHeader);
   H.props = S.props;
   H.max_rows = S.max_rows; H.max_cols = S.max_cols; H.rows = S.h.rows; H.cols = S.h.cols;
@@ -541,7 +542,7 @@ int dm_volume_restore(dm_volume *vol, const void *buf, uint64_t size) {
   Sess &S = *vol;
   SnapHeader H;
   memcpy(&H, buf, sizeof H);
-  if (memcmp(H.magic, kSnapMagic, 8) != 0 || H.version != 1 || H.hdr_bytes != sizeof(SnapHeader)) {
+  if (memcmp(H.magic, kSnapMagic, 8) != 0 || H.version != 2 || H.hdr_bytes != sizeof(SnapHeader)) {
     set_error("dm_volume_restore: not a snapshot of this format");
     return DM_E_INVALID;
   }
@@ -571,13 +572,14 @@ int dm_volume_restore(dm_volume *vol, const void *buf, uint64_t size) {
   if (rest) {
     DM_HIP(hipMemsetAsync(S.h.vox + (size_t)H.num_alloc * kBlockVox, 0, rest * kBlockVox * 8, S.stream));
     DM_HIP(hipMemsetAsync(S.h.blk_near + H.num_alloc, 0, rest, S.stream));
-    DM_HIP(hipMemsetAsync(S.h.blk_ndil + H.num_alloc, 0, rest, S.stream));
   }
   DM_HIP(hipMemsetAsync(S.h.bucket_head, 0xff, (size_t)S.props.num_buckets * 4, S.stream));
   for (int l = 0; l < 3; ++l) S.h.iters[l] = H.iters[l];
   S.h.icp_dmin = H.icp_f[0]; S.h.icp_dmax = H.icp_f[1]; S.h.dist_max = H.icp_f[2];
   S.h.w_photo = H.icp_f[3]; S.h.lambda = H.icp_f[4]; S.h.max_inc = H.icp_f[5];
   DM_TRY(S.push_view());
+  rebuild_far_maps(S.launch());
+  DM_HIP(hipGetLastError());
   DM_HIP(hipStreamSynchronize(S.stream));
   return DM_OK;
 }

@@ -20,6 +20,12 @@ constexpr int kIcpVals = 32;               // per-workgroup ICP partial record (
 // kCoarse^3 cells centred on block (0, 0, 0); cells outside it count as near.
 constexpr int kCoarse = 64;
 constexpr int kCoarseWords = kCoarse * kCoarse * kCoarse / 32;
+// Block-level bitmaps over the same window (kWin^3 blocks centred on block
+// (0, 0, 0), x fastest): "near" (an allocated block with an observed voxel
+// below the far threshold) and "dilated" (a near block within one block).
+// Blocks outside the window are looked up through the hash table.
+constexpr int kWin = kCoarse * 8;
+constexpr size_t kWinWords = (size_t)kWin * kWin * kWin / 32;
 
 // per-volume counters, int32 each
 enum Ctr : int {
@@ -64,8 +70,9 @@ struct DmView {
   uint64_t *vox;        // num_blocks * 512 voxel words (sdf bits | r | g<<8 | b<<16 | w<<24 <<32)
   int4 *block_key;      // num_blocks block coordinates
   uint8_t *blk_near;    // num_blocks: 1 when an observed voxel lies below the raycast far threshold
-  uint8_t *blk_ndil;    // num_blocks: blk_near dilated over the 26 neighbouring blocks
-  uint32_t *coarse;     // kCoarse^3 bits: cell of 8^3 blocks holds a block with blk_ndil set
+  uint32_t *nearmap;    // kWinWords: window bit of every block with blk_near set
+  uint32_t *dilmap;     // kWinWords: window bit of every block within one block of a near block
+  uint32_t *coarse;     // kCoarse^3 bits: cell of 8^3 blocks holds a block with its dilmap bit set
   uint64_t *dset;       // per-frame new-key set, dset_mask + 1 slots, 0 = empty
   uint64_t *newkeys;    // newcap
   int32_t *newdslot;    // newcap
@@ -113,6 +120,7 @@ void launch_alloc(const Launch &L);
 void launch_alloc_rays(const Launch &L);  // traversal kernel only (profiling)
 void launch_integrate(const Launch &L);
 void launch_raycast(const Launch &L);
+void rebuild_far_maps(const Launch &L);  // clear and rebuild nearmap/dilmap/coarse (not for graph capture)
 void launch_source_maps(const Launch &L);
 // ICP linearisation at one level (one FP32 record per workgroup); with solve,
 // a second kernel sums the records and takes the Gauss-Newton step on the

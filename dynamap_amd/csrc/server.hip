@@ -68,7 +68,8 @@ int sess_reset(Sess &S, const double init_pose[16]) {
   DM_HIP(hipMemsetAsync(S.h.buckets, 0, (size_t)S.h.num_buckets * S.h.bstride, S.stream));
   DM_HIP(hipMemsetAsync(S.h.vox, 0, (size_t)n * kBlockVox * 8, S.stream));
   DM_HIP(hipMemsetAsync(S.h.blk_near, 0, (size_t)n, S.stream));
-  DM_HIP(hipMemsetAsync(S.h.blk_ndil, 0, (size_t)n, S.stream));
+  DM_HIP(hipMemsetAsync(S.h.nearmap, 0, kWinWords * 4, S.stream));
+  DM_HIP(hipMemsetAsync(S.h.dilmap, 0, kWinWords * 4, S.stream));
   DM_HIP(hipMemsetAsync(S.h.coarse, 0, (size_t)kCoarseWords * 4, S.stream));
   DM_HIP(hipMemsetAsync(S.h.ctr, 0, C_COUNT * 4, S.stream));
   double pd[PD_COUNT] = {0};

@@ -337,6 +337,37 @@ __device__ __forceinline__ bool coarse_bit(int cx, int cy, int cz, int &word, in
   return true;
 }
 
+// Bit of block (x, y, z) in the block window, false outside it.
+__device__ __forceinline__ bool win_bit(int x, int y, int z, uint32_t &idx) {
+  const unsigned X = (unsigned)(x + kWin / 2), Y = (unsigned)(y + kWin / 2), Z = (unsigned)(z + kWin / 2);
+  if (X >= (unsigned)kWin || Y >= (unsigned)kWin || Z >= (unsigned)kWin) return false;
+  idx = (Z * (unsigned)kWin + Y) * (unsigned)kWin + X;
+  return true;
+}
+__device__ __forceinline__ bool test_bit(const uint32_t *map, uint32_t idx) { return (map[idx >> 5] >> (idx & 31)) & 1u; }
+
+// Row (dy, dz) = (row % 3 - 1, row / 3 - 1) of the 3^3 neighbourhood of block
+// (x, y, z) in the window: bits x-1..x+1 (clipped to the window) lie in words
+// w0 and w1 (equal or adjacent) under mask0 and mask1.  False when the row is
+// outside the window.
+__device__ __forceinline__ bool nbr_row(int x, int y, int z, int row, uint32_t &w0, uint32_t &w1, uint32_t &mask0,
+                                        uint32_t &mask1) {
+  const int yy = y + row % 3 - 1, zz = z + row / 3 - 1;
+  const int xl = max(x - 1, -kWin / 2), xh = min(x + 1, kWin / 2 - 1);
+  uint32_t i0;
+  if (xl > xh || !win_bit(xl, yy, zz, i0)) return false;
+  const uint32_t i1 = i0 + (uint32_t)(xh - xl);
+  w0 = i0 >> 5;
+  w1 = i1 >> 5;
+  mask0 = 0u;
+  mask1 = 0u;
+  for (uint32_t i = i0; i <= i1; ++i) {
+    if ((i >> 5) == w0) mask0 |= 1u << (i & 31);
+    else mask1 |= 1u << (i & 31);
+  }
+  return true;
+}
+
 __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ views, const int *__restrict__ active) {
   const int slot_ = active[blockIdx.y];
   if (slot_ < 0) return;
@@ -347,12 +378,26 @@ __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ vi
   const float vs = v.voxel_size, tau = v.trunc;
   const double half = 0.5 * (double)vs * 8.0;
   const float far_thr = far_threshold(tau);
-  // the coarse bitmap is rebuilt by k_dilate_near
+  // the raycast's bitmaps are rebuilt by k_dilate_near: the coarse one is
+  // cleared whole, the block-level ones at the words that the near blocks of
+  // the previous frame set (whole words: every bit in them belongs to a block
+  // near one of those, and k_dilate_near sets it again if it still is)
   for (int w = blockIdx.x * 256 + threadIdx.x; w < kCoarseWords; w += gridDim.x * 256) v.coarse[w] = 0u;
   int nvis = 0;
   for (int blk = blockIdx.x; blk < nalloc; blk += gridDim.x) {
-    if (threadIdx.x == 0) v.blk_ndil[blk] = 0;  // rebuilt by k_dilate_near
     const int4 key = v.block_key[blk];
+    if (threadIdx.x < 10 && v.blk_near[blk]) {
+      if (threadIdx.x < 9) {
+        uint32_t w0, w1, m0, m1;
+        if (nbr_row(key.x, key.y, key.z, threadIdx.x, w0, w1, m0, m1)) {
+          v.dilmap[w0] = 0u;
+          v.dilmap[w1] = 0u;
+        }
+      } else {
+        uint32_t idx;
+        if (win_bit(key.x, key.y, key.z, idx)) v.nearmap[idx >> 5] = 0u;
+      }
+    }
     const f3 pos = mk((float)key.x * vs * 8.0f, (float)key.y * vs * 8.0f, (float)key.z * vs * 8.0f);
     const f3 pc = xform(Ti, pos);
     if (pc.z < 0) continue;
@@ -378,10 +423,10 @@ __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ vi
   if (threadIdx.x == 0 && nvis) atomicAdd(&v.ctr[C_NVIS], nvis);
 }
 
-// Dilate the near flags over the 26 neighbouring blocks: blk_ndil[b] == 0
-// means b and every allocated neighbour have no observed voxel below the far
-// threshold, so every voxel within one block of b is far.  Runs after the
-// integration (which clears blk_ndil of every allocated block).
+// Rebuild the raycast's bitmaps from the near blocks (after the integration
+// has cleared the words of the previous frame's near blocks): the window bit
+// of every near block, the dilated bits of its 3^3 neighbourhood, and the
+// coarse cells that hold a block of that neighbourhood (at most 8).
 __global__ __launch_bounds__(256) void k_dilate_near(const DmView *__restrict__ views, const int *__restrict__ active) {
   const int slot_ = active[blockIdx.y];
   if (slot_ < 0) return;
@@ -390,13 +435,14 @@ __global__ __launch_bounds__(256) void k_dilate_near(const DmView *__restrict__ 
   for (int blk = blockIdx.x * 256 + threadIdx.x; blk < nalloc; blk += gridDim.x * 256) {
     if (!v.blk_near[blk]) continue;
     const int4 key = v.block_key[blk];
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int ptr = (dx | dy | dz) == 0 ? blk : find_block(v, key.x + dx, key.y + dy, key.z + dz);
-          if (ptr >= 0) v.blk_ndil[ptr] = 1;
-        }
-    // coarse cells that hold a block within one block of this one (at most 8)
+    uint32_t idx;
+    if (win_bit(key.x, key.y, key.z, idx)) atomicOr(&v.nearmap[idx >> 5], 1u << (idx & 31));
+    for (int row = 0; row < 9; ++row) {
+      uint32_t w0, w1, m0, m1;
+      if (!nbr_row(key.x, key.y, key.z, row, w0, w1, m0, m1)) continue;
+      if ((v.dilmap[w0] & m0) != m0) atomicOr(&v.dilmap[w0], m0);
+      if (m1 && (v.dilmap[w1] & m1) != m1) atomicOr(&v.dilmap[w1], m1);
+    }
     const int cx0 = coarse_of(key.x - 1), cx1 = coarse_of(key.x + 1), cy0 = coarse_of(key.y - 1),
               cy1 = coarse_of(key.y + 1), cz0 = coarse_of(key.z - 1), cz1 = coarse_of(key.z + 1);
     for (int cz = cz0; cz <= cz1; ++cz)
@@ -407,6 +453,20 @@ __global__ __launch_bounds__(256) void k_dilate_near(const DmView *__restrict__ 
           if (!(v.coarse[w] & (1u << b))) atomicOr(&v.coarse[w], 1u << b);
         }
   }
+}
+
+// Clears every bitmap word (grid-stride) before a full rebuild.
+__global__ __launch_bounds__(256) void k_clear_far_maps(const DmView *__restrict__ views,
+                                                        const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t w = (size_t)blockIdx.x * 256 + threadIdx.x; w < kWinWords; w += stride) {
+    v.nearmap[w] = 0u;
+    v.dilmap[w] = 0u;
+  }
+  for (size_t w = (size_t)blockIdx.x * 256 + threadIdx.x; w < (size_t)kCoarseWords; w += stride) v.coarse[w] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -448,17 +508,29 @@ __device__ __forceinline__ bool block_far(const DmView &v, int bx, int by, int b
   return f;
 }
 
+// Far test of one block for the precise path: the window's near bit, or
+// the hash table (through the two-entry cache) outside the window.
+__device__ __forceinline__ bool block_far_any(const DmView &v, int bx, int by, int bz, FarCache &c) {
+  uint32_t idx;
+  if (win_bit(bx, by, bz, idx)) return !test_bit(v.nearmap, idx);
+  return block_far(v, bx, by, bz, c);
+}
+
 // Conservative empty-space run.  pv = (origin + z * direction) / vs is the
 // approximate sample position in voxel units (error about 1e-3 voxel); the
 // voxels GetInterpolatedVoxel reads at the exact position lie in
 // [floor(pv - e) - 1, floor(pv + 1 + e)] per axis (the exact indices round
-// p/vs -+ 0.5 half away from zero).  When the block holding the (approximate)
-// centre voxel is allocated and its dilated flag is clear, all those voxels
-// are far; the run continues while they stay within that block's 27
-// neighbours: returns true and zmax, the last z of the run.  A coarse cell of
-// 8^3 blocks none of which has its dilated flag set extends the run to the
-// cell and its one-block margin.  An unallocated centre block falls back to
-// checking the blocks of the voxel range.
+// p/vs -+ 0.5 half away from zero).  Three tests, coarsest first:
+//  - the coarse cell (8^3 blocks) of the block holding the approximate centre
+//    voxel has no dilated bit: the run extends over the cell and its
+//    one-block margin;
+//  - that block's dilated bit is clear (no near block among its 3^3
+//    neighbourhood, allocated or not): the run extends over the neighbourhood;
+//  - otherwise every block of the voxel range must be far (near bit clear,
+//    or unallocated / not near through the hash table outside the window),
+//    and the run extends over those blocks.
+// Runs end e + 1 voxel inside the region, so the exact positions up to zmax
+// keep their voxel range inside it.
 __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, float z, FarCache &c,
                                         float &zmax) {
   const float e = 0.05f;
@@ -467,15 +539,12 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, f
   const int bx = floordiv8((int)floorf(px + 0.5f)), by = floordiv8((int)floorf(py + 0.5f)),
             bz = floordiv8((int)floorf(pz + 0.5f));
   int x0, x1, y0, y1, z0, z1;
-  // a coarse cell without dilated-near blocks: every block of the cell and its
-  // one-block margin is far, and the run extends over the whole cell
   const int cx = coarse_of(bx), cy = coarse_of(by), cz = coarse_of(bz);
   int cw, cb;
-  const bool cfar = coarse_bit(cx, cy, cz, cw, cb) && !(v.coarse[cw] & (1u << cb));
-  const int ptr = cfar ? -1 : find_block(v, bx, by, bz);
-  if (cfar) {
+  uint32_t idx;
+  if (coarse_bit(cx, cy, cz, cw, cb) && !(v.coarse[cw] & (1u << cb))) {
     x0 = 8 * cx - 1; x1 = 8 * cx + 8; y0 = 8 * cy - 1; y1 = 8 * cy + 8; z0 = 8 * cz - 1; z1 = 8 * cz + 8;
-  } else if (ptr >= 0 && v.blk_ndil[ptr] == 0) {
+  } else if (win_bit(bx, by, bz, idx) && !test_bit(v.dilmap, idx)) {
     x0 = bx - 1; x1 = bx + 1; y0 = by - 1; y1 = by + 1; z0 = bz - 1; z1 = bz + 1;
   } else {
     x0 = floordiv8((int)floorf(px - e) - 1); x1 = floordiv8((int)floorf(px + 1.0f + e));
@@ -484,7 +553,7 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, f
     for (int qx = x0; qx <= x1; ++qx)
       for (int qy = y0; qy <= y1; ++qy)
         for (int qz = z0; qz <= z1; ++qz)
-          if (!block_far(v, qx, qy, qz, c)) return false;
+          if (!block_far_any(v, qx, qy, qz, c)) return false;
   }
   float zm = 3.0e38f;
   const float m0 = 1.0f + 2.0f * e, m1 = 7.0f - 2.0f * e;
@@ -634,6 +703,12 @@ void launch_integrate(const Launch &L) {
   if (L.nact <= 0) return;
   hipLaunchKernelGGL(k_integrate, dim3((unsigned)L.integ_wg, (unsigned)L.nact), dim3(256), 0, L.stream, L.views,
                      L.active);
+  hipLaunchKernelGGL(k_dilate_near, dim3(256, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
+}
+
+void rebuild_far_maps(const Launch &L) {
+  if (L.nact <= 0) return;
+  hipLaunchKernelGGL(k_clear_far_maps, dim3(1024, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
   hipLaunchKernelGGL(k_dilate_near, dim3(256, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
 }
 
