@@ -9,6 +9,12 @@
 
 namespace dm {
 
+// Load through a global-address-space pointer (a global_load instead of a
+// flat one: pointers read from a DmView carry no address space).
+template <class T> __device__ __forceinline__ T ldg(const T *p) {
+  return *(const __attribute__((address_space(1))) T *)p;
+}
+
 struct f3 { float x, y, z; };
 __device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
 __device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
@@ -72,6 +78,14 @@ __device__ __forceinline__ f3 get_point3d(int i, float depth, const Intr &s) {
   p.y = (((float)v - s.cy) * p.z) / s.fy;
   return p;
 }
+// getPoint3d of pixel (u, v) = (i % cols, i / cols), without the division
+__device__ __forceinline__ f3 get_point3d_uv(int u, int v, float depth, const Intr &s) {
+  f3 p;
+  p.z = depth;
+  p.x = (((float)u - s.cx) * p.z) / s.fx;
+  p.y = (((float)v - s.cy) * p.z) / s.fy;
+  return p;
+}
 // include/utils/utils.h:114-119 Project
 __device__ __forceinline__ void project(f3 p, const Intr &s, int &u, int &v) {
   u = to_i32(roundf((s.fx * p.x) / p.z + s.cx));
@@ -109,7 +123,7 @@ __device__ __forceinline__ const uint8_t *bucket_ptr(const DmView &v, int b) {
   return v.buckets + (size_t)b * (size_t)v.bstride;
 }
 __device__ __forceinline__ int bucket_count(const DmView &v, const uint8_t *bk) {
-  return *(const int32_t *)(bk + 12 * v.bucket_size);
+  return ldg((const int32_t *)(bk + 12 * v.bucket_size));
 }
 
 // FindHashEntry (hashTable.cu:136-150): pool index or -1.  Only the occupied
@@ -121,15 +135,16 @@ __device__ __forceinline__ int find_block(const DmView &v, int x, int y, int z) 
   const int cnt = bucket_count(v, bk);
   const uint64_t *keys = (const uint64_t *)bk;
   for (int s = 0; s < cnt; ++s)
-    if (keys[s] == key) return ((const int32_t *)(bk + 8 * v.bucket_size))[s];
+    if (ldg(keys + s) == key) return ldg((const int32_t *)(bk + 8 * v.bucket_size) + s);
   return -1;
 }
 
 // world -> global voxel (tsdf.cu:189-193): trunc(x/vs + sign(x)*0.5)
 __device__ __forceinline__ float signf_(float x) { return (float)((x > 0) - (x < 0)); }
 __device__ __forceinline__ int world_to_voxel(float x, float vs) { return to_i32(x / vs + signf_(x) * 0.5f); }
-__device__ __forceinline__ int floordiv8(int g) { return g < 0 ? (g - 8 + 1) / 8 : g / 8; }
-__device__ __forceinline__ int mod8(int g) { int m = g % 8; return m < 0 ? m + 8 : m; }
+// floor(g / 8) and g mod 8 (in 0..7) of a two's-complement int
+__device__ __forceinline__ int floordiv8(int g) { return g >> 3; }
+__device__ __forceinline__ int mod8(int g) { return g & 7; }
 
 // voxel word: low 32 bits sdf, then r, g, b, weight bytes
 __device__ __forceinline__ float vox_sdf(uint64_t w) { return __uint_as_float((uint32_t)w); }
@@ -149,7 +164,7 @@ __device__ __forceinline__ uint64_t voxel_at(const DmView &v, int gx, int gy, in
     c.ptr = find_block(v, bx, by, bz);
   }
   if (c.ptr < 0) return 0ull;
-  return v.vox[(size_t)c.ptr * kBlockVox + mod8(gx) * 64 + mod8(gy) * 8 + mod8(gz)];
+  return ldg(v.vox + ((size_t)c.ptr * kBlockVox + mod8(gx) * 64 + mod8(gy) * 8 + mod8(gz)));
 }
 
 // GetVoxel (tsdf.cu:227-239); returns the voxel word, 0 when unallocated.

@@ -93,7 +93,7 @@ __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r, unsigned
   const M4 T = load_m4(v.posef + PF_T);
   const float block_m = 8.0f * v.voxel_size;
   const f3 start = mk(T.m[3], T.m[7], T.m[11]);
-  const f3 point = xform(T, get_point3d(i, d, s));
+  const f3 point = xform(T, get_point3d_uv(u, r, d, s));
   if (point.x == 0 && point.y == 0 && point.z == 0) return 0;
   const f3 dir = normalize(point - start);
   const float sd = norm(start - point);
@@ -326,7 +326,7 @@ __device__ __forceinline__ float far_threshold(float trunc) {
 }
 
 // Coarse cell (8^3 blocks) of a block coordinate, and its bit in the window.
-__device__ __forceinline__ int coarse_of(int b) { return b < 0 ? (b - 7) / 8 : b / 8; }
+__device__ __forceinline__ int coarse_of(int b) { return b >> 3; }
 __device__ __forceinline__ bool coarse_bit(int cx, int cy, int cz, int &word, int &bit) {
   const int x = cx + kCoarse / 2, y = cy + kCoarse / 2, z = cz + kCoarse / 2;
   if ((unsigned)x >= (unsigned)kCoarse || (unsigned)y >= (unsigned)kCoarse || (unsigned)z >= (unsigned)kCoarse)
@@ -344,7 +344,7 @@ __device__ __forceinline__ bool win_bit(int x, int y, int z, uint32_t &idx) {
   idx = (Z * (unsigned)kWin + Y) * (unsigned)kWin + X;
   return true;
 }
-__device__ __forceinline__ bool test_bit(const uint32_t *map, uint32_t idx) { return (map[idx >> 5] >> (idx & 31)) & 1u; }
+__device__ __forceinline__ bool test_bit(const uint32_t *map, uint32_t idx) { return (ldg(map + (idx >> 5)) >> (idx & 31)) & 1u; }
 
 // Row (dy, dz) = (row % 3 - 1, row / 3 - 1) of the 3^3 neighbourhood of block
 // (x, y, z) in the window: bits x-1..x+1 (clipped to the window) lie in words
@@ -481,6 +481,131 @@ __device__ __forceinline__ float intensity_of(uint32_t r, uint32_t g, uint32_t b
   return (0.2126f * (float)r + 0.7152f * (float)g + 0.0722f * (float)b) * (1.0f / 255.0f);
 }
 
+// GetInterpolatedVoxel for the raycast: the arithmetic of interp_voxel, with
+// the block pointers of the samples kept in a per-thread window of 2x2x2
+// blocks.  The window's pool indices sit in LDS (thread-strided), a register
+// bit per slot says which are looked up, and a slot is filled from the hash
+// table on first use; the nine voxel words are then loaded together.  A
+// sample whose voxels span more than two blocks on an axis (possible only
+// through rounding at huge coordinates) looks its voxels up one by one.
+struct BlockWin {
+  int wx, wy, wz;      // window base block
+  unsigned resolved;   // bit k: LDS slot k holds the pool index (or -1)
+  int dx, dy, dz;      // 1 where the ray runs towards -axis (the window extends that way)
+  int *slot;           // this thread's slots: slot[k * 256], k = x + 2y + 4z
+};
+
+// voxel word of global voxel (gx, gy, gz) through the hash table
+__device__ __forceinline__ uint64_t voxel_lookup(const DmView &v, int gx, int gy, int gz) {
+  const int ptr = find_block(v, floordiv8(gx), floordiv8(gy), floordiv8(gz));
+  if (ptr < 0) return 0ull;
+  return ldg(v.vox + ((uint32_t)ptr * (uint32_t)kBlockVox + (uint32_t)(mod8(gx) * 64 + mod8(gy) * 8 + mod8(gz))));
+}
+
+__device__ __forceinline__ bool interp_voxel_win(const DmView &v, f3 p, BlockWin &W, float &sdf_out, float *col) {
+  const float vs = v.voxel_size;
+  const float inv_vs = 1.0f / vs;
+  const f3 vp = mk(p.x / vs, p.y / vs, p.z / vs);
+  const int cx = to_i32(vp.x + signf_(p.x) * 0.5f), cy = to_i32(vp.y + signf_(p.y) * 0.5f),
+            cz = to_i32(vp.z + signf_(p.z) * 0.5f);
+  const f3 dual = p - mk(vs / 2.0f, vs / 2.0f, vs / 2.0f);
+  const int ix[2] = {world_to_voxel_fast(dual.x + 0.0f, vs, inv_vs), world_to_voxel_fast(dual.x + vs, vs, inv_vs)};
+  const int iy[2] = {world_to_voxel_fast(dual.y + 0.0f, vs, inv_vs), world_to_voxel_fast(dual.y + vs, vs, inv_vs)};
+  const int iz[2] = {world_to_voxel_fast(dual.z + 0.0f, vs, inv_vs), world_to_voxel_fast(dual.z + vs, vs, inv_vs)};
+  // block range of the centre and corner voxels per axis
+  const int lox = min(min(ix[0], ix[1]), cx) >> 3, hix = max(max(ix[0], ix[1]), cx) >> 3;
+  const int loy = min(min(iy[0], iy[1]), cy) >> 3, hiy = max(max(iy[0], iy[1]), cy) >> 3;
+  const int loz = min(min(iz[0], iz[1]), cz) >> 3, hiz = max(max(iz[0], iz[1]), cz) >> 3;
+  const f3 wt = mk(vp.x - floorf(vp.x), vp.y - floorf(vp.y), vp.z - floorf(vp.z));
+  if (hix - lox > 1 || hiy - loy > 1 || hiz - loz > 1) {
+    // voxels over more than two blocks on an axis: one lookup per voxel (a
+    // rolled loop, same order and arithmetic as below)
+    const uint64_t w0 = voxel_lookup(v, cx, cy, cz);
+    if ((w0 >> 56) == 0) return false;
+    float dist = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+#pragma unroll 1
+    for (int k = 0; k < 8; ++k) {
+      const int ox = (0xD2 >> k) & 1, oy = (0xB4 >> k) & 1, oz = (0xE8 >> k) & 1;
+      const uint64_t w = voxel_lookup(v, ox ? ix[1] : ix[0], oy ? iy[1] : iy[0], oz ? iz[1] : iz[0]);
+      const float a = ox ? wt.x : 1.0f - wt.x;
+      const float b = oy ? wt.y : 1.0f - wt.y;
+      const float g = oz ? wt.z : 1.0f - wt.z;
+      const float coef = a * b * g;
+      const uint64_t sw = (w >> 56) == 0 ? w0 : w;
+      dist += coef * vox_sdf(sw);
+      if (col) {
+        const uint32_t rgbw = vox_rgbw(sw);
+        c0 += coef * (float)(rgbw & 0xff);
+        c1 += coef * (float)((rgbw >> 8) & 0xff);
+        c2 += coef * (float)((rgbw >> 16) & 0xff);
+      }
+    }
+    sdf_out = dist;
+    if (col) { col[0] = c0; col[1] = c1; col[2] = c2; }
+    return true;
+  }
+  if (lox < W.wx || hix > W.wx + 1 || loy < W.wy || hiy > W.wy + 1 || loz < W.wz || hiz > W.wz + 1) {
+    W.wx = lox - (hix == lox ? W.dx : 0);
+    W.wy = loy - (hiy == loy ? W.dy : 0);
+    W.wz = loz - (hiz == loz ? W.dz : 0);
+    W.resolved = 0u;
+  }
+  const int kx[2] = {(ix[0] >> 3) - W.wx, (ix[1] >> 3) - W.wx};
+  const int ky[2] = {2 * ((iy[0] >> 3) - W.wy), 2 * ((iy[1] >> 3) - W.wy)};
+  const int kz[2] = {4 * ((iz[0] >> 3) - W.wz), 4 * ((iz[1] >> 3) - W.wz)};
+  const int kc = ((cx >> 3) - W.wx) + 2 * ((cy >> 3) - W.wy) + 4 * ((cz >> 3) - W.wz);
+  unsigned need = 1u << kc;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ox = (0xD2 >> k) & 1, oy = (0xB4 >> k) & 1, oz = (0xE8 >> k) & 1;
+    need |= 1u << (kx[ox] + ky[oy] + kz[oz]);
+  }
+  unsigned todo = need & ~W.resolved;
+  while (todo) {
+    const int k = __builtin_ctz(todo);
+    todo &= todo - 1u;
+    W.slot[k * 256] = find_block(v, W.wx + (k & 1), W.wy + ((k >> 1) & 1), W.wz + (k >> 2));
+    W.resolved |= 1u << k;
+  }
+  const uint64_t *vox = v.vox;
+  // voxel word of (gx, gy, gz) in window slot k, 0 when the block is unallocated
+  auto word = [&](int k, int gx, int gy, int gz) -> uint64_t {
+    const int ptr = W.slot[k * 256];
+    const uint32_t vi = (uint32_t)(ptr < 0 ? 0 : ptr) * (uint32_t)kBlockVox +
+                        (uint32_t)(mod8(gx) * 64 + mod8(gy) * 8 + mod8(gz));
+    const uint64_t w = ldg(vox + vi);
+    return ptr < 0 ? 0ull : w;
+  };
+  const uint64_t w0 = word(kc, cx, cy, cz);
+  uint64_t wc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ox = (0xD2 >> k) & 1, oy = (0xB4 >> k) & 1, oz = (0xE8 >> k) & 1;
+    wc[k] = word(kx[ox] + ky[oy] + kz[oz], ix[ox], iy[oy], iz[oz]);
+  }
+  if ((w0 >> 56) == 0) return false;
+  float dist = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ox = (0xD2 >> k) & 1, oy = (0xB4 >> k) & 1, oz = (0xE8 >> k) & 1;
+    const float a = ox ? wt.x : 1.0f - wt.x;
+    const float b = oy ? wt.y : 1.0f - wt.y;
+    const float g = oz ? wt.z : 1.0f - wt.z;
+    const float coef = a * b * g;
+    const uint64_t sw = (wc[k] >> 56) == 0 ? w0 : wc[k];
+    dist += coef * vox_sdf(sw);
+    if (col) {
+      const uint32_t rgbw = vox_rgbw(sw);
+      c0 += coef * (float)(rgbw & 0xff);
+      c1 += coef * (float)((rgbw >> 8) & 0xff);
+      c2 += coef * (float)((rgbw >> 16) & 0xff);
+    }
+  }
+  sdf_out = dist;
+  if (col) { col[0] = c0; col[1] = c1; col[2] = c2; }
+  return true;
+}
+
 // Empty-space test of the raycast.  A sample is "far" when every block that
 // can hold one of the voxels GetInterpolatedVoxel reads at it is unallocated
 // or has no observed voxel below the far threshold: the sample is then
@@ -489,31 +614,13 @@ __device__ __forceinline__ float intensity_of(uint32_t r, uint32_t g, uint32_t b
 // position (ray origin + z * direction, no divisions) and a voxel range
 // widened by a margin far larger than its error, so it is conservative: a
 // sample it calls far is far for the exact evaluation too.
-struct FarCache {
-  int bx[2], by[2], bz[2];
-  bool far[2];
-  int next;
-  __device__ void init() { bx[0] = bx[1] = 0x7fffffff; by[0] = by[1] = bz[0] = bz[1] = 0; next = 0; }
-};
-
-__device__ __forceinline__ bool block_far(const DmView &v, int bx, int by, int bz, FarCache &c) {
-#pragma unroll
-  for (int k = 0; k < 2; ++k)
-    if (c.bx[k] == bx && c.by[k] == by && c.bz[k] == bz) return c.far[k];
-  const int ptr = find_block(v, bx, by, bz);
-  const bool f = ptr < 0 || v.blk_near[ptr] == 0;
-  const int k = c.next;
-  c.bx[k] = bx; c.by[k] = by; c.bz[k] = bz; c.far[k] = f;
-  c.next = k ^ 1;
-  return f;
-}
-
 // Far test of one block for the precise path: the window's near bit, or
-// the hash table (through the two-entry cache) outside the window.
-__device__ __forceinline__ bool block_far_any(const DmView &v, int bx, int by, int bz, FarCache &c) {
+// the hash table outside the window.
+__device__ __forceinline__ bool block_far_any(const DmView &v, int bx, int by, int bz) {
   uint32_t idx;
   if (win_bit(bx, by, bz, idx)) return !test_bit(v.nearmap, idx);
-  return block_far(v, bx, by, bz, c);
+  const int ptr = find_block(v, bx, by, bz);
+  return ptr < 0 || ldg(v.blk_near + ptr) == 0;
 }
 
 // Conservative empty-space run.  pv = (origin + z * direction) / vs is the
@@ -531,8 +638,7 @@ __device__ __forceinline__ bool block_far_any(const DmView &v, int bx, int by, i
 //    and the run extends over those blocks.
 // Runs end e + 1 voxel inside the region, so the exact positions up to zmax
 // keep their voxel range inside it.
-__device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, float z, FarCache &c,
-                                        float &zmax) {
+__device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, float z, float &zmax) {
   const float e = 0.05f;
   const float px = po.x + pd.x * z, py = po.y + pd.y * z, pz = po.z + pd.z * z;
   if (!(fabsf(px) < 1.0e6f && fabsf(py) < 1.0e6f && fabsf(pz) < 1.0e6f)) return false;
@@ -542,7 +648,7 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, f
   const int cx = coarse_of(bx), cy = coarse_of(by), cz = coarse_of(bz);
   int cw, cb;
   uint32_t idx;
-  if (coarse_bit(cx, cy, cz, cw, cb) && !(v.coarse[cw] & (1u << cb))) {
+  if (coarse_bit(cx, cy, cz, cw, cb) && !(ldg(v.coarse + cw) & (1u << cb))) {
     x0 = 8 * cx - 1; x1 = 8 * cx + 8; y0 = 8 * cy - 1; y1 = 8 * cy + 8; z0 = 8 * cz - 1; z1 = 8 * cz + 8;
   } else if (win_bit(bx, by, bz, idx) && !test_bit(v.dilmap, idx)) {
     x0 = bx - 1; x1 = bx + 1; y0 = by - 1; y1 = by + 1; z0 = bz - 1; z1 = bz + 1;
@@ -553,7 +659,7 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, f
     for (int qx = x0; qx <= x1; ++qx)
       for (int qy = y0; qy <= y1; ++qy)
         for (int qz = z0; qz <= z1; ++qz)
-          if (!block_far_any(v, qx, qy, qz, c)) return false;
+          if (!block_far_any(v, qx, qy, qz)) return false;
   }
   float zm = 3.0e38f;
   const float m0 = 1.0f + 2.0f * e, m1 = 7.0f - 2.0f * e;
@@ -584,19 +690,20 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
   const int i = r * v.cols + u;
   const Intr s = intr_of(v);
   const M4 T = load_m4(v.posef + PF_T);
-  const M4 Ti = load_m4(v.posef + PF_TINV);
   const float vs = v.voxel_size, tau = v.trunc;
   const float big = 0.5f * tau, small = 0.5f * vs;
   const int max_steps = (int)ceil((double)((v.max_depth - v.min_depth) / small)) + 2;
-  BlockCache c;
-  c.init();
-  FarCache fc;
-  fc.init();
   const float inv_vs = 1.0f / vs;
   // ray in voxel units for the approximate empty-space test
   const f3 po = mk(T.m[3], T.m[7], T.m[11]) * inv_vs;
   const f3 pd = rot(T, mk(((float)u - s.cx) / s.fx, ((float)r - s.cy) / s.fy, 1.0f)) * inv_vs;
   const f3 ipd = mk(pd.x != 0.f ? 1.0f / pd.x : 0.f, pd.y != 0.f ? 1.0f / pd.y : 0.f, pd.z != 0.f ? 1.0f / pd.z : 0.f);
+  __shared__ int win_lds[8 * 256];
+  BlockWin c;
+  c.wx = 0x3fffffff; c.wy = 0; c.wz = 0;
+  c.resolved = 0u;
+  c.dx = pd.x < 0.f; c.dy = pd.y < 0.f; c.dz = pd.z < 0.f;
+  c.slot = win_lds + threadIdx.x;
   float z = v.min_depth, pz = 0.f, ps = 0.f, zh = 0.f, zprev = 0.f;
   bool pobs = false, hit = false, prev_fast = false;
   for (int it = 0; it < max_steps && z < v.max_depth; ++it) {
@@ -604,7 +711,7 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
     // far samples: the step is 0.5*trunc whether they are observed or not;
     // the run of far samples up to zmax is stepped over without lookups (the
     // last one's interpolation is recomputed below if the next sample needs it)
-    if (far_run(v, po, pd, ipd, z, fc, zmax)) {
+    if (far_run(v, po, pd, ipd, z, zmax)) {
       prev_fast = true;
       zprev = z;
       z = z + big;
@@ -615,12 +722,12 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
       }
       continue;
     }
-    const f3 p = xform(T, get_point3d(i, z, s));
-    if (!interp_voxel(v, p, c, sdf, nullptr)) { pobs = false; prev_fast = false; zprev = z; z = z + big; continue; }
+    const f3 p = xform(T, get_point3d_uv(u, r, z, s));
+    if (!interp_voxel_win(v, p, c, sdf, nullptr)) { pobs = false; prev_fast = false; zprev = z; z = z + big; continue; }
     if (sdf <= 0.f) {
       if (prev_fast) {
         float s2;
-        pobs = interp_voxel(v, xform(T, get_point3d(i, zprev, s)), c, s2, nullptr);
+        pobs = interp_voxel_win(v, xform(T, get_point3d_uv(u, r, zprev, s)), c, s2, nullptr);
         ps = s2;
         pz = zprev;
       }
@@ -638,23 +745,38 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
   uint32_t cr = 0, cg = 0, cb = 0;
   float dout = 0.f;
   if (hit) {
-    const f3 pc = get_point3d(i, zh, s);
+    const f3 pc = get_point3d_uv(u, r, zh, s);
     const f3 pw = xform(T, pc);
     dout = zh;
-    float sp[3], sm[3], col[3];
+    // the six central-difference samples pw +- vs along x, y, z, then the
+    // colour sample at pw; one (not unrolled) copy of the interpolation keeps
+    // the kernel's code small
+    float gx = 0.f, gy = 0.f, gz = 0.f, splus = 0.f;
     bool ok = true;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const f3 e = mk(a == 0 ? vs : 0.0f, a == 1 ? vs : 0.0f, a == 2 ? vs : 0.0f);
-      ok &= interp_voxel(v, pw + e, c, sp[a], nullptr);
-      ok &= interp_voxel(v, pw - e, c, sm[a], nullptr);
+#pragma unroll 1
+    for (int a = 0; a < 7; ++a) {
+      const int ax = a >> 1;
+      const float d = a == 6 ? 0.0f : vs;
+      const f3 e = mk(ax == 0 ? d : 0.0f, ax == 1 ? d : 0.0f, ax == 2 ? d : 0.0f);
+      const f3 q = (a & 1) ? pw - e : pw + e;
+      float sv, col[3];
+      const bool o = interp_voxel_win(v, a == 6 ? pw : q, c, sv, col);
+      if (a == 6) {
+        if (o) { cr = to_u8(col[0]); cg = to_u8(col[1]); cb = to_u8(col[2]); }
+      } else if ((a & 1) == 0) {
+        ok &= o;
+        splus = sv;
+      } else {
+        ok &= o;
+        const float g = splus - sv;
+        if (ax == 0) gx = g; else if (ax == 1) gy = g; else gz = g;
+      }
     }
     if (ok) {
-      const f3 n = rot(Ti, normalize(mk(sp[0] - sm[0], sp[1] - sm[1], sp[2] - sm[2])));
+      const M4 Ti = load_m4(v.posef + PF_TINV);
+      const f3 n = rot(Ti, normalize(mk(gx, gy, gz)));
       nrm = make_float4(n.x, n.y, n.z, 0.f);
     }
-    float sdf;
-    if (interp_voxel(v, pw, c, sdf, col)) { cr = to_u8(col[0]); cg = to_u8(col[1]); cb = to_u8(col[2]); }
     vert = make_float4(pc.x, pc.y, pc.z, intensity_of(cr, cg, cb));
   }
   v.tvert[0][i] = vert;
