@@ -502,21 +502,36 @@ __device__ __forceinline__ uint64_t voxel_lookup(const DmView &v, int gx, int gy
   return ldg(v.vox + ((uint32_t)ptr * (uint32_t)kBlockVox + (uint32_t)(mod8(gx) * 64 + mod8(gy) * 8 + mod8(gz))));
 }
 
-__device__ __forceinline__ bool interp_voxel_win(const DmView &v, f3 p, BlockWin &W, float &sdf_out, float *col) {
-  const float vs = v.voxel_size;
-  const float inv_vs = 1.0f / vs;
-  const f3 vp = mk(p.x / vs, p.y / vs, p.z / vs);
-  const int cx = to_i32(vp.x + signf_(p.x) * 0.5f), cy = to_i32(vp.y + signf_(p.y) * 0.5f),
-            cz = to_i32(vp.z + signf_(p.z) * 0.5f);
-  const f3 dual = p - mk(vs / 2.0f, vs / 2.0f, vs / 2.0f);
-  const int ix[2] = {world_to_voxel_fast(dual.x + 0.0f, vs, inv_vs), world_to_voxel_fast(dual.x + vs, vs, inv_vs)};
-  const int iy[2] = {world_to_voxel_fast(dual.y + 0.0f, vs, inv_vs), world_to_voxel_fast(dual.y + vs, vs, inv_vs)};
-  const int iz[2] = {world_to_voxel_fast(dual.z + 0.0f, vs, inv_vs), world_to_voxel_fast(dual.z + vs, vs, inv_vs)};
+// The per-axis part of GetInterpolatedVoxel at coordinate q (the three axes
+// are independent): centre voxel (q/vs rounded half away from zero), the two
+// corner voxels (those of q - vs/2 and q - vs/2 + vs) and the trilinear
+// weight q/vs - floor(q/vs).
+struct AxisSample {
+  int c, i0, i1;
+  float w;
+};
+__device__ __forceinline__ AxisSample axis_sample(float q, float vs, float inv_vs) {
+  AxisSample a;
+  const float vq = q / vs;
+  a.c = to_i32(vq + signf_(q) * 0.5f);
+  const float dual = q - vs / 2.0f;
+  a.i0 = world_to_voxel_fast(dual + 0.0f, vs, inv_vs);
+  a.i1 = world_to_voxel_fast(dual + vs, vs, inv_vs);
+  a.w = vq - floorf(vq);
+  return a;
+}
+
+__device__ __forceinline__ bool interp_axes(const DmView &v, const AxisSample &X, const AxisSample &Y,
+                                            const AxisSample &Z, BlockWin &W, float &sdf_out, float *col) {
+  const int cx = X.c, cy = Y.c, cz = Z.c;
+  const int ix[2] = {X.i0, X.i1};
+  const int iy[2] = {Y.i0, Y.i1};
+  const int iz[2] = {Z.i0, Z.i1};
   // block range of the centre and corner voxels per axis
   const int lox = min(min(ix[0], ix[1]), cx) >> 3, hix = max(max(ix[0], ix[1]), cx) >> 3;
   const int loy = min(min(iy[0], iy[1]), cy) >> 3, hiy = max(max(iy[0], iy[1]), cy) >> 3;
   const int loz = min(min(iz[0], iz[1]), cz) >> 3, hiz = max(max(iz[0], iz[1]), cz) >> 3;
-  const f3 wt = mk(vp.x - floorf(vp.x), vp.y - floorf(vp.y), vp.z - floorf(vp.z));
+  const f3 wt = mk(X.w, Y.w, Z.w);
   if (hix - lox > 1 || hiy - loy > 1 || hiz - loz > 1) {
     // voxels over more than two blocks on an axis: one lookup per voxel (a
     // rolled loop, same order and arithmetic as below)
@@ -606,6 +621,13 @@ __device__ __forceinline__ bool interp_voxel_win(const DmView &v, f3 p, BlockWin
   return true;
 }
 
+__device__ __forceinline__ bool interp_voxel_win(const DmView &v, f3 p, BlockWin &W, float &sdf_out, float *col) {
+  const float vs = v.voxel_size;
+  const float inv_vs = 1.0f / vs;
+  return interp_axes(v, axis_sample(p.x, vs, inv_vs), axis_sample(p.y, vs, inv_vs), axis_sample(p.z, vs, inv_vs), W,
+                     sdf_out, col);
+}
+
 // Empty-space test of the raycast.  A sample is "far" when every block that
 // can hold one of the voxels GetInterpolatedVoxel reads at it is unallocated
 // or has no observed voxel below the far threshold: the sample is then
@@ -693,6 +715,8 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
   const float vs = v.voxel_size, tau = v.trunc;
   const float big = 0.5f * tau, small = 0.5f * vs;
   const int max_steps = (int)ceil((double)((v.max_depth - v.min_depth) / small)) + 2;
+  const int it_last = max_steps - 1;
+  const float zpre_max = nextafterf(v.max_depth, -INFINITY);  // largest float below max_depth
   const float inv_vs = 1.0f / vs;
   // ray in voxel units for the approximate empty-space test
   const f3 po = mk(T.m[3], T.m[7], T.m[11]) * inv_vs;
@@ -715,7 +739,8 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
       prev_fast = true;
       zprev = z;
       z = z + big;
-      while (z <= zmax && it + 1 < max_steps && z < v.max_depth) {
+      const float zlim = fminf(zmax, zpre_max);  // z < max_depth <=> z <= zpre_max
+      while (z <= zlim && it < it_last) {
         ++it;
         zprev = z;
         z = z + big;
@@ -748,29 +773,29 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
     const f3 pc = get_point3d_uv(u, r, zh, s);
     const f3 pw = xform(T, pc);
     dout = zh;
-    // the six central-difference samples pw +- vs along x, y, z, then the
-    // colour sample at pw; one (not unrolled) copy of the interpolation keeps
-    // the kernel's code small
-    float gx = 0.f, gy = 0.f, gz = 0.f, splus = 0.f;
+    // The six central-difference samples pw +- vs e_a and the colour sample
+    // at pw.  The axes are independent in the interpolation, and an axis
+    // without offset (pw.y + 0.0f for the x samples) gives the same axis
+    // sample as pw itself, so 9 axis samples serve the 7 interpolations.
+    // One (not unrolled) copy of the gather keeps the kernel's code small.
+    const float inv_vs = 1.0f / vs;
+    const AxisSample X0 = axis_sample(pw.x, vs, inv_vs), Y0 = axis_sample(pw.y, vs, inv_vs),
+                     Z0 = axis_sample(pw.z, vs, inv_vs);
+    float gx = 0.f, gy = 0.f, gz = 0.f;
     bool ok = true;
 #pragma unroll 1
-    for (int a = 0; a < 7; ++a) {
-      const int ax = a >> 1;
-      const float d = a == 6 ? 0.0f : vs;
-      const f3 e = mk(ax == 0 ? d : 0.0f, ax == 1 ? d : 0.0f, ax == 2 ? d : 0.0f);
-      const f3 q = (a & 1) ? pw - e : pw + e;
+    for (int ax = 0; ax < 3; ++ax) {
+      const float q = ax == 0 ? pw.x : ax == 1 ? pw.y : pw.z;
+      const AxisSample P = axis_sample(q + vs, vs, inv_vs), M = axis_sample(q - vs, vs, inv_vs);
+      float sp, sm;
+      ok &= interp_axes(v, ax == 0 ? P : X0, ax == 1 ? P : Y0, ax == 2 ? P : Z0, c, sp, nullptr);
+      ok &= interp_axes(v, ax == 0 ? M : X0, ax == 1 ? M : Y0, ax == 2 ? M : Z0, c, sm, nullptr);
+      const float g = sp - sm;
+      if (ax == 0) gx = g; else if (ax == 1) gy = g; else gz = g;
+    }
+    {
       float sv, col[3];
-      const bool o = interp_voxel_win(v, a == 6 ? pw : q, c, sv, col);
-      if (a == 6) {
-        if (o) { cr = to_u8(col[0]); cg = to_u8(col[1]); cb = to_u8(col[2]); }
-      } else if ((a & 1) == 0) {
-        ok &= o;
-        splus = sv;
-      } else {
-        ok &= o;
-        const float g = splus - sv;
-        if (ax == 0) gx = g; else if (ax == 1) gy = g; else gz = g;
-      }
+      if (interp_axes(v, X0, Y0, Z0, c, sv, col)) { cr = to_u8(col[0]); cg = to_u8(col[1]); cb = to_u8(col[2]); }
     }
     if (ok) {
       const M4 Ti = load_m4(v.posef + PF_TINV);
