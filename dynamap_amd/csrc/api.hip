@@ -28,9 +28,9 @@ static int pow2_at_least(int x) {
 
 int validate_props(const dm_tsdf_props &p) {
   if (!(p.voxel_size > 0.f) || p.num_buckets <= 0 || p.bucket_size <= 0 || p.bucket_size > 64 ||
-      p.num_blocks <= 0 || p.num_blocks > (1 << 23) || p.block_size != 8 || p.max_sdf_weight < 1 || p.max_sdf_weight > 255 ||
+      p.num_blocks <= 0 || p.num_blocks >= (1 << 23) || p.block_size != 8 || p.max_sdf_weight < 1 || p.max_sdf_weight > 255 ||
       !(p.truncation_distance > 0.f) || !(p.max_sensor_depth > p.min_sensor_depth)) {
-    set_error("invalid tsdf properties (block_size must be 8, bucket_size 1..64, num_blocks 1..2^23, "
+    set_error("invalid tsdf properties (block_size must be 8, bucket_size 1..64, num_blocks 1..2^23-1, "
               "max weight 1..255)");
     return DM_E_INVALID;
   }
@@ -47,7 +47,8 @@ size_t session_bytes(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   const size_t P = (size_t)rows * cols;
   const size_t nc = (size_t)new_cap(p.num_blocks);
   size_t b = (size_t)p.num_buckets * bucket_stride(p.bucket_size);
-  b += (size_t)p.num_blocks * (kBlockVox * 8 + 16 + 1) + ((size_t)kCoarseWords + 2 * kWinWords) * 4;
+  b += ((size_t)p.num_blocks + 1) * kBlockVox * 8 + (size_t)p.num_blocks * (16 + 1);
+  b += ((size_t)kCoarseWords + 2 * kWinWords) * 4;
   b += (size_t)pow2_at_least((int)(2 * nc)) * 8 + nc * (8 + 4 * 4) + (size_t)p.num_buckets * 4;
   size_t maps = 0;
   for (int l = 0; l < kLevels; ++l) maps += (size_t)(rows >> l) * (cols >> l) * 16 * 4;
@@ -94,7 +95,9 @@ int Sess::init(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   h.dset_mask = dsz - 1;
   const size_t P = (size_t)rows * cols;
   DM_TRY(dalloc(&h.buckets, (size_t)p.num_buckets * h.bstride));
-  DM_TRY(dalloc(&h.vox, (size_t)p.num_blocks * kBlockVox));
+  // one block more than the pool: block num_blocks stays all zero and stands
+  // for unallocated blocks in the raycast's gathers
+  DM_TRY(dalloc(&h.vox, ((size_t)p.num_blocks + 1) * kBlockVox));
   DM_TRY(dalloc(&h.block_key, (size_t)p.num_blocks));
   DM_TRY(dalloc(&h.blk_near, (size_t)p.num_blocks));
   DM_TRY(dalloc(&h.nearmap, kWinWords));

@@ -176,13 +176,15 @@ __device__ __forceinline__ uint64_t get_voxel(const DmView &v, f3 p, BlockCache 
 // world_to_voxel(x, vs) without a division in the common case: x * (1/vs)
 // is within a few ulp of x / vs, so trunc(q + sign(x) * 0.5) is the same
 // integer unless q + sign(x) * 0.5 lies within ~8 ulp of an integer; then
-// (and for huge or zero values) the exact expression is evaluated.
+// (and for huge values) the exact expression is evaluated.  copysignf(0.5f,
+// x) stands in for signf_(x) * 0.5f: they differ only at x = +-0, where both
+// truncate to voxel 0.
 __device__ __forceinline__ int world_to_voxel_fast(float x, float vs, float inv_vs) {
-  const float t = x * inv_vs + signf_(x) * 0.5f;
+  const float t = x * inv_vs + copysignf(0.5f, x);
   const float tr = truncf(t);
   const float fr = fabsf(t - tr);
   const float tol = fabsf(t) * 1.0e-6f;
-  if (!(fabsf(t) < 1.0e8f) || fr <= tol || fr >= 1.0f - tol || x == 0.0f) return world_to_voxel(x, vs);
+  if (!(fabsf(t) < 1.0e8f) || fr <= tol || fr >= 1.0f - tol) return world_to_voxel(x, vs);
   return (int)tr;
 }
 

@@ -485,7 +485,8 @@ __device__ __forceinline__ float intensity_of(uint32_t r, uint32_t g, uint32_t b
 // the block pointers of the samples kept in a per-thread window of 2x2x2
 // blocks.  The window's pool indices sit in LDS (thread-strided), a register
 // bit per slot says which are looked up, and a slot is filled from the hash
-// table on first use; the nine voxel words are then loaded together.  A
+// table on first use, an unallocated block with the index of the all-zero
+// block past the pool; the nine voxel words are then loaded together.  A
 // sample whose voxels span more than two blocks on an axis (possible only
 // through rounding at huge coordinates) looks its voxels up one by one.
 struct BlockWin {
@@ -513,7 +514,7 @@ struct AxisSample {
 __device__ __forceinline__ AxisSample axis_sample(float q, float vs, float inv_vs) {
   AxisSample a;
   const float vq = q / vs;
-  a.c = to_i32(vq + signf_(q) * 0.5f);
+  a.c = to_i32(vq + copysignf(0.5f, q));  // = signf_(q) * 0.5f but at q = +-0, where both give 0
   const float dual = q - vs / 2.0f;
   a.i0 = world_to_voxel_fast(dual + 0.0f, vs, inv_vs);
   a.i1 = world_to_voxel_fast(dual + vs, vs, inv_vs);
@@ -579,17 +580,16 @@ __device__ __forceinline__ bool interp_axes(const DmView &v, const AxisSample &X
   while (todo) {
     const int k = __builtin_ctz(todo);
     todo &= todo - 1u;
-    W.slot[k * 256] = find_block(v, W.wx + (k & 1), W.wy + ((k >> 1) & 1), W.wz + (k >> 2));
+    const int ptr = find_block(v, W.wx + (k & 1), W.wy + ((k >> 1) & 1), W.wz + (k >> 2));
+    W.slot[k * 256] = ptr < 0 ? v.num_blocks : ptr;  // unallocated: the pool's all-zero block
     W.resolved |= 1u << k;
   }
   const uint64_t *vox = v.vox;
   // voxel word of (gx, gy, gz) in window slot k, 0 when the block is unallocated
   auto word = [&](int k, int gx, int gy, int gz) -> uint64_t {
-    const int ptr = W.slot[k * 256];
-    const uint32_t vi = (uint32_t)(ptr < 0 ? 0 : ptr) * (uint32_t)kBlockVox +
+    const uint32_t vi = (uint32_t)W.slot[k * 256] * (uint32_t)kBlockVox +
                         (uint32_t)(mod8(gx) * 64 + mod8(gy) * 8 + mod8(gz));
-    const uint64_t w = ldg(vox + vi);
-    return ptr < 0 ? 0ull : w;
+    return ldg(vox + vi);
   };
   const uint64_t w0 = word(kc, cx, cy, cz);
   uint64_t wc[8];
