@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--no-nonrigid", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--nonrigid-frames", type=int, default=20)
+    ap.add_argument("--no-mesh", action="store_true")
     return ap.parse_args()
 
 
@@ -117,6 +118,49 @@ def bench_nonrigid(dm, nframes, with_cpu):
                                    cores=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
                                    sample=f"FP64 oracle non-rigid solve, frames 0..{k - 1}")
     g.close()
+    return res
+
+
+def bench_mesh(dm, with_cpu, reps=5):
+    """Marching cubes of a 640x480 / 8 mm room volume (30 integrated frames),
+    every allocated block meshed; the extraction call includes its host
+    copies of positions, colours and edge ids."""
+    from dynamap_amd import configs
+    name = "rigid-640x480-8mm"
+    w = configs.WORKLOADS[name]
+    depth, rgb, gt = configs.make_sequence(name, 30, session=8)
+    vol = dm.TsdfVolume(w["tsdf"], w["sensor"]["rows"], w["sensor"]["cols"])
+    for f in range(30):
+        vol.IntegrateScan(depth[f], None if rgb is None else rgb[f], w["sensor"], gt[f].astype(np.float32))
+    m = vol.ExtractMesh(max_triangles=1 << 23)
+    t0 = time.time()
+    for _ in range(reps):
+        m = vol.ExtractMesh(max_triangles=1 << 23)
+    el = (time.time() - t0) / reps
+    nb = vol.GetNumAllocatedBlocks()
+    res = dict(workload="640x480 / 8 mm room, 30 frames integrated, all allocated blocks", blocks=nb,
+               triangles=len(m), ms_per_extraction=1000 * el, triangles_per_s=len(m) / el,
+               voxel_bytes=nb * 512 * 8)
+    log(f"mesh: {len(m)} triangles from {nb} blocks in {1000 * el:.2f} ms per extraction (host copies included)")
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        ent = vol.GetHashEntries()
+        _, gs, gc, gw = vol.blocks()
+        occ = ent[ent[:, 3] >= 0]
+        mir = O.OracleVolume(w["tsdf"], 32)
+        mir.set_blocks(occ[:, :3], gs[occ[:, 3]].astype(np.float64), gw[occ[:, 3]], gc[occ[:, 3]])
+        t0 = time.time()
+        op, _, oe = O.mc_extract(mir)
+        elc = time.time() - t0
+        res["bit_exact_vs_oracle"] = bool(len(op) == len(m) and np.array_equal(oe, m.edge_ids)
+                                          and np.array_equal(op.view(np.uint32), m.triangles.view(np.uint32)))
+        L32 = O.load(32)
+        L32.or_get_threads.restype = C.c_int
+        res["cpu_baseline"] = dict(value=1000 * elc, unit="ms per extraction", kind="port",
+                                   cores=int(L32.or_get_threads()), sample="FP32 oracle marching cubes, same volume")
+        log(f"mesh: oracle {1000 * elc:.1f} ms, bit-exact {res['bit_exact_vs_oracle']}")
+    vol.close()
     return res
 
 
@@ -265,6 +309,9 @@ def main():
     # ---- non-rigid workload: bending cylinder, 200k-vertex mesh, 640x480
     if not args.no_nonrigid:
         out["nonrigid"] = bench_nonrigid(dm, args.nonrigid_frames, not args.no_cpu)
+    # ---- marching cubes of one room volume
+    if not args.no_mesh:
+        out["mesh"] = bench_mesh(dm, not args.no_cpu)
 
     res = {
         "metric": "aggregate frames/s + p50/p99 frame latency, batched vs per-frame, ATE≤1mm",

@@ -104,64 +104,101 @@ __device__ __forceinline__ bool halo_voxel(const DmView &v, const Halo &H, int k
   return (w >> 56) != 0;
 }
 
-// TrilinearInterpolation (meshExtractor.cu:22-93), strict: false when any of
-// the 8 voxels is unobserved.
-__device__ __forceinline__ bool trilinear_strict(const DmView &v, const Halo &H, int kx, int ky, int kz, f3 p,
-                                                 float &dist) {
-  const float vs = v.voxel_size;
-  const f3 dual = p - mk(vs / 2.f, vs / 2.f, vs / 2.f);
-  const f3 vp = mk(p.x / vs, p.y / vs, p.z / vs);
-  const f3 wt = mk(vp.x - floorf(vp.x), vp.y - floorf(vp.y), vp.z - floorf(vp.z));
-  // corner c: offsets (cx, cy, cz) = (0,0,0) (1,0,0) (0,1,0) (0,0,1) (1,1,0) (0,1,1) (1,0,1) (1,1,1)
-  constexpr uint8_t cxm = 0xD2, cym = 0xB4, czm = 0xE8;
-  float d = 0.f;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const bool bx = (cxm >> c) & 1, by = (cym >> c) & 1, bz = (czm >> c) & 1;
-    const f3 q = dual + mk(bx ? vs : 0.f, by ? vs : 0.f, bz ? vs : 0.f);
-    float s;
-    if (!halo_voxel(v, H, kx, ky, kz, world_to_voxel(q.x, vs), world_to_voxel(q.y, vs), world_to_voxel(q.z, vs), s))
-      return false;
-    const float a = bx ? wt.x : 1.f - wt.x;
-    const float b = by ? wt.y : 1.f - wt.y;
-    const float g = bz ? wt.z : 1.f - wt.z;
-    d += a * b * g * s;
-  }
-  dist = d;
-  return true;
-}
-
 struct BoxArg {
   int use;
   double lo[3], hi[3];
 };
 
 // Corner positions and samples of cell (gx, gy, gz); false when the cell is
-// outside the box or a sample is missing.
-__device__ __forceinline__ bool cell_samples(const DmView &v, const Halo &H, int kx, int ky, int kz, int gx, int gy,
-                                             int gz, const BoxArg &box, f3 pc[8], float d[8]) {
+// outside the box or a sample is missing.  Each sample is
+// TrilinearInterpolation (meshExtractor.cu:22-93), strict: false when any of
+// its 8 voxels is unobserved.  The corner samples sit at c +- vs/2 per axis,
+// and every per-axis quantity of the interpolation (the voxel indices of
+// dual = p - vs/2 and dual + vs, the weight p/vs - floor(p/vs)) depends only
+// on the axis and the sign, so it is evaluated once per (axis, sign) and
+// combined: the same arithmetic as interpolating each corner on its own.
+struct AxisCorner {
+  float p;     // corner coordinate c +- vs/2
+  int i0, i1;  // voxel indices of dual + 0 and dual + vs
+  float w;     // trilinear weight
+};
+__device__ __forceinline__ AxisCorner axis_corner(float c, float off, float vs) {
+  AxisCorner a;
+  a.p = c + off;
+  const float dual = a.p - vs / 2.f;
+  a.i0 = world_to_voxel(dual + 0.f, vs);
+  a.i1 = world_to_voxel(dual + vs, vs);
+  const float vp = a.p / vs;
+  a.w = vp - floorf(vp);
+  return a;
+}
+
+__device__ __forceinline__ bool halo_axis_ok(const AxisCorner *A, int k) {
+  const int lo = 8 * k - 1, hi = 8 * k + 8;
+  return A[0].i0 >= lo && A[0].i1 >= lo && A[1].i0 >= lo && A[1].i1 >= lo && A[0].i0 <= hi && A[0].i1 <= hi &&
+         A[1].i0 <= hi && A[1].i1 <= hi;
+}
+
+// The 8 corner samples of a cell (kWant: positions and values, else only
+// the cube index).  Each voxel comes from the block's LDS halo; a cell whose
+// indices leave the halo (only through rounding at huge coordinates) reads
+// them through the hash table in a rolled loop.
+template <bool kWant>
+__device__ __forceinline__ bool cell_eval(const DmView &v, const Halo &H, int kx, int ky, int kz, int gx, int gy,
+                                          int gz, const BoxArg &box, f3 *pc, float *d, int &cube) {
   const float vs = v.voxel_size;
   const f3 c = mk((float)gx * vs + 0.f, (float)gy * vs + 0.f, (float)gz * vs + 0.f);
   if (box.use && ((double)c.x < box.lo[0] || (double)c.y < box.lo[1] || (double)c.z < box.lo[2] ||
                   (double)c.x > box.hi[0] || (double)c.y > box.hi[1] || (double)c.z > box.hi[2]))
     return false;
   const float P = vs / 2.f, M = -P;
-  bool ok = true;
+  const AxisCorner X[2] = {axis_corner(c.x, M, vs), axis_corner(c.x, P, vs)};
+  const AxisCorner Y[2] = {axis_corner(c.y, M, vs), axis_corner(c.y, P, vs)};
+  const AxisCorner Z[2] = {axis_corner(c.z, M, vs), axis_corner(c.z, P, vs)};
+  // corner c: offsets (cx, cy, cz) = (0,0,0) (1,0,0) (0,1,0) (0,0,1) (1,1,0) (0,1,1) (1,0,1) (1,1,1)
+  constexpr uint8_t cxm = 0xD2, cym = 0xB4, czm = 0xE8;
+  cube = 0;
+  if (halo_axis_ok(X, kx) && halo_axis_ok(Y, ky) && halo_axis_ok(Z, kz)) {
+    const int bx = 1 - 8 * kx, by = 1 - 8 * ky, bz = 1 - 8 * kz;
 #pragma unroll
-  for (int ci = 0; ci < 8; ++ci) {
-    pc[ci] = c + mk((ci & 4) ? P : M, (ci & 2) ? P : M, (ci & 1) ? P : M);
-    ok = ok && trilinear_strict(v, H, kx, ky, kz, pc[ci], d[ci]);
+    for (int ci = 0; ci < 8; ++ci) {
+      const AxisCorner &ax = X[(ci >> 2) & 1], &ay = Y[(ci >> 1) & 1], &az = Z[ci & 1];
+      float dd = 0.f;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const bool ox = (cxm >> k) & 1, oy = (cym >> k) & 1, oz = (czm >> k) & 1;
+        const int h = ((ox ? ax.i1 : ax.i0) + bx) * 100 + ((oy ? ay.i1 : ay.i0) + by) * 10 + (oz ? az.i1 : az.i0) + bz;
+        if (!H.obs[h]) return false;
+        const float a = ox ? ax.w : 1.f - ax.w;
+        const float b = oy ? ay.w : 1.f - ay.w;
+        const float g = oz ? az.w : 1.f - az.w;
+        dd += a * b * g * H.sdf[h];
+      }
+      if (kWant) { pc[ci] = mk(ax.p, ay.p, az.p); d[ci] = dd; }
+      if (dd < 0.f) cube += c_corner_bit[ci];
+    }
+    return true;
   }
-  return ok;
+#pragma unroll 1
+  for (int ci = 0; ci < 8; ++ci) {
+    const AxisCorner &ax = X[(ci >> 2) & 1], &ay = Y[(ci >> 1) & 1], &az = Z[ci & 1];
+    float dd = 0.f;
+#pragma unroll 1
+    for (int k = 0; k < 8; ++k) {
+      const bool ox = (cxm >> k) & 1, oy = (cym >> k) & 1, oz = (czm >> k) & 1;
+      float sv;
+      if (!halo_voxel(v, H, kx, ky, kz, ox ? ax.i1 : ax.i0, oy ? ay.i1 : ay.i0, oz ? az.i1 : az.i0, sv)) return false;
+      const float a = ox ? ax.w : 1.f - ax.w;
+      const float b = oy ? ay.w : 1.f - ay.w;
+      const float g = oz ? az.w : 1.f - az.w;
+      dd += a * b * g * sv;
+    }
+    if (kWant) { pc[ci] = mk(ax.p, ay.p, az.p); d[ci] = dd; }
+    if (dd < 0.f) cube += c_corner_bit[ci];
+  }
+  return true;
 }
 
-__device__ __forceinline__ int cube_index(const float d[8]) {
-  int cube = 0;
-#pragma unroll
-  for (int ci = 0; ci < 8; ++ci)
-    if (d[ci] < 0.f) cube += c_corner_bit[ci];
-  return cube;
-}
 
 __global__ __launch_bounds__(kCellThreads) void k_mc_count(const DmView *__restrict__ vp, const int *__restrict__ list,
                                                            BoxArg box, uint8_t *__restrict__ cubes,
@@ -174,11 +211,9 @@ __global__ __launch_bounds__(kCellThreads) void k_mc_count(const DmView *__restr
   load_halo(v, key.x, key.y, key.z, H);
   const int li = threadIdx.x;
   const int gx = key.x * 8 + (li >> 6), gy = key.y * 8 + ((li >> 3) & 7), gz = key.z * 8 + (li & 7);
-  f3 pc[8];
-  float d[8];
-  int cube = 0, ntri = 0;
-  if (cell_samples(v, H, key.x, key.y, key.z, gx, gy, gz, box, pc, d)) {
-    cube = cube_index(d);
+  int cube = 0, ntri = 0, cb;
+  if (cell_eval<false>(v, H, key.x, key.y, key.z, gx, gy, gz, box, nullptr, nullptr, cb)) {
+    cube = cb;
     ntri = mc_num_tri(c_mc_tri[cube]);
     if (ntri == 0) cube = 0;
   }
@@ -230,7 +265,8 @@ __global__ __launch_bounds__(kCellThreads) void k_mc_emit(const DmView *__restri
   const int gx = key.x * 8 + (li >> 6), gy = key.y * 8 + ((li >> 3) & 7), gz = key.z * 8 + (li & 7);
   f3 pc[8];
   float d[8];
-  cell_samples(v, H, key.x, key.y, key.z, gx, gy, gz, box, pc, d);  // true: the count pass kept this cell
+  int cube_e;
+  cell_eval<true>(v, H, key.x, key.y, key.z, gx, gy, gz, box, pc, d, cube_e);  // true: the count pass kept it
   const uint32_t rgbw = vox_rgbw(v.vox[(size_t)ptr * kBlockVox + li]);
   const float cr = (float)(rgbw & 0xFF) / 255.f, cg = (float)((rgbw >> 8) & 0xFF) / 255.f,
               cb = (float)((rgbw >> 16) & 0xFF) / 255.f;

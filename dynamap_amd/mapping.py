@@ -220,14 +220,15 @@ class MeshExtractor:
         hi = np.ascontiguousarray(upper_corner if box else [0, 0, 0], np.float64)
         n = C.c_int64()
         h = volume._h
-        N.check(self._lib.dm_mesh_extract(h, N.ptr(lo) if box else None, N.ptr(hi) if box else None, 0,
-                                          None, None, None, C.byref(n)))
+        # one pass into buffers of the extractor's capacity (untouched pages
+        # of the zeroed arrays cost no memory)
+        cap = max(int(self.max_triangles), 1)
+        pos = np.zeros((cap, 3, 3), np.float32)
+        col = np.zeros((cap, 3, 3), np.float32)
+        eid = np.zeros((cap, 3), np.uint64)
+        N.check(self._lib.dm_mesh_extract(h, N.ptr(lo) if box else None, N.ptr(hi) if box else None,
+                                          int(self.max_triangles), N.ptr(pos), N.ptr(col), N.ptr(eid), C.byref(n)))
         m = min(n.value, self.max_triangles)
-        pos = np.zeros((max(m, 1), 3, 3), np.float32)
-        col = np.zeros((max(m, 1), 3, 3), np.float32)
-        eid = np.zeros((max(m, 1), 3), np.uint64)
-        N.check(self._lib.dm_mesh_extract(h, N.ptr(lo) if box else None, N.ptr(hi) if box else None, m,
-                                          N.ptr(pos), N.ptr(col), N.ptr(eid), C.byref(n)))
         self._mesh = Mesh(pos[:m], col[:m], eid[:m], n.value, self.max_triangles)
         return self._mesh
 
