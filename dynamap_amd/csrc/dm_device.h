@@ -167,12 +167,6 @@ __device__ __forceinline__ uint64_t voxel_at(const DmView &v, int gx, int gy, in
   return ldg(v.vox + ((size_t)c.ptr * kBlockVox + mod8(gx) * 64 + mod8(gy) * 8 + mod8(gz)));
 }
 
-// GetVoxel (tsdf.cu:227-239); returns the voxel word, 0 when unallocated.
-__device__ __forceinline__ uint64_t get_voxel(const DmView &v, f3 p, BlockCache &c) {
-  const float vs = v.voxel_size;
-  return voxel_at(v, world_to_voxel(p.x, vs), world_to_voxel(p.y, vs), world_to_voxel(p.z, vs), c);
-}
-
 // world_to_voxel(x, vs) without a division in the common case: x * (1/vs)
 // is within a few ulp of x / vs, so trunc(q + sign(x) * 0.5) is the same
 // integer unless q + sign(x) * 0.5 lies within ~8 ulp of an integer; then
@@ -186,47 +180,6 @@ __device__ __forceinline__ int world_to_voxel_fast(float x, float vs, float inv_
   const float tol = fabsf(t) * 1.0e-6f;
   if (!(fabsf(t) < 1.0e8f) || fr <= tol || fr >= 1.0f - tol) return world_to_voxel(x, vs);
   return (int)tr;
-}
-
-// GetInterpolatedVoxel (tsdf.cu:47-175).  Returns false when the centre
-// voxel is unobserved.  A corner of weight 0 contributes the centre voxel.
-// The centre voxel reuses the quotients p / vs of the trilinear weights;
-// the corner indices (per axis those of p - vs/2 and p - vs/2 + vs) use
-// world_to_voxel_fast; both give the reference's integers exactly.
-__device__ __forceinline__ bool interp_voxel(const DmView &v, f3 p, BlockCache &c, float &sdf_out,
-                                             float *col) {
-  const float vs = v.voxel_size;
-  const float inv_vs = 1.0f / vs;
-  const f3 vp = mk(p.x / vs, p.y / vs, p.z / vs);
-  const uint64_t w0 = voxel_at(v, to_i32(vp.x + signf_(p.x) * 0.5f), to_i32(vp.y + signf_(p.y) * 0.5f),
-                               to_i32(vp.z + signf_(p.z) * 0.5f), c);
-  if ((w0 >> 56) == 0) return false;
-  const f3 dual = p - mk(vs / 2.0f, vs / 2.0f, vs / 2.0f);
-  const int ix[2] = {world_to_voxel_fast(dual.x + 0.0f, vs, inv_vs), world_to_voxel_fast(dual.x + vs, vs, inv_vs)};
-  const int iy[2] = {world_to_voxel_fast(dual.y + 0.0f, vs, inv_vs), world_to_voxel_fast(dual.y + vs, vs, inv_vs)};
-  const int iz[2] = {world_to_voxel_fast(dual.z + 0.0f, vs, inv_vs), world_to_voxel_fast(dual.z + vs, vs, inv_vs)};
-  const f3 wt = mk(vp.x - floorf(vp.x), vp.y - floorf(vp.y), vp.z - floorf(vp.z));
-  float dist = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int ox = (0xD2 >> k) & 1, oy = (0xB4 >> k) & 1, oz = (0xE8 >> k) & 1;
-    const uint64_t w = voxel_at(v, ix[ox], iy[oy], iz[oz], c);
-    const float a = ox ? wt.x : 1.0f - wt.x;
-    const float b = oy ? wt.y : 1.0f - wt.y;
-    const float g = oz ? wt.z : 1.0f - wt.z;
-    const float coef = a * b * g;
-    const uint64_t s = (w >> 56) == 0 ? w0 : w;
-    dist += coef * vox_sdf(s);
-    if (col) {
-      const uint32_t rgbw = vox_rgbw(s);
-      c0 += coef * (float)(rgbw & 0xff);
-      c1 += coef * (float)((rgbw >> 8) & 0xff);
-      c2 += coef * (float)((rgbw >> 16) & 0xff);
-    }
-  }
-  sdf_out = dist;
-  if (col) { col[0] = c0; col[1] = c1; col[2] = c2; }
-  return true;
 }
 
 __device__ __forceinline__ uint32_t to_u8(float x) {

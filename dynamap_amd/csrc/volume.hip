@@ -481,8 +481,12 @@ __device__ __forceinline__ float intensity_of(uint32_t r, uint32_t g, uint32_t b
   return (0.2126f * (float)r + 0.7152f * (float)g + 0.0722f * (float)b) * (1.0f / 255.0f);
 }
 
-// GetInterpolatedVoxel for the raycast: the arithmetic of interp_voxel, with
-// the block pointers of the samples kept in a per-thread window of 2x2x2
+// GetInterpolatedVoxel (tsdf.cu:47-175) for the raycast.  Returns false when
+// the centre voxel (p/vs rounded half away from zero) is unobserved; the 8
+// corners are the voxels of p - vs/2 and p - vs/2 + vs per axis, weighted by
+// p/vs - floor(p/vs), and an unobserved corner (integration weight 0)
+// contributes the centre voxel instead.
+// The block pointers of the samples are kept in a per-thread window of 2x2x2
 // blocks.  The window's pool indices sit in LDS (thread-strided), a register
 // bit per slot says which are looked up, and a slot is filled from the hash
 // table on first use, an unallocated block with the index of the all-zero
