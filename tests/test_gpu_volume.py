@@ -148,3 +148,46 @@ def test_raycast_parity(seq):
     okn = both & np.isfinite(on).all(-1) & np.isfinite(gn).all(-1)
     np.testing.assert_allclose(gn[okn], on[okn], atol=1e-4)
     assert np.mean(np.all(gc[both] == oc[both], -1)) > 0.99
+
+
+def mirror(vol, props):
+    """FP32 transcription volume holding exactly the GPU volume's blocks."""
+    ent = vol.GetHashEntries()
+    _, gs, gc, gw = vol.blocks()
+    occ = ent[ent[:, 3] >= 0]
+    mir = O.OracleVolume(props, 32)
+    mir.set_blocks(occ[:, :3], gs[occ[:, 3]].astype(np.float64), gw[occ[:, 3]], gc[occ[:, 3]])
+    return mir
+
+
+# Scene at the origin, and shifted by +38.9 m in x so that it straddles the
+# edge of the raycast's 512^3-block bitmap window (block 256 starts at
+# 40.96 m at 2 cm voxels): blocks beyond the edge take the hash-table
+# branches of the empty-space test and the clipped dilation.
+@pytest.mark.parametrize("shift", [0.0, 38.9])
+def test_raycast_on_identical_volume(seq, shift):
+    depth, rgb, poses = seq
+    props = small_tsdf()
+    off = np.eye(4)
+    off[0, 3] = shift
+    P = np.stack([off @ p for p in poses])
+    vol = dm.TsdfVolume(props, SENS["rows"], SENS["cols"])
+    for f in range(3):
+        vol.IntegrateScan(depth[f], rgb[f], SENS, P[f].astype(np.float32))
+    keys = vol.GetHashEntries()
+    bx = keys[keys[:, 3] >= 0, 0]
+    if shift:
+        assert (bx >= 256).any() and (bx < 256).any(), "scene does not straddle the window edge"
+    mir = mirror(vol, props)
+    Tf = P[2].astype(np.float32)
+    gd, gv, gn, gc = vol.Raycast(Tf, SENS)
+    od, ov, on, oc = mir.raycast(Tf.astype(np.float64), SENS)
+    hit = od > 0
+    assert hit.mean() > 0.8
+    print("shift", shift, "hit", hit.mean(), "depth equal", np.mean(gd == od), "max diff", np.abs(gd - od).max())
+    assert np.array_equal(gd, od), "raycast depth differs from the transcription on the same volume"
+    np.testing.assert_allclose(gv[hit], ov[hit], atol=1e-6, rtol=0)
+    okn = hit & np.isfinite(on).all(-1)
+    assert np.array_equal(np.isfinite(gn).all(-1), np.isfinite(on).all(-1))
+    np.testing.assert_allclose(gn[okn], on[okn], atol=1e-6)
+    assert np.array_equal(gc, oc)
