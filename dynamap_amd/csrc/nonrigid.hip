@@ -610,11 +610,17 @@ __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk
 }
 
 // ---------------------------------------------------------------- PCG
+// Block-Jacobi PCG in the Chronopoulos-Gear form: with u = M r and w = A u,
+// s = A p is carried along (s = w + beta s), so an iteration is one SpMV
+// kernel (w = A u, partials of w.u) and one update kernel (alpha, beta from
+// the partials; p, s, x, r, u; partials of r.u).  Mathematically the
+// standard PCG iteration, with one kernel boundary less per iteration.
 struct PcgState {
-  double *x, *r, *z, *p, *q;  // 6 N each
-  double *part_pq;            // per-workgroup partials of p.q (written by spmv, read by update)
-  double *part_rz;            // per-workgroup partials of r.z (written by init/update, read by dir)
-  double *rz;                 // rz[k]: r.z after iteration k (k = 0 .. max)
+  double *x, *r, *z, *p, *q, *s;  // 6 N each; z = u = M r, q = w = A u, s = A p
+  double *part_pq;            // per-workgroup partials of w.u (written by spmv, read by update)
+  double *part_rz;            // per-workgroup partials of r.u (written by init/update, read by update)
+  double *rz;                 // rz[k] = r_k . u_k (k = 0 .. max)
+  double *al;                 // al[k]: step length of iteration k
   int *flag;                  // [0] converged, [1] iterations run
   int nwg;
 };
@@ -665,22 +671,13 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgState S, const doub
   double z = 0.0;
   if (act) {
     z = precond_row(Minv, rl + (threadIdx.x - row), node, row);
-    S.x[t] = 0.0; S.r[t] = r; S.z[t] = z; S.p[t] = z;
+    S.x[t] = 0.0; S.r[t] = r; S.z[t] = z;
   }
   wg_partial(r * z, lds, S.part_rz);
   if (t == 0) { S.flag[0] = 0; S.flag[1] = 0; }
 }
 
-__global__ __launch_bounds__(kPcgThreads) void k_pcg_init_done(PcgState S) {
-  __shared__ double l1[1];
-  const double rz = sum_partials(S.part_rz, S.nwg, l1);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    S.rz[0] = rz;
-    if (!(rz > 0.0)) S.flag[0] = 1;
-  }
-}
-
-// q = H p (one thread per row), partial p.q
+// w = H u (one thread per row), partial w.u
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(PcgState S, const float *__restrict__ H,
                                                          const int *__restrict__ row_start,
                                                          const uint32_t *__restrict__ row_ent,
@@ -698,32 +695,54 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(PcgState S, const floa
       const bool tr = ent >> 31;
       const int col = tr ? blk_a[blk] : blk_b[blk];
       const float *B = H + 36 * (size_t)blk;
-      const double *pc = S.p + 6 * (size_t)col;
+      const double *pc = S.z + 6 * (size_t)col;
       double s = 0.0;
 #pragma unroll
       for (int c = 0; c < 6; ++c) s += (double)(tr ? B[6 * c + row] : B[6 * row + c]) * pc[c];
       qv += s;
     }
     S.q[t] = qv;
-    pv = S.p[t];
+    pv = S.z[t];
   }
   wg_partial(pv * qv, lds, S.part_pq);
 }
 
-// alpha = rz / pq; x += alpha p; r -= alpha q; z = Minv r; partial r.z
+// Iteration k: gamma = r.u and delta = w.u from the partials (fixed order,
+// the same in every workgroup); stop when gamma <= tol^2 gamma_0; beta =
+// gamma / gamma_{k-1}, alpha = gamma / (delta - beta gamma / alpha_{k-1})
+// (k = 0: beta = 0, alpha = gamma / delta); p = u + beta p; s = w + beta s;
+// x += alpha p; r -= alpha s; u = Minv r; partial r.u.
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const double *__restrict__ Minv, int nn,
-                                                           int it) {
+                                                           int it, const NrFrame *Fp) {
   if (S.flag[0]) return;
   __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1], l1[1];
-  const double pq = sum_partials(S.part_pq, S.nwg, l1);
-  const double alpha = S.rz[it] / pq;
+  const double gamma = sum_partials(S.part_rz, S.nwg, l1);
+  const double delta = sum_partials(S.part_pq, S.nwg, l1);
+  const double tol = Fp->pcg_tol;
+  const bool stop = it == 0 ? !(gamma > 0.0) : (!(gamma > tol * tol * S.rz[0]) || !(gamma > 0.0));
+  double beta = 0.0, alpha = gamma / delta;
+  if (it > 0) {
+    beta = gamma / S.rz[it - 1];
+    alpha = gamma / (delta - beta * gamma / S.al[it - 1]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.rz[it] = gamma;
+    S.al[it] = alpha;
+    if (stop) S.flag[0] = 1;
+    else S.flag[1] = it + 1;
+  }
+  if (stop) return;
   const int t = blockIdx.x * kPcgThreads + threadIdx.x;
   const int node = t / 6, row = t - node * 6;
   const bool act = node < nn;
   double r = 0.0;
   if (act) {
-    S.x[t] += alpha * S.p[t];
-    r = S.r[t] - alpha * S.q[t];
+    const double pk = it == 0 ? S.z[t] : S.z[t] + beta * S.p[t];
+    const double sk = it == 0 ? S.q[t] : S.q[t] + beta * S.s[t];
+    S.p[t] = pk;
+    S.s[t] = sk;
+    S.x[t] += alpha * pk;
+    r = S.r[t] - alpha * sk;
     S.r[t] = r;
   }
   rl[threadIdx.x] = r;
@@ -735,22 +754,6 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const do
   }
   __syncthreads();
   wg_partial(r * z, lds, S.part_rz);
-}
-
-// beta = rz_new / rz; p = z + beta p; convergence test
-__global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(PcgState S, int nn, int it, const NrFrame *Fp) {
-  if (S.flag[0]) return;
-  const double tol = Fp->pcg_tol;
-  __shared__ double l1[1];
-  const double rzn = sum_partials(S.part_rz, S.nwg, l1);
-  const double beta = rzn / S.rz[it];
-  const int t = blockIdx.x * kPcgThreads + threadIdx.x;
-  if (t < 6 * nn) S.p[t] = S.z[t] + beta * S.p[t];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    S.rz[it + 1] = rzn;
-    S.flag[1] = it + 1;
-    if (!(rzn > tol * tol * S.rz[0]) || !(rzn > 0.0)) S.flag[0] = 1;
-  }
 }
 
 // T_i <- exp(x_i) T_i, then the node's dual quaternion
@@ -996,6 +999,7 @@ static int graph_build(dm_graph *g, const float *h_verts, const float *h_normals
   DM_TRY(g->dalloc(&S.z, nn * 6));
   DM_TRY(g->dalloc(&S.p, nn * 6));
   DM_TRY(g->dalloc(&S.q, nn * 6));
+  DM_TRY(g->dalloc(&S.s, nn * 6));
   DM_TRY(g->dalloc(&S.part_pq, S.nwg));
   DM_TRY(g->dalloc(&S.part_rz, S.nwg));
   DM_TRY(g->dalloc(&S.flag, 2));
@@ -1031,12 +1035,10 @@ static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its) {
     if (it + 1 == gn)
       hipLaunchKernelGGL(k_nr_stats, dim3(1), dim3(1024), 0, st, g->vdata, g->nv, g->edata, ne, F, g->stats);
     hipLaunchKernelGGL(k_pcg_init, dim3(pg), dim3(kPcgThreads), 0, st, S, g->rhs, g->Minv, nn);
-    hipLaunchKernelGGL(k_pcg_init_done, dim3(1), dim3(kPcgThreads), 0, st, S);
     for (int k = 0; k < pcg_its; ++k) {
       hipLaunchKernelGGL(k_pcg_spmv, dim3(pg), dim3(kPcgThreads), 0, st, S, g->H, g->row_start, g->row_ent,
                          g->blk_a, g->blk_b, nn);
-      hipLaunchKernelGGL(k_pcg_update, dim3(pg), dim3(kPcgThreads), 0, st, S, g->Minv, nn, k);
-      hipLaunchKernelGGL(k_pcg_dir, dim3(pg), dim3(kPcgThreads), 0, st, S, nn, k, F);
+      hipLaunchKernelGGL(k_pcg_update, dim3(pg), dim3(kPcgThreads), 0, st, S, g->Minv, nn, k, F);
     }
     hipLaunchKernelGGL(k_nr_update, dim3(grid_of(nn)), dim3(256), 0, st, S.x, g->T, nn, g->dq);
   }
@@ -1155,6 +1157,7 @@ int dm_nonrigid_solve(dm_graph *g, const float *depth, const dm_sensor *sensor, 
   }
   if (pr->pcg_iterations + 1 > g->pcg_cap) {
     DM_TRY(g->dalloc(&g->pcg.rz, pr->pcg_iterations + 1));
+    DM_TRY(g->dalloc(&g->pcg.al, pr->pcg_iterations + 1));
     g->pcg_cap = pr->pcg_iterations + 1;
     g->key_gn = -1;
   }
