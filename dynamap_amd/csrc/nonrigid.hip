@@ -622,15 +622,19 @@ struct PcgState {
   double *rz;                 // rz[k] = r_k . u_k (k = 0 .. max)
   double *al;                 // al[k]: step length of iteration k
   int *flag;                  // [0] converged, [1] iterations run
-  int nwg;
+  int nwg;                    // workgroups of init/update (kPcgThreads rows each)
+  int nwg_s;                  // workgroups of the SpMV (kSpmvNodes nodes each)
 };
 
-// fixed-order sum of the workgroup partials, computed by every workgroup
+// fixed-order sum of the workgroup partials, computed by every workgroup:
+// lane l of the first wavefront adds partials l, l + 64, ...; the xor tree
+// of wave_sum adds the lanes, and lane 0's total is the result
 __device__ double sum_partials(const double *part, int n, double *lds1) {
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {
     double s = 0.0;
-    for (int k = 0; k < n; ++k) s += part[k];
-    lds1[0] = s;
+    for (int k = threadIdx.x; k < n; k += 64) s += part[k];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) lds1[0] = s;
   }
   __syncthreads();
   const double s = lds1[0];
@@ -677,34 +681,55 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgState S, const doub
   if (t == 0) { S.flag[0] = 0; S.flag[1] = 0; }
 }
 
-// w = H u (one thread per row), partial w.u
-__global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(PcgState S, const float *__restrict__ H,
-                                                         const int *__restrict__ row_start,
-                                                         const uint32_t *__restrict__ row_ent,
-                                                         const int *__restrict__ blk_a, const int *__restrict__ blk_b,
-                                                         int nn) {
+// w = H u, one wavefront per node: lane l < 60 takes output row l % 6 of the
+// block entries l / 6, l / 6 + 10, ... of the node's block row, so all the
+// entries' loads are in flight together; the 10 lane sums of a row are
+// added in a fixed order through LDS.  Partials of w.u per workgroup.
+constexpr int kSpmvNodes = 4;  // wavefronts (nodes) per workgroup
+__global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_spmv(PcgState S, const float *__restrict__ H,
+                                                             const int *__restrict__ row_start,
+                                                             const uint32_t *__restrict__ row_ent,
+                                                             const int *__restrict__ blk_a,
+                                                             const int *__restrict__ blk_b, int nn) {
   if (S.flag[0]) return;
-  __shared__ double lds[kPcgThreads / 64 + 1];
-  const int t = blockIdx.x * kPcgThreads + threadIdx.x;
-  const int node = t / 6, row = t - node * 6;
-  double qv = 0.0, pv = 0.0;
-  if (node < nn) {
-    for (int k = row_start[node]; k < row_start[node + 1]; ++k) {
-      const uint32_t ent = row_ent[k];
+  __shared__ double acc_l[64 * kSpmvNodes], lds[kSpmvNodes];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int node = blockIdx.x * kSpmvNodes + wv;
+  double acc = 0.0;
+  if (node < nn && lane < 60) {
+    const int k0 = row_start[node], n6 = 6 * (row_start[node + 1] - k0);
+    const int row = lane % 6;
+    for (int pair = lane; pair < n6; pair += 60) {
+      const uint32_t ent = row_ent[k0 + pair / 6];
       const int blk = (int)(ent & 0x7fffffffu);
       const bool tr = ent >> 31;
       const int col = tr ? blk_a[blk] : blk_b[blk];
       const float *B = H + 36 * (size_t)blk;
       const double *pc = S.z + 6 * (size_t)col;
-      double s = 0.0;
+      double sv = 0.0;
 #pragma unroll
-      for (int c = 0; c < 6; ++c) s += (double)(tr ? B[6 * c + row] : B[6 * row + c]) * pc[c];
-      qv += s;
+      for (int c = 0; c < 6; ++c) sv += (double)(tr ? B[6 * c + row] : B[6 * row + c]) * pc[c];
+      acc += sv;
     }
-    S.q[t] = qv;
-    pv = S.z[t];
   }
-  wg_partial(pv * qv, lds, S.part_pq);
+  acc_l[threadIdx.x] = acc;
+  __syncthreads();
+  double contrib = 0.0;
+  if (node < nn && lane < 6) {
+    double q = 0.0;
+    for (int j = lane; j < 60; j += 6) q += acc_l[64 * wv + j];
+    const size_t t = 6 * (size_t)node + lane;
+    S.q[t] = q;
+    contrib = S.z[t] * q;
+  }
+  const double w = wave_sum(contrib);
+  if (lane == 0) lds[wv] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (int k = 0; k < kSpmvNodes; ++k) sum += lds[k];
+    S.part_pq[blockIdx.x] = sum;
+  }
 }
 
 // Iteration k: gamma = r.u and delta = w.u from the partials (fixed order,
@@ -717,7 +742,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const do
   if (S.flag[0]) return;
   __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1], l1[1];
   const double gamma = sum_partials(S.part_rz, S.nwg, l1);
-  const double delta = sum_partials(S.part_pq, S.nwg, l1);
+  const double delta = sum_partials(S.part_pq, S.nwg_s, l1);
   const double tol = Fp->pcg_tol;
   const bool stop = it == 0 ? !(gamma > 0.0) : (!(gamma > tol * tol * S.rz[0]) || !(gamma > 0.0));
   double beta = 0.0, alpha = gamma / delta;
@@ -994,13 +1019,14 @@ static int graph_build(dm_graph *g, const float *h_verts, const float *h_normals
   DM_TRY(g->dalloc(&g->stats, 4));
   PcgState &S = g->pcg;
   S.nwg = (int)grid_of(6 * nn, kPcgThreads);
+  S.nwg_s = (int)grid_of(nn, kSpmvNodes);
   DM_TRY(g->dalloc(&S.x, nn * 6));
   DM_TRY(g->dalloc(&S.r, nn * 6));
   DM_TRY(g->dalloc(&S.z, nn * 6));
   DM_TRY(g->dalloc(&S.p, nn * 6));
   DM_TRY(g->dalloc(&S.q, nn * 6));
   DM_TRY(g->dalloc(&S.s, nn * 6));
-  DM_TRY(g->dalloc(&S.part_pq, S.nwg));
+  DM_TRY(g->dalloc(&S.part_pq, S.nwg_s));
   DM_TRY(g->dalloc(&S.part_rz, S.nwg));
   DM_TRY(g->dalloc(&S.flag, 2));
   DM_TRY(g->dalloc(&g->d_frame, 1));
@@ -1036,8 +1062,8 @@ static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its) {
       hipLaunchKernelGGL(k_nr_stats, dim3(1), dim3(1024), 0, st, g->vdata, g->nv, g->edata, ne, F, g->stats);
     hipLaunchKernelGGL(k_pcg_init, dim3(pg), dim3(kPcgThreads), 0, st, S, g->rhs, g->Minv, nn);
     for (int k = 0; k < pcg_its; ++k) {
-      hipLaunchKernelGGL(k_pcg_spmv, dim3(pg), dim3(kPcgThreads), 0, st, S, g->H, g->row_start, g->row_ent,
-                         g->blk_a, g->blk_b, nn);
+      hipLaunchKernelGGL(k_pcg_spmv, dim3((unsigned)S.nwg_s), dim3(64 * kSpmvNodes), 0, st, S, g->H, g->row_start,
+                         g->row_ent, g->blk_a, g->blk_b, nn);
       hipLaunchKernelGGL(k_pcg_update, dim3(pg), dim3(kPcgThreads), 0, st, S, g->Minv, nn, k, F);
     }
     hipLaunchKernelGGL(k_nr_update, dim3(grid_of(nn)), dim3(256), 0, st, S.x, g->T, nn, g->dq);
