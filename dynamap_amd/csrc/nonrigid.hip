@@ -793,18 +793,22 @@ __global__ __launch_bounds__(256) void k_nr_update(const double *__restrict__ x,
   dq[i] = dq_of_T(Tn);
 }
 
-// energies at the last linearisation: [0] residual count, [1] data, [2] ARAP
-__global__ __launch_bounds__(1024) void k_nr_stats(const float *__restrict__ vdata, int64_t nv,
-                                                  const double *__restrict__ edata, int ne, const NrFrame *Fp,
-                                                  double *__restrict__ out) {
+// energies at the last linearisation: [0] residual count, [1] data, [2] ARAP.
+// kStatsWg workgroups write fixed-order partials (grid-stride over vertices
+// and edges); k_nr_stats_sum adds them in order.
+constexpr int kStatsWg = 128;
+__global__ __launch_bounds__(256) void k_nr_stats(const float *__restrict__ vdata, int64_t nv,
+                                                 const double *__restrict__ edata, int ne, const NrFrame *Fp,
+                                                 double *__restrict__ part) {
   const double lam = Fp->lambda;
-  __shared__ double lds[3][1024 / 64];
+  __shared__ double lds[3][256 / 64];
   double c = 0.0, ed = 0.0, er = 0.0;
-  for (int64_t v = threadIdx.x; v < nv; v += 1024) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nv; v += stride) {
     const float *d = vdata + 8 * v;
     if (d[7] != 0.f) { c += 1.0; ed += (double)d[6] * (double)d[6]; }
   }
-  for (int e = threadIdx.x; e < ne; e += 1024) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < ne; e += stride) {
     const double *q = edata + 8 * (size_t)e;
     for (int k = 0; k < 3; ++k) {
       const double rc = q[6] * (q[k] - q[3 + k]);
@@ -814,12 +818,18 @@ __global__ __launch_bounds__(1024) void k_nr_stats(const float *__restrict__ vda
   c = wave_sum(c); ed = wave_sum(ed); er = wave_sum(er);
   if ((threadIdx.x & 63) == 0) { lds[0][threadIdx.x >> 6] = c; lds[1][threadIdx.x >> 6] = ed; lds[2][threadIdx.x >> 6] = er; }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double s[3] = {0, 0, 0};
-    for (int k = 0; k < 1024 / 64; ++k)
-      for (int q = 0; q < 3; ++q) s[q] += lds[q][k];
-    out[0] = s[0]; out[1] = s[1]; out[2] = s[2];
+  if (threadIdx.x < 3) {
+    double s = 0.0;
+    for (int k = 0; k < 256 / 64; ++k) s += lds[threadIdx.x][k];
+    part[3 * blockIdx.x + threadIdx.x] = s;
   }
+}
+
+__global__ void k_nr_stats_sum(const double *__restrict__ part, double *__restrict__ out) {
+  if (threadIdx.x >= 3) return;
+  double s = 0.0;
+  for (int b = 0; b < kStatsWg; ++b) s += part[3 * b + threadIdx.x];
+  out[threadIdx.x] = s;
 }
 
 __global__ void k_set_identity(double *T, int nn) {
@@ -851,7 +861,7 @@ struct dm_graph {
   uint32_t *codes = nullptr, *row_ent = nullptr;
   // per-iteration buffers
   float *pv = nullptr, *pn = nullptr, *vdata = nullptr, *H = nullptr;
-  double *edata = nullptr, *rhs = nullptr, *Minv = nullptr, *stats = nullptr;
+  double *edata = nullptr, *rhs = nullptr, *Minv = nullptr, *stats = nullptr, *stats_part = nullptr;
   PcgState pcg{};
   int pcg_cap = 0;
   // frame staging
@@ -1017,6 +1027,7 @@ static int graph_build(dm_graph *g, const float *h_verts, const float *h_normals
   DM_TRY(g->dalloc(&g->rhs, (size_t)nn * 6));
   DM_TRY(g->dalloc(&g->Minv, (size_t)nn * 36));
   DM_TRY(g->dalloc(&g->stats, 4));
+  DM_TRY(g->dalloc(&g->stats_part, 3 * kStatsWg));
   PcgState &S = g->pcg;
   S.nwg = (int)grid_of(6 * nn, kPcgThreads);
   S.nwg_s = (int)grid_of(nn, kSpmvNodes);
@@ -1058,8 +1069,11 @@ static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its) {
     hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 4)), dim3(256), 0, st, g->blk_a, g->blk_b, g->cstart,
                        g->codes, g->nb, g->vdata, g->vnn, g->vnn_wn, g->node_of_vertex, g->nbr, g->edata, F, g->H,
                        g->rhs, g->Minv);
-    if (it + 1 == gn)
-      hipLaunchKernelGGL(k_nr_stats, dim3(1), dim3(1024), 0, st, g->vdata, g->nv, g->edata, ne, F, g->stats);
+    if (it + 1 == gn) {
+      hipLaunchKernelGGL(k_nr_stats, dim3(kStatsWg), dim3(256), 0, st, g->vdata, g->nv, g->edata, ne, F,
+                         g->stats_part);
+      hipLaunchKernelGGL(k_nr_stats_sum, dim3(1), dim3(64), 0, st, g->stats_part, g->stats);
+    }
     hipLaunchKernelGGL(k_pcg_init, dim3(pg), dim3(kPcgThreads), 0, st, S, g->rhs, g->Minv, nn);
     for (int k = 0; k < pcg_its; ++k) {
       hipLaunchKernelGGL(k_pcg_spmv, dim3((unsigned)S.nwg_s), dim3(64 * kSpmvNodes), 0, st, S, g->H, g->row_start,
