@@ -28,8 +28,8 @@ static int pow2_at_least(int x) {
 
 int validate_props(const dm_tsdf_props &p) {
   if (!(p.voxel_size > 0.f) || p.num_buckets <= 0 || p.bucket_size <= 0 || p.bucket_size > 64 ||
-      p.num_blocks <= 0 || p.num_blocks >= (1 << 23) || p.block_size != 8 || p.max_sdf_weight < 1 || p.max_sdf_weight > 255 ||
-      !(p.truncation_distance > 0.f) || !(p.max_sensor_depth > p.min_sensor_depth)) {
+      p.num_blocks <= 0 || p.num_blocks >= (1 << 23) || p.block_size != 8 || p.max_sdf_weight < 1 ||
+      p.max_sdf_weight > 255 || !(p.truncation_distance > 0.f) || !(p.max_sensor_depth > p.min_sensor_depth)) {
     set_error("invalid tsdf properties (block_size must be 8, bucket_size 1..64, num_blocks 1..2^23-1, "
               "max weight 1..255)");
     return DM_E_INVALID;

@@ -47,7 +47,8 @@ typedef struct dm_sensor {
 } dm_sensor;
 
 /* tsdfProperties, include/geometry/tsdf/tsdf.h:19-35 (entriesNum and
- * voxelNumPerAxes are derived; blockSize must be 8) */
+ * voxelNumPerAxes are derived; blockSize must be 8; num_blocks < 2^23, so
+ * that a voxel index fits 32 bits with one all-zero block past the pool) */
 typedef struct dm_tsdf_props {
   float voxel_size;
   int32_t num_buckets, bucket_size, num_blocks, block_size;
