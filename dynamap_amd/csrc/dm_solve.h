@@ -13,9 +13,9 @@ namespace dm {
 
 #define DM_HD __host__ __device__ __forceinline__
 
-// LDL^T without pivoting (A symmetric positive definite, row-major 6x6).
-DM_HD int ldlt6(const double *A, const double *b, double *x) {
-  double L[36], D[6];
+// LDL^T without pivoting (A symmetric positive definite, row-major 6x6):
+// the factorisation, then the solve for one right-hand side.
+DM_HD int ldlt6_factor(const double *A, double *L, double *D) {
   for (int k = 0; k < 36; ++k) L[k] = 0.0;
   for (int j = 0; j < 6; ++j) {
     double d = A[6 * j + j];
@@ -29,6 +29,10 @@ DM_HD int ldlt6(const double *A, const double *b, double *x) {
       L[6 * i + j] = a / d;
     }
   }
+  return 0;
+}
+
+DM_HD void ldlt6_solve(const double *L, const double *D, const double *b, double *x) {
   double y[6];
   for (int i = 0; i < 6; ++i) {
     double a = b[i];
@@ -41,6 +45,12 @@ DM_HD int ldlt6(const double *A, const double *b, double *x) {
     for (int k = i + 1; k < 6; ++k) a -= L[6 * k + i] * x[k];
     x[i] = a;
   }
+}
+
+DM_HD int ldlt6(const double *A, const double *b, double *x) {
+  double L[36], D[6];
+  if (ldlt6_factor(A, L, D) != 0) return -1;
+  ldlt6_solve(L, D, b, x);
   return 0;
 }
 

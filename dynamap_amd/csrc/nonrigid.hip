@@ -515,7 +515,7 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 
 // One wavefront per block: H_ab (and for a == b the rhs, the damping and the
-// FP64 inverse of the block).  Lanes take contributions strided by 64; the
+// FP64 block itself, inverted by k_nr_invdiag).  Lanes take contributions strided by 64; the
 // lane sums are combined by a fixed xor tree.
 __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
                                                     const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
@@ -524,7 +524,7 @@ __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk
                                                     const int *__restrict__ node_of_vertex, const int *__restrict__ nbr,
                                                     const double *__restrict__ edata, const NrFrame *Fp,
                                                     float *__restrict__ H, double *__restrict__ rhs,
-                                                    double *__restrict__ Minv) {
+                                                    double *__restrict__ Hd) {
   const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (blk >= nb) return;  // uniform per wavefront
@@ -572,8 +572,14 @@ __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk
         arap_rows(ed, c3, Ji, Jj);
         const double rc = ed[6] * (ed[c3] - ed[3 + c3]);
         // the block is (min, max): transpose the (i, j) product when i > j
-        const double *U = role == 0 ? Ji : (role == 1 ? Jj : (i <= j ? Ji : Jj));
-        const double *W = role == 0 ? Ji : (role == 1 ? Jj : (i <= j ? Jj : Ji));
+        // (values selected, not pointers, so the rows stay in registers)
+        const bool ui = role == 0 || (role == 2 && i <= j), wi = role == 0 || (role == 2 && i > j);
+        double U[6], W[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          U[r] = ui ? Ji[r] : Jj[r];
+          W[r] = wi ? Ji[r] : Jj[r];
+        }
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
@@ -599,12 +605,26 @@ __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk
   if (!diag) return;
 #pragma unroll
   for (int r = 0; r < 6; ++r) rhs[6 * a + r] = g[r];
+  double *D = Hd + 36 * (size_t)a;
+#pragma unroll
+  for (int q = 0; q < 36; ++q) D[q] = h[q];
+}
+
+// Block-Jacobi preconditioner: the FP64 inverse of each diagonal block (one
+// thread per node; LDL^T once, then the 6 unit columns; a block that is not
+// positive definite gets 0).
+__global__ __launch_bounds__(256) void k_nr_invdiag(const double *__restrict__ Hd, int nn,
+                                                   double *__restrict__ Minv) {
+  const int a = blockIdx.x * 256 + threadIdx.x;
+  if (a >= nn) return;
+  double h[36], L[36], D[6];
+  for (int q = 0; q < 36; ++q) h[q] = Hd[36 * (size_t)a + q];
+  const bool ok = ldlt6_factor(h, L, D) == 0;
   double *M = Minv + 36 * (size_t)a;
   for (int c = 0; c < 6; ++c) {
-    double e6[6] = {0, 0, 0, 0, 0, 0}, col[6];
+    double e6[6] = {0, 0, 0, 0, 0, 0}, col[6] = {0, 0, 0, 0, 0, 0};
     e6[c] = 1.0;
-    if (ldlt6(h, e6, col) != 0)
-      for (int r = 0; r < 6; ++r) col[r] = 0.0;
+    if (ok) ldlt6_solve(L, D, e6, col);
     for (int r = 0; r < 6; ++r) M[6 * r + c] = col[r];
   }
 }
@@ -861,7 +881,7 @@ struct dm_graph {
   uint32_t *codes = nullptr, *row_ent = nullptr;
   // per-iteration buffers
   float *pv = nullptr, *pn = nullptr, *vdata = nullptr, *H = nullptr;
-  double *edata = nullptr, *rhs = nullptr, *Minv = nullptr, *stats = nullptr, *stats_part = nullptr;
+  double *edata = nullptr, *rhs = nullptr, *Minv = nullptr, *Hd = nullptr, *stats = nullptr, *stats_part = nullptr;
   PcgState pcg{};
   int pcg_cap = 0;
   // frame staging
@@ -1026,6 +1046,7 @@ static int graph_build(dm_graph *g, const float *h_verts, const float *h_normals
   DM_TRY(g->dalloc(&g->H, (size_t)nb * 36));
   DM_TRY(g->dalloc(&g->rhs, (size_t)nn * 6));
   DM_TRY(g->dalloc(&g->Minv, (size_t)nn * 36));
+  DM_TRY(g->dalloc(&g->Hd, (size_t)nn * 36));
   DM_TRY(g->dalloc(&g->stats, 4));
   DM_TRY(g->dalloc(&g->stats_part, 3 * kStatsWg));
   PcgState &S = g->pcg;
@@ -1068,7 +1089,8 @@ static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its) {
                        g->nbr_w, nn, g->edata);
     hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 4)), dim3(256), 0, st, g->blk_a, g->blk_b, g->cstart,
                        g->codes, g->nb, g->vdata, g->vnn, g->vnn_wn, g->node_of_vertex, g->nbr, g->edata, F, g->H,
-                       g->rhs, g->Minv);
+                       g->rhs, g->Hd);
+    hipLaunchKernelGGL(k_nr_invdiag, dim3(grid_of(nn)), dim3(256), 0, st, g->Hd, nn, g->Minv);
     if (it + 1 == gn) {
       hipLaunchKernelGGL(k_nr_stats, dim3(kStatsWg), dim3(256), 0, st, g->vdata, g->nv, g->edata, ne, F,
                          g->stats_part);
