@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--nonrigid-frames", type=int, default=20)
     ap.add_argument("--no-mesh", action="store_true")
+    ap.add_argument("--no-stress", action="store_true", help="skip the 65-session stress mix")
+    ap.add_argument("--stress-frames", type=int, default=24)
     return ap.parse_args()
 
 
@@ -161,6 +163,59 @@ def bench_mesh(dm, with_cpu, reps=5):
                                    cores=int(L32.or_get_threads()), sample="FP32 oracle marching cubes, same volume")
         log(f"mesh: oracle {1000 * elc:.1f} ms, bit-exact {res['bit_exact_vs_oracle']}")
     vol.close()
+    return res
+
+
+def bench_stress(dm, nframes, poisson_hz, warm=3):
+    """Large stress mix on one GPU: 32 x (320x240, 2 cm) + 32 x (640x480,
+    8 mm) room sessions plus one 640x480 / 4 mm session of the 8 x 8 x 3 m
+    multi-room scene (2^22 buckets, 2^21 blocks), all in one serving loop.
+    Reports the closed-loop aggregate rate, the per-frame device path on the
+    same frames, Poisson-arrival latencies and trajectory agreement."""
+    from dynamap_amd import configs
+    N = dm._native
+    names = (["rigid-320x240-2cm"] * 32 + ["rigid-640x480-8mm"] * 32 + ["large-640x480-4mm"])
+    t0 = time.time()
+    srv = dm.Server(len(names), 480, 640, hbm_budget_bytes=250 << 30, use_graph=True)
+    gts = []
+    for s, name in enumerate(names):
+        depth, rgb, gt = configs.make_sequence(name, nframes, session=s)
+        w = configs.WORKLOADS[name]
+        srv.add_trace(w["tsdf"], w["icp"], w["sensor"], gt[0], depth, rgb)
+        gts.append(gt)
+        del depth, rgb
+    gts = np.stack(gts)
+    hbm = sum(srv.trace_bytes(t) for t in range(len(names)))
+    log(f"stress: {len(names)} sessions x {nframes} frames set up in {time.time() - t0:.1f}s, "
+        f"{hbm / 2**30:.1f} GiB of session state (frames excluded)")
+    ns, K = len(names), nframes - warm
+    srv.run(N.DM_RUN_BATCHED, 0, warm)
+    r = srv.run(N.DM_RUN_BATCHED, warm, K, want_poses=True)
+    srv.run(N.DM_RUN_PER_FRAME_DEV, 0, warm)
+    rd = srv.run(N.DM_RUN_PER_FRAME_DEV, warm, K, want_poses=True)
+    rng = np.random.default_rng(configs.SEED_BASE + 5000)
+    gaps = rng.exponential(1.0 / poisson_hz, size=(ns, nframes))
+    arrivals = np.cumsum(gaps, axis=1) - gaps[:, :1]
+    rp = srv.run(N.DM_RUN_BATCHED, 0, nframes, arrivals=arrivals)
+    L = (rp["completions"] - arrivals) * 1000.0
+    pb, pd = r["poses"], rd["poses"]
+    res = dict(workload="stress-65: 32 x 320x240 @ 2 cm + 32 x 640x480 @ 8 mm + 1 x 640x480 @ 4 mm multi-room "
+                        "(2^22 buckets)", sessions=ns, frames_per_session=nframes, timed_frames=K,
+               session_state_gib=hbm / 2**30,
+               batched_fps=ns * K / r["wall"], ms_per_batch=1000 * r["wall"] / K,
+               per_frame_device_fps=ns * K / rd["wall"],
+               poisson=dict(hz_per_session=poisson_hz, p50=float(np.percentile(L, 50)),
+                            p99=float(np.percentile(L, 99)), max=float(L.max()),
+                            large_session_p99=float(np.percentile(L[-1], 99)), batches=rp["batches"]),
+               max_abs_pose_diff_batched_vs_per_frame_device=float(np.abs(pb - pd).max()),
+               ate_mm_vs_ground_truth={"2cm": 1000 * ate(pb[:32], gts[:32, warm:]),
+                                       "8mm": 1000 * ate(pb[32:64], gts[32:64, warm:]),
+                                       "4mm_large": 1000 * ate(pb[64], gts[64, warm:])})
+    res["ratio_batched_vs_per_frame_device"] = res["batched_fps"] / res["per_frame_device_fps"]
+    log(f"stress: batched {res['batched_fps']:.1f} frames/s ({res['ms_per_batch']:.2f} ms/batch), per-frame device "
+        f"{res['per_frame_device_fps']:.1f} frames/s, poisson p50 {res['poisson']['p50']:.2f} p99 "
+        f"{res['poisson']['p99']:.2f} ms")
+    srv.close()
     return res
 
 
@@ -312,6 +367,10 @@ def main():
     # ---- marching cubes of one room volume
     if not args.no_mesh:
         out["mesh"] = bench_mesh(dm, not args.no_cpu)
+    # ---- 65-session stress mix, after the serve-16 sessions are released
+    srv.close()
+    if not args.no_stress:
+        out["stress"] = bench_stress(dm, args.stress_frames, args.poisson_hz)
 
     res = {
         "metric": "aggregate frames/s + p50/p99 frame latency, batched vs per-frame, ATE≤1mm",

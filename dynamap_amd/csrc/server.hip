@@ -32,6 +32,7 @@ struct Trace {
   size_t bytes = 0;
   Sess *sess = nullptr;
   int slot = -1;
+  int next_frame = -1;  // trace frame the live session continues at; -1 when it cannot continue
 };
 
 int sess_reset(Sess &S, const double init_pose[16]);
@@ -144,6 +145,7 @@ static void retire(dm_server *srv, int ti) {
   srv->pool.push_back(t.sess);
   t.sess = nullptr;
   t.slot = -1;
+  t.next_frame = -1;
 }
 
 static Launch server_launch(const dm_server *srv) {
@@ -307,9 +309,20 @@ int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int32_t nfr
       set_error("trace has fewer frames than requested");
       return DM_E_INVALID;
     }
-  // fresh sessions when starting from frame 0
+  // fresh sessions when starting from frame 0; otherwise every session must
+  // be live and have stopped exactly at first_frame
   if (first_frame == 0)
     for (int t = 0; t < NT; ++t) retire(srv, t);
+  else
+    for (int t = 0; t < NT; ++t)
+      if (!srv->traces[t].sess || srv->traces[t].next_frame != first_frame) {
+        set_error("a run from frame " + std::to_string(first_frame) + " must continue where trace " +
+                  std::to_string(t) + " stopped (" +
+                  (srv->traces[t].sess ? "at frame " + std::to_string(srv->traces[t].next_frame)
+                                       : std::string("no live session")) + ")");
+        return DM_E_STATE;
+      }
+  for (int t = 0; t < NT; ++t) srv->traces[t].next_frame = -1;  // set again when the run completes
   std::vector<int> order(NT);
   for (int t = 0; t < NT; ++t) order[t] = t;
   int r = admit(srv, order);
@@ -505,6 +518,8 @@ int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int32_t nfr
       if (next[t] >= nframes && first_frame + nframes >= tr.nframes) retire(srv, t);
     }
   }
+  for (int t = 0; t < NT; ++t)
+    if (srv->traces[t].sess) srv->traces[t].next_frame = first_frame + next[t];
   if (wall_seconds) *wall_seconds = secs_since(t0);
   if (batches) *batches = nbatch;
   return DM_OK;
@@ -525,6 +540,7 @@ static int stage_setup(dm_server *srv, int32_t frame, std::vector<int> &act, int
   for (int t = 0; t < (int)srv->traces.size(); ++t)
     if (srv->traces[t].sess && frame < srv->traces[t].nframes) act.push_back(t);
   if (act.empty()) { set_error("no admitted session at this frame"); return DM_E_STATE; }
+  for (int t : act) srv->traces[t].next_frame = -1;  // the timed launches leave no continuable state
   DM_HIP(hipStreamSynchronize(srv->stream));
   DmView *tab = srv->h_table[0];
   int *lst = srv->h_list[0];

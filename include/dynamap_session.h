@@ -91,8 +91,11 @@ enum {
 };
 
 /* Serve frames [first_frame, first_frame + nframes) of every trace, starting
- * each trace from a fresh session when first_frame == 0 (a session persists
- * across calls otherwise).  arrivals: per trace, per frame arrival time in
+ * each trace from a fresh session when first_frame == 0.  A run from
+ * first_frame > 0 continues the live sessions and needs every trace to have
+ * stopped at first_frame in the previous run (DM_E_STATE otherwise, e.g.
+ * after a trace's last frame retired its session, or after
+ * dm_server_profile / dm_server_autotune advanced the volumes).  arrivals: per trace, per frame arrival time in
  * seconds relative to the call (trace-major, nframes per trace), or NULL
  * when every frame is available at once (closed loop).  Outputs (may be
  * NULL): completion time of every frame (seconds, same layout), the pose of

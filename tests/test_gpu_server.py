@@ -89,3 +89,17 @@ def test_autotune_keeps_results(traces):
     p = srv.run(N.DM_RUN_BATCHED, 0, NF, want_poses=True)["poses"]
     srv.close()
     assert np.array_equal(p, ref)
+
+
+def test_continued_runs_and_their_checks(traces):
+    srv = server(traces)
+    ref = srv.run(N.DM_RUN_BATCHED, 0, NF, want_poses=True)["poses"]
+    # the full run retired every session: a continuation has nothing to continue
+    with pytest.raises(RuntimeError, match="no live session"):
+        srv.run(N.DM_RUN_BATCHED, 2, NF - 2)
+    a = srv.run(N.DM_RUN_BATCHED, 0, 2, want_poses=True)["poses"]
+    with pytest.raises(RuntimeError, match="stopped"):
+        srv.run(N.DM_RUN_PER_FRAME_DEV, 3, NF - 3)
+    b = srv.run(N.DM_RUN_PER_FRAME_DEV, 2, NF - 2, want_poses=True)["poses"]
+    srv.close()
+    assert np.array_equal(np.concatenate([a, b], 1), ref)
