@@ -277,6 +277,20 @@ __global__ void k_alloc_close(const DmView *__restrict__ views, const int *__res
 // One workgroup per voxel block (grid-stride over the allocated blocks), two
 // voxels per thread, coalesced 8-byte voxel words; unchanged voxels are not
 // written back.
+// One colour channel of Voxel::mergeVoxel (geom_Types.h:85-90):
+// uchar((c*w + c_new) / (w + 1)) with the float quotient truncated.  Here n =
+// c*w + c_new and m = w + 1 are integers (c, c_new, w <= 255, so n < 2^16 and
+// n/m <= 255), and the correctly rounded float quotient truncates to
+// floor(n/m): a non-integer n/m lies at least 1/m >= 2^-8 from the integers,
+// far beyond the quotient's rounding error (2^-24 * 255).  floor(n/m) is
+// taken as trunc((n + 1/2) * rm) with rm = rcp(m) (v_rcp_f32, 1 ulp):
+// (n + 1/2)/m lies at least 1/(2m) >= 2^-9 from every integer, while the
+// reciprocal's and the product's rounding move it by at most
+// 255.5 * (2^-23 + 2^-24) < 2^-14.
+__device__ __forceinline__ uint32_t merge_channel(uint32_t c, uint32_t w, uint32_t c_new, float rm) {
+  return (uint32_t)(((float)(c * w + c_new) + 0.5f) * rm);
+}
+
 // One voxel of IntegrateScanKernel (tsdf.cu:301-341): returns the new voxel
 // word (the old one when the voxel is not updated).
 __device__ __forceinline__ uint64_t integrate_voxel(const DmView &v, const Intr &s, const M4 &Ti, f3 pos, int t,
@@ -292,7 +306,7 @@ __device__ __forceinline__ uint64_t integrate_voxel(const DmView &v, const Intr 
   const float d = v.depth[idx];
   if (d < v.min_depth) return old;
   if (d > v.max_depth) return old;
-  const f3 p3 = get_point3d(idx, d, s);
+  const f3 p3 = get_point3d_uv(pu, pv, d, s);
   const float surf = norm(mk(0.f, 0.f, 0.f) - p3);
   const float vox = norm(mk(0.f, 0.f, 0.f) - vc);
   float nsdf;
@@ -304,15 +318,16 @@ __device__ __forceinline__ uint64_t integrate_voxel(const DmView &v, const Intr 
     nr = v.rgb[3 * idx]; ng = v.rgb[3 * idx + 1]; nb = v.rgb[3 * idx + 2];
   }
   const uint32_t rgbw = vox_rgbw(old);
-  const float w = (float)(rgbw >> 24);
+  const uint32_t wi = rgbw >> 24;
+  const float w = (float)wi;
   const float sdf = (vox_sdf(old) * w + nsdf * 1.0f) / (w + 1.0f);
-  const float cr = ((float)(rgbw & 0xff) * w + (float)nr * 1.0f) / (w + 1.0f);
-  const float cg = ((float)((rgbw >> 8) & 0xff) * w + (float)ng * 1.0f) / (w + 1.0f);
-  const float cb = ((float)((rgbw >> 16) & 0xff) * w + (float)nb * 1.0f) / (w + 1.0f);
-  int nw = (int)(rgbw >> 24) + 1;
+  const float rm = __builtin_amdgcn_rcpf((float)(wi + 1u));
+  const uint32_t cr = merge_channel(rgbw & 0xff, wi, nr, rm);
+  const uint32_t cg = merge_channel((rgbw >> 8) & 0xff, wi, ng, rm);
+  const uint32_t cb = merge_channel((rgbw >> 16) & 0xff, wi, nb, rm);
+  int nw = (int)wi + 1;
   if (nw > v.max_weight) nw = v.max_weight;
-  const uint32_t nrgbw = ((uint32_t)cr & 0xff) | (((uint32_t)cg & 0xff) << 8) | (((uint32_t)cb & 0xff) << 16) |
-                         ((uint32_t)nw << 24);
+  const uint32_t nrgbw = cr | (cg << 8) | (cb << 16) | ((uint32_t)nw << 24);
   return ((uint64_t)nrgbw << 32) | (uint64_t)__float_as_uint(sdf);
 }
 

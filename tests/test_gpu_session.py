@@ -28,3 +28,32 @@ def test_trajectory_320_vs_fp64(host):
     print("ATE gpu-vs-fp64 (m)", ate(G, R), "gpu-vs-gt", ate(G, gt), "fp64-vs-gt", ate(R, gt))
     assert ate(G, R) < 1e-3
     assert ate(G, gt) < 5e-3
+
+
+def test_dropout_and_nan_frames_vs_fp64():
+    """A frame with no valid depth (sensor dropout) keeps the pose and adds
+    no blocks; NaN pixels fail every depth-range test of the reference
+    (tsdf.cu:291-298, :371) and are skipped.  The GPU and the FP64 oracle
+    agree on the whole sequence."""
+    name = "rigid-320x240-2cm"
+    W = configs.WORKLOADS[name]
+    n = 7
+    depth, rgb, gt = configs.make_sequence(name, n, session=5)
+    depth = depth.copy()
+    depth[2] = 0.0
+    rng = np.random.default_rng(11)
+    depth[4][rng.random(depth[4].shape) < 0.05] = np.nan
+    s = dm.Session(W["tsdf"], W["icp"], W["sensor"], gt[0])
+    ora = O.OraclePipeline(W["tsdf"], W["icp"], W["sensor"], gt[0], bits=64)
+    G, nb = [], []
+    for f in range(n):
+        G.append(s.process_frame(depth[f], None))
+        nb.append(s.volume.GetNumAllocatedBlocks())
+    G = np.stack(G)
+    R = np.stack([ora.process_frame(depth[f], None) for f in range(n)])
+    print("blocks per frame", nb, "ATE gpu-vs-fp64 (m)", ate(G, R), flush=True)
+    assert np.array_equal(G[2], G[1]) and nb[2] == nb[1]
+    assert np.all(np.isfinite(G))
+    assert ate(G, R) < 1e-3
+    assert ate(G, gt) < 5e-3
+    s.close()
