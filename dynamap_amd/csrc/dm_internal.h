@@ -85,6 +85,10 @@ struct DmView {
   const float *depth;
   const float *depth_track;  // depth the tracking maps are built from (filtered), or null = depth
   const uint8_t *rgb;   // interleaved r,g,b (may be null)
+  // per pixel of the current frame, written by the allocation pass for the
+  // integration: |getPoint3d(u, v, d)| (the ray distance to the measured
+  // surface), sign bit set when d == max_depth, NaN when d is out of range
+  float *surf;
   // maps: vertex+intensity and normal per level, source (live frame) and
   // target (model raycast); level l is (rows >> l) x (cols >> l)
   float4 *svert[kLevels], *snorm[kLevels];

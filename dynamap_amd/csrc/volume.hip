@@ -88,12 +88,19 @@ __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r, unsigned
   if (u >= v.cols || r >= v.rows) return 0;
   const int i = r * v.cols + u;
   const float d = v.depth[i];
-  if (d < v.min_depth || d > v.max_depth) return 0;
+  if (d < v.min_depth || d > v.max_depth) {
+    v.surf[i] = __builtin_nanf("");
+    return 0;
+  }
   const Intr s = intr_of(v);
+  const f3 p3 = get_point3d_uv(u, r, d, s);
+  // the integration's surface distance of this pixel (tsdf.cu:319-322)
+  const float surf = norm(mk(0.f, 0.f, 0.f) - p3);
+  v.surf[i] = d < v.max_depth ? surf : -surf;
   const M4 T = load_m4(v.posef + PF_T);
   const float block_m = 8.0f * v.voxel_size;
   const f3 start = mk(T.m[3], T.m[7], T.m[11]);
-  const f3 point = xform(T, get_point3d_uv(u, r, d, s));
+  const f3 point = xform(T, p3);
   if (point.x == 0 && point.y == 0 && point.z == 0) return 0;
   const f3 dir = normalize(point - start);
   const float sd = norm(start - point);
@@ -303,14 +310,16 @@ __device__ __forceinline__ uint64_t integrate_voxel(const DmView &v, const Intr 
   project(vc, s, pu, pv);
   if (!(pu >= 0 && pv >= 0 && pu < s.cols && pv < s.rows)) return old;
   const int idx = pv * s.cols + pu;
-  const float d = v.depth[idx];
-  if (d < v.min_depth) return old;
-  if (d > v.max_depth) return old;
-  const f3 p3 = get_point3d_uv(pu, pv, d, s);
-  const float surf = norm(mk(0.f, 0.f, 0.f) - p3);
+  // |getPoint3d(pu, pv, d)| from the allocation pass, sign bit set when
+  // d == max_depth (no band update); NaN when d is out of [min_depth,
+  // max_depth] or NaN, where the reference's comparisons all fail and the
+  // voxel keeps its value
+  const float sv = ldg(v.surf + idx);
+  if (sv != sv) return old;
+  const float surf = fabsf(sv);
   const float vox = norm(mk(0.f, 0.f, 0.f) - vc);
   float nsdf;
-  if (vox > surf - tau && vox < surf + tau && d < v.max_depth) nsdf = surf - vox;
+  if (vox > surf - tau && vox < surf + tau && !signbit(sv)) nsdf = surf - vox;
   else if (vox < surf - tau) nsdf = tau;
   else return old;
   uint32_t nr = 0, ng = 0, nb = 0;
