@@ -319,6 +319,17 @@ def main():
                     frac=achieved / HBM_PEAK_GBS, traffic=None, algorithmic_bytes=int(bytes_),
                     launch_ms=stage_kernels[dom], vb=vb, Nvis=nvis, pixels=P,
                     stage_ms={k: round(v, 4) for k, v in stage_ms.items()})
+    # every stage against the same roofline (algorithmic bytes of one batched
+    # launch over all sessions / its HIP-event duration)
+    alg = {"alloc_all": 4 * P + 16 * work["unique_blocks"] + 16 * work["new_blocks"],
+           "integrate": nvis * 512 * 2 * vb + P * (4 + 3),
+           "raycast": nvis * 512 * 4 + P * (4 + 12 + 12 + 3),
+           "icp_l0": 64 * P}
+    roofline["stages"] = {k: dict(ms=round(stage_ms[k], 4), algorithmic_bytes=int(b),
+                                  achieved_gbs=round(b / (stage_ms[k] * 1e-3) / 1e9, 1),
+                                  frac=round(b / (stage_ms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+                          for k, b in alg.items()}
+    roofline["Nuniq"], roofline["Nalloc"] = work["unique_blocks"], work["new_blocks"]
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_serve16.json")
     if os.path.exists(tfile):
         with open(tfile) as f:

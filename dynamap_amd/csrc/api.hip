@@ -53,6 +53,7 @@ size_t session_bytes(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   size_t maps = 0;
   for (int l = 0; l < kLevels; ++l) maps += (size_t)(rows >> l) * (cols >> l) * 16 * 4;
   b += maps + P * (4 + 3) * 2 + P * 4 + ((P + 255) / 256 + 1) * kIcpVals * 4 + 4096;
+  b += ((size_t)p.num_blocks + 31) / 32 * 4;
   return b;
 }
 
@@ -115,6 +116,7 @@ int Sess::init(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   DM_TRY(dalloc(&d_depth, P));
   DM_TRY(dalloc(&d_rgb, P * 3));
   DM_TRY(dalloc(&h.surf, P));
+  DM_TRY(dalloc(&h.visited, ((size_t)p.num_blocks + 31) / 32));
   for (int l = 0; l < kLevels; ++l) {
     const size_t Pl = (size_t)(rows >> l) * (cols >> l);
     DM_TRY(dalloc(&h.svert[l], Pl));

@@ -89,6 +89,7 @@ struct DmView {
   // integration: |getPoint3d(u, v, d)| (the ray distance to the measured
   // surface), sign bit set when d == max_depth, NaN when d is out of range
   float *surf;
+  uint32_t *visited;     // num_blocks bits: blocks found by a counting traversal (profiling)
   // maps: vertex+intensity and normal per level, source (live frame) and
   // target (model raycast); level l is (rows >> l) x (cols >> l)
   float4 *svert[kLevels], *snorm[kLevels];
@@ -121,7 +122,9 @@ struct Launch {
 };
 
 void launch_alloc(const Launch &L);
-void launch_alloc_rays(const Launch &L);  // traversal kernel only (profiling)
+// traversal kernel only (profiling); count_unique also marks the pool index
+// of every block found in v.visited
+void launch_alloc_rays(const Launch &L, bool count_unique = false);
 void launch_integrate(const Launch &L);
 void launch_raycast(const Launch &L);
 void rebuild_far_maps(const Launch &L);  // clear and rebuild nearmap/dilmap/coarse (not for graph capture)

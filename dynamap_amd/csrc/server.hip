@@ -532,7 +532,8 @@ int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int32_t nfr
 // 2 ICP solve, 3 allocation (all six kernels), 4 integration, 5 raycast,
 // 6 ray traversal kernel alone.  work (summed over sessions, per launch):
 // [0] pixels, [1] visible blocks, [2] block visits of the traversal, [3] new
-// blocks, [4] sessions, [5] allocated blocks.
+// blocks of the frame, [4] sessions, [5] allocated blocks, [6] distinct blocks
+// visited by the traversal.
 // Put every admitted session that has frame `frame` into the batch table at
 // that frame and run one full frame, so stages can be timed on a real state.
 static int stage_setup(dm_server *srv, int32_t frame, std::vector<int> &act, int64_t &pixels) {
@@ -573,6 +574,12 @@ int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage
   int64_t pixels = 0;
   DM_TRY(stage_setup(srv, frame, act, pixels));
   Launch L = server_launch(srv);
+  int64_t new_blocks = 0;  // blocks inserted by the frame's allocation (stage_setup)
+  for (int t : act) {
+    int32_t c[C_COUNT];
+    DM_HIP(hipMemcpy(c, srv->traces[t].sess->h.ctr, sizeof c, hipMemcpyDeviceToHost));
+    new_blocks += c[C_NNEW];
+  }
   hipEvent_t e0, e1;
   DM_HIP(hipEventCreate(&e0));
   DM_HIP(hipEventCreate(&e1));
@@ -614,7 +621,26 @@ int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage
       if (st == 6) { work[2] = sum_visit / reps; }
     }
   }
-  if (work) { work[0] = pixels; work[3] = 0; work[4] = (int64_t)act.size(); }
+  if (work) {
+    work[0] = pixels; work[3] = new_blocks; work[4] = (int64_t)act.size();
+    // distinct blocks of the traversal: one counting pass marks the pool
+    // index of every block its rays find (the frame's blocks are all
+    // allocated by now; keys dropped for lack of room are not counted)
+    for (int t : act) {
+      Sess &S = *srv->traces[t].sess;
+      DM_HIP(hipMemsetAsync(S.h.visited, 0, ((size_t)S.h.num_blocks + 31) / 32 * 4, srv->stream));
+    }
+    launch_alloc_rays(L, true);
+    DM_HIP(hipStreamSynchronize(srv->stream));
+    int64_t uniq = 0;
+    for (int t : act) {
+      Sess &S = *srv->traces[t].sess;
+      std::vector<uint32_t> bits(((size_t)S.h.num_blocks + 31) / 32);
+      DM_HIP(hipMemcpy(bits.data(), S.h.visited, bits.size() * 4, hipMemcpyDeviceToHost));
+      for (uint32_t w : bits) uniq += __builtin_popcount(w);
+    }
+    work[6] = uniq;
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   DM_HIP(hipGetLastError());

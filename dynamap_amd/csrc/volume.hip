@@ -61,6 +61,10 @@ __device__ __forceinline__ void note_new_key(const DmView &v, uint64_t key) {
 constexpr int kSeenSlots = 1024;
 constexpr int kSeenProbes = 8;
 
+// kCount (profiling only): also mark the pool index of every block found in
+// the table in v.visited, so that the distinct blocks of a traversal can be
+// counted.
+template <bool kCount>
 __device__ __forceinline__ void visit_block(const DmView &v, int x, int y, int z, unsigned long long *seen) {
   if (!in_key_range(x, y, z)) { atomicAdd(&v.ctr[C_DROP_RANGE], 1); return; }
   const uint64_t key = pack_key(x, y, z);
@@ -75,7 +79,11 @@ __device__ __forceinline__ void visit_block(const DmView &v, int x, int y, int z
     }
     h = (h + 1u) & (kSeenSlots - 1);
   }
-  if (find_block(v, x, y, z) >= 0) return;
+  const int ptr = find_block(v, x, y, z);
+  if (ptr >= 0) {
+    if (kCount) atomicOr(&v.visited[ptr >> 5], 1u << (ptr & 31));
+    return;
+  }
   note_new_key(v, key);
 }
 
@@ -84,6 +92,7 @@ __device__ __forceinline__ void visit_block(const DmView &v, int x, int y, int z
 // same workgroup.  K: ray from the camera centre to surface + trunc, the
 // (b + step) * bs boundary for both directions and the extra "neg_ray" block.
 // D: the traversal is bounded by the L1 distance to the end block.
+template <bool kCount>
 __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r, unsigned long long *seen) {
   if (u >= v.cols || r >= v.rows) return 0;
   const int i = r * v.cols + u;
@@ -124,11 +133,11 @@ __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r, unsigned
   if (bx != ex && dir.x < 0) { dx--; neg = true; }
   if (by != ey && dir.y < 0) { dy--; neg = true; }
   if (bz != ez && dir.z < 0) { dz--; neg = true; }
-  visit_block(v, bx, by, bz, seen);
+  visit_block<kCount>(v, bx, by, bz, seen);
   int nvisit = 1;
   if (neg) {
     bx += dx; by += dy; bz += dz;
-    visit_block(v, bx, by, bz, seen);
+    visit_block<kCount>(v, bx, by, bz, seen);
     ++nvisit;
   }
   const long long bound = llabs((long long)ex - bx) + llabs((long long)ey - by) + llabs((long long)ez - bz);
@@ -138,12 +147,13 @@ __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r, unsigned
     } else {
       if (mty < mtz) { by += sty; mty += dty; } else { bz += stz; mtz += dtz; }
     }
-    visit_block(v, bx, by, bz, seen);
+    visit_block<kCount>(v, bx, by, bz, seen);
     ++nvisit;
   }
   return nvisit;
 }
 
+template <bool kCount>
 __global__ __launch_bounds__(256) void k_alloc_rays(const DmView *__restrict__ views,
                                                      const int *__restrict__ active) {
   const int slot_ = active[blockIdx.y];
@@ -159,7 +169,7 @@ __global__ __launch_bounds__(256) void k_alloc_rays(const DmView *__restrict__ v
   __syncthreads();
   const int u = (tile % tiles_x) * 16 + (threadIdx.x & 15);
   const int r = (tile / tiles_x) * 16 + (threadIdx.x >> 4);
-  const int n = alloc_ray(v, u, r, seen);
+  const int n = alloc_ray<kCount>(v, u, r, seen);
   if (n) atomicAdd(&visits, n);
   __syncthreads();
   if (threadIdx.x == 0 && visits) atomicAdd(&v.ctr[C_NVISIT], visits);
@@ -859,8 +869,8 @@ static constexpr unsigned kListWG = 64;  // workgroups per session for the new-k
 
 void launch_alloc(const Launch &L) {
   if (L.nact <= 0) return;
-  hipLaunchKernelGGL(k_alloc_rays, dim3((unsigned)L.max_tiles, (unsigned)L.nact), dim3(256), 0, L.stream, L.views,
-                     L.active);
+  hipLaunchKernelGGL(k_alloc_rays<false>, dim3((unsigned)L.max_tiles, (unsigned)L.nact), dim3(256), 0, L.stream,
+                     L.views, L.active);
   hipLaunchKernelGGL(k_alloc_link, dim3(kListWG, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
   hipLaunchKernelGGL(k_alloc_rank, dim3(kListWG, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
   hipLaunchKernelGGL(k_alloc_commit, dim3(kListWG, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
@@ -868,10 +878,14 @@ void launch_alloc(const Launch &L) {
   hipLaunchKernelGGL(k_alloc_close, dim3((unsigned)L.nact), dim3(64), 0, L.stream, L.views, L.active);
 }
 
-void launch_alloc_rays(const Launch &L) {
+void launch_alloc_rays(const Launch &L, bool count_unique) {
   if (L.nact <= 0) return;
-  hipLaunchKernelGGL(k_alloc_rays, dim3((unsigned)L.max_tiles, (unsigned)L.nact), dim3(256), 0, L.stream, L.views,
-                     L.active);
+  if (count_unique)
+    hipLaunchKernelGGL(k_alloc_rays<true>, dim3((unsigned)L.max_tiles, (unsigned)L.nact), dim3(256), 0, L.stream,
+                       L.views, L.active);
+  else
+    hipLaunchKernelGGL(k_alloc_rays<false>, dim3((unsigned)L.max_tiles, (unsigned)L.nact), dim3(256), 0, L.stream,
+                       L.views, L.active);
 }
 
 void launch_integrate(const Launch &L) {

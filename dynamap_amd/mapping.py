@@ -513,8 +513,8 @@ class Server:
 
     def profile(self, frame, reps=5):
         ms = np.zeros(7)
-        work = np.zeros(6, np.int64)
+        work = np.zeros(7, np.int64)
         N.check(self._lib.dm_server_profile(self._h, frame, reps, N.ptr(ms), N.ptr(work)))
         return dict(zip(self.STAGES, ms.tolist())), dict(
-            pixels=int(work[0]), visible_blocks=int(work[1]), block_visits=int(work[2]), sessions=int(work[4]),
-            allocated_blocks=int(work[5]))
+            pixels=int(work[0]), visible_blocks=int(work[1]), block_visits=int(work[2]), new_blocks=int(work[3]),
+            sessions=int(work[4]), allocated_blocks=int(work[5]), unique_blocks=int(work[6]))

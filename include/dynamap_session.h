@@ -108,8 +108,9 @@ DM_API int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int3
  * stream) over the admitted sessions at trace frame `frame`; advances their
  * state.  stage_ms[7]: source maps, ICP reduction (level 0), ICP solve,
  * allocation (all kernels), integration, raycast, ray traversal alone.
- * work[6]: pixels, visible blocks, traversal block visits, -, sessions,
- * allocated blocks. */
+ * work[7]: pixels, visible blocks, traversal block visits, blocks inserted
+ * by the frame, sessions, allocated blocks, distinct blocks visited by the
+ * traversal. */
 DM_API int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage_ms, int64_t *work);
 /* Retire every session (free their state). */
 DM_API int dm_server_reset(dm_server *srv);
