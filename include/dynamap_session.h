@@ -74,10 +74,10 @@ typedef struct dm_server_config {
   int32_t use_graph;          /* capture the batched frame pipeline in a HIP graph */
 } dm_server_config;
 
-/* Register a reconstruction request: parameters, first pose and its frames
- * (nframes x rows x cols depth, optional rgb), copied to device memory. */
 DM_API int dm_server_create(const dm_server_config *cfg, dm_server **out);
 DM_API int dm_server_destroy(dm_server *srv);
+/* Register a reconstruction request: parameters, first pose and its frames
+ * (nframes x rows x cols depth, optional rgb), copied to device memory. */
 DM_API int dm_server_add_trace(dm_server *srv, const dm_tsdf_props *props, const dm_icp_props *icp,
                                const dm_sensor *sensor, const double init_pose[16], int32_t nframes,
                                const float *host_depth, const uint8_t *host_rgb, int32_t *trace_id);
