@@ -117,3 +117,25 @@ def test_bending_sequence_200k():
     print(f"nodes {g.nodeNum} blocks {g.num_blocks()} {10 / el:.1f} frames/s, surface err mean "
           f"{1000 * err.mean():.3f} mm max {1000 * err.max():.3f} mm, last {st}")
     assert err.mean() < 5e-4
+
+
+def test_bending_full_length_120():
+    """The deforming-cylinder workload at its full length: 120 frames (three
+    bend periods of 40 frames, through both bend directions), one solve per
+    frame; the blended surface follows the analytic bend at every 20th
+    frame."""
+    V, Nn = cylinder(500, 400)
+    sens = dict(rows=480, cols=640, fx=525.0, fy=525.0, cx=319.5, cy=239.5)
+    g = dm.DefGraph(V, Nn, 0.025, 0.05)
+    solver = dm.NonRigidICP()
+    vis = V[:, 2] < -0.07
+    errs = []
+    for f in range(120):
+        pose = cyl_pose(f)
+        solver.solve(g, cyl_depth(f, pose, sens), sens, pose)
+        if f % 20 == 19:
+            gv, _ = dm.DqBlender(g).blendMesh()
+            errs.append(float(bend_err(gv, f)[vis].mean()))
+    print("surface error mean (mm) at frames 19, 39, ..., 119:", [round(1000 * e, 4) for e in errs])
+    g.close()
+    assert max(errs) < 5e-4
