@@ -38,7 +38,7 @@ class dm_tsdf_props(C.Structure):
 class dm_icp_props(C.Structure):
     _fields_ = [("lvl_iterations", C.c_int32 * 3), ("min_depth", C.c_float), ("max_depth", C.c_float),
                 ("dist_max", C.c_float), ("photo_weight", C.c_float), ("regularization", C.c_float),
-                ("max_increment", C.c_float)]
+                ("max_increment", C.c_float), ("min_increment", C.c_float)]
 
 
 class dm_server_config(C.Structure):

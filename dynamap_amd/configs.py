@@ -30,7 +30,7 @@ TSDF_8MM = dict(voxel_size=0.008, num_buckets=1 << 20, bucket_size=10, num_block
 TSDF_4MM = dict(voxel_size=0.004, num_buckets=1 << 22, bucket_size=10, num_blocks=1 << 21, block_size=8,
                 max_sdf_weight=64, truncation_distance=0.02, max_sensor_depth=5.0, min_sensor_depth=0.3)
 ICP_DEPTH = dict(lvl_iterations=(10, 5, 4), min_depth=0.3, max_depth=5.0, dist_max=0.1, photo_weight=0.0,
-                 regularization=0.02, max_increment=0.3)
+                 regularization=0.02, max_increment=0.3, min_increment=1e-3)
 ICP_RGBD = dict(ICP_DEPTH, photo_weight=1e-3)
 
 

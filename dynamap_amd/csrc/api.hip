@@ -52,7 +52,7 @@ size_t session_bytes(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   b += (size_t)pow2_at_least((int)(2 * nc)) * 8 + nc * (8 + 4 * 4) + (size_t)p.num_buckets * 4;
   size_t maps = 0;
   for (int l = 0; l < kLevels; ++l) maps += (size_t)(rows >> l) * (cols >> l) * 16 * 4;
-  b += maps + P * (4 + 3) * 2 + P * 4 + ((P + 255) / 256 + 1) * kIcpVals * 4 + 4096;
+  b += maps + P * (4 + 3) * 2 + P * 8 + ((P + 255) / 256 + 1) * kIcpVals * 4 + 4096;
   b += ((size_t)p.num_blocks + 31) / 32 * 4;
   return b;
 }
@@ -248,7 +248,7 @@ static int run_icp(Sess &S, bool host_solve) {
   DM_HIP(hipMemcpyAsync(&has_model, S.h.ctr + C_HAS_MODEL, sizeof(int), hipMemcpyDeviceToHost, S.stream));
   DM_HIP(hipStreamSynchronize(S.stream));
   if (!has_model) return DM_OK;
-  double *Tinc = pd + PD_TINC, *acc = pd + PD_ACC;
+  double *Tinc = pd + PD_TINC, *acc = pd + PD_ACC, *pacc = pd + PD_PREVACC;
   std::vector<float> part;
   for (int l = kLevels - 1; l >= 0; --l) {
     const int P = (S.h.rows >> l) * (S.h.cols >> l);
@@ -268,11 +268,13 @@ static int run_icp(Sess &S, bool host_solve) {
       se3_exp(xi, E);
       mat4_mul(E, Tinc, Tinc);
       for (int k = 0; k < 6; ++k) acc[k] += xi[k];
+      const bool stop = level_converged(acc, pacc, (double)S.h.min_inc);
       float tf[16];
       for (int k = 0; k < 16; ++k) tf[k] = (float)Tinc[k];
       DM_HIP(hipMemcpyAsync(S.h.posef + PF_TINC, tf, sizeof tf, hipMemcpyHostToDevice, S.stream));
       DM_HIP(hipMemcpyAsync(S.h.posed + PD_TINC, Tinc, 16 * sizeof(double), hipMemcpyHostToDevice, S.stream));
       DM_HIP(hipStreamSynchronize(S.stream));
+      if (stop) break;
     }
   }
   if (total > 0) {
@@ -318,11 +320,12 @@ struct SnapHeader {
   uint32_t max_rows, max_cols, rows, cols;
   float fx, fy, cx, cy;
   int32_t num_alloc, iters[3];
-  float icp_f[6];  // icp dmin, dmax, dist_max, w_photo, lambda, max_inc
+  float icp_f[7];  // icp dmin, dmax, dist_max, w_photo, lambda, max_inc, min_inc
   int32_t ctr[C_COUNT];
   uint64_t payload;
 };
-static const char kSnapMagic[8] = {'D', 'M', 'S', 'N', 'A', 'P', '0', '2'};
+static const char kSnapMagic[8] = {'D', 'M', 'S', 'N', 'A', 'P', '0', '3'};
+static const uint32_t kSnapVersion = 3;
 
 // device sections of a volume / session state, in file order; the raycast's
 // bitmaps are not stored (restore rebuilds them from blk_near and the keys)
@@ -331,7 +334,7 @@ static std::vector<std::pair<void *, size_t>> snap_sections(const Sess &S, int n
   std::vector<std::pair<void *, size_t>> v = {
       {h.buckets, (size_t)h.num_buckets * h.bstride}, {h.block_key, (size_t)nalloc * sizeof(int4)},
       {h.vox, (size_t)nalloc * kBlockVox * 8},        {h.blk_near, (size_t)nalloc},
-      {h.posed, (size_t)PD_COUNT * 8},               {h.posef, (size_t)PF_COUNT * 4}};
+      {h.posed, (size_t)PD_PERSIST * 8},             {h.posef, (size_t)PF_COUNT * 4}};
   for (int l = 0; l < kLevels; ++l) {
     const size_t Pl = (size_t)(h.rows >> l) * (h.cols >> l);
     v.push_back({h.tvert[l], Pl * 16});
@@ -513,7 +516,7 @@ int dm_volume_snapshot(dm_volume *vol, void *buf, uint64_t cap, uint64_t *size) 
   Sess &S = *vol;
   SnapHeader H{};
   memcpy(H.magic, kSnapMagic, 8);
-  H.version = 2;
+  H.version = kSnapVersion;
   H.hdr_bytes = sizeof(SnapHeader);
   H.props = S.props;
   H.max_rows = S.max_rows; H.max_cols = S.max_cols; H.rows = S.h.rows; H.cols = S.h.cols;
@@ -522,8 +525,9 @@ int dm_volume_snapshot(dm_volume *vol, void *buf, uint64_t cap, uint64_t *size) 
   DM_HIP(hipStreamSynchronize(S.stream));
   H.num_alloc = H.ctr[C_NUM_ALLOC];
   for (int l = 0; l < 3; ++l) H.iters[l] = S.h.iters[l];
-  const float f6[6] = {S.h.icp_dmin, S.h.icp_dmax, S.h.dist_max, S.h.w_photo, S.h.lambda, S.h.max_inc};
-  memcpy(H.icp_f, f6, sizeof f6);
+  const float f7[7] = {S.h.icp_dmin, S.h.icp_dmax, S.h.dist_max, S.h.w_photo, S.h.lambda, S.h.max_inc,
+                       S.h.min_inc};
+  memcpy(H.icp_f, f7, sizeof f7);
   const auto secs = snap_sections(S, H.num_alloc);
   uint64_t total = sizeof(SnapHeader);
   for (const auto &x : secs) total += x.second;
@@ -549,7 +553,7 @@ int dm_volume_restore(dm_volume *vol, const void *buf, uint64_t size) {
   Sess &S = *vol;
   SnapHeader H;
   memcpy(&H, buf, sizeof H);
-  if (memcmp(H.magic, kSnapMagic, 8) != 0 || H.version != 2 || H.hdr_bytes != sizeof(SnapHeader)) {
+  if (memcmp(H.magic, kSnapMagic, 8) != 0 || H.version != kSnapVersion || H.hdr_bytes != sizeof(SnapHeader)) {
     set_error("dm_volume_restore: not a snapshot of this format");
     return DM_E_INVALID;
   }
@@ -583,7 +587,7 @@ int dm_volume_restore(dm_volume *vol, const void *buf, uint64_t size) {
   DM_HIP(hipMemsetAsync(S.h.bucket_head, 0xff, (size_t)S.props.num_buckets * 4, S.stream));
   for (int l = 0; l < 3; ++l) S.h.iters[l] = H.iters[l];
   S.h.icp_dmin = H.icp_f[0]; S.h.icp_dmax = H.icp_f[1]; S.h.dist_max = H.icp_f[2];
-  S.h.w_photo = H.icp_f[3]; S.h.lambda = H.icp_f[4]; S.h.max_inc = H.icp_f[5];
+  S.h.w_photo = H.icp_f[3]; S.h.lambda = H.icp_f[4]; S.h.max_inc = H.icp_f[5]; S.h.min_inc = H.icp_f[6];
   DM_TRY(S.push_view());
   rebuild_far_maps(S.launch());
   DM_HIP(hipGetLastError());
@@ -608,6 +612,7 @@ static int apply_icp(Sess &S, const dm_icp_props *icp) {
   S.h.w_photo = icp->photo_weight;
   S.h.lambda = icp->regularization;
   S.h.max_inc = icp->max_increment;
+  S.h.min_inc = icp->min_increment;
   return DM_OK;
 }
 
@@ -753,8 +758,9 @@ int dm_session_icp_track(dm_session *s, int32_t solve_on_host, double Tinc_out[1
   for (int k = 0; k < PF_COUNT; ++k) pf[k] = (k % 16) % 5 == 0 ? 1.f : 0.f;
   DM_HIP(hipMemcpyAsync(s->h.posed, pd, sizeof pd, hipMemcpyHostToDevice, s->stream));
   DM_HIP(hipMemcpyAsync(s->h.posef, pf, sizeof pf, hipMemcpyHostToDevice, s->stream));
-  int one = 1;
+  const int one = 1, zero = 0;
   DM_HIP(hipMemcpyAsync(s->h.ctr + C_HAS_MODEL, &one, 4, hipMemcpyHostToDevice, s->stream));
+  DM_HIP(hipMemcpyAsync(s->h.ctr + C_ICP_STOP, &zero, 4, hipMemcpyHostToDevice, s->stream));
   DM_HIP(hipStreamSynchronize(s->stream));
   DM_TRY(run_icp(*s, solve_on_host != 0));
   DM_HIP(hipMemcpyAsync(pd, s->h.posed, sizeof pd, hipMemcpyDeviceToHost, s->stream));

@@ -42,6 +42,7 @@ enum Ctr : int {
   C_NVIS = 10,       // blocks that passed the frustum test in the last integration
   C_NVISIT = 11,     // block visits of the last allocation's ray traversal
   C_NNEW = 12,       // keys inserted by the last allocation
+  C_ICP_STOP = 13,   // level + 1 of the ICP level that ended early this frame, 0 = none
   C_COUNT = 16
 };
 
@@ -52,7 +53,9 @@ enum PoseD : int {
   PD_TINC = 32,   // increment: current camera -> model camera
   PD_ACC = 48,    // summed twist of the frame (6)
   PD_TINIT = 64,  // pose used when no model exists yet
-  PD_COUNT = 80
+  PD_PERSIST = 80,  // the entries above persist between frames (snapshot)
+  PD_PREVACC = 80,  // summed twist after the last step that did not end its level (6)
+  PD_COUNT = 86
 };
 // pose block (float) offsets
 enum PoseF : int { PF_T = 0, PF_TINV = 16, PF_TINC = 32, PF_TPREV = 48, PF_COUNT = 64 };
@@ -86,9 +89,10 @@ struct DmView {
   const float *depth_track;  // depth the tracking maps are built from (filtered), or null = depth
   const uint8_t *rgb;   // interleaved r,g,b (may be null)
   // per pixel of the current frame, written by the allocation pass for the
-  // integration: |getPoint3d(u, v, d)| (the ray distance to the measured
-  // surface), sign bit set when d == max_depth, NaN when d is out of range
-  float *surf;
+  // integration: x = bits of |getPoint3d(u, v, d)| (the ray distance to the
+  // measured surface), sign bit set when d == max_depth, NaN when d is out of
+  // range; y = the pixel's colour r | g << 8 | b << 16 (0 without colour)
+  uint2 *surf;
   uint32_t *visited;     // num_blocks bits: blocks found by a counting traversal (profiling)
   // maps: vertex+intensity and normal per level, source (live frame) and
   // target (model raycast); level l is (rows >> l) x (cols >> l)
@@ -104,7 +108,7 @@ struct DmView {
   uint8_t *ray_rgb;
   // tracking parameters
   int iters[kLevels];
-  float icp_dmin, icp_dmax, dist_max, w_photo, lambda, max_inc;
+  float icp_dmin, icp_dmax, dist_max, w_photo, lambda, max_inc, min_inc;
   int pad_[2];
 };
 

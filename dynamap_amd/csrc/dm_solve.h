@@ -126,6 +126,20 @@ DM_HD int icp_step(const double *A_in, const double *b_in, int it, double lambda
   return 0;
 }
 
+// gnSolver.cu:413-415 with the summed twist acc and its value prev after the
+// last step that did not end a level: true (the level ends) when
+// |acc - prev| <= min_inc; otherwise prev <- acc.
+DM_HD bool level_converged(const double *acc, double *prev, double min_inc) {
+  double c2 = 0.0;
+  for (int k = 0; k < 6; ++k) {
+    const double c = acc[k] - prev[k];
+    c2 += c * c;
+  }
+  if (sqrt(c2) <= min_inc) return true;
+  for (int k = 0; k < 6; ++k) prev[k] = acc[k];
+  return false;
+}
+
 // Expand the 21 upper-triangular sums (row-major order) into a full 6x6.
 DM_HD void sym6_from_upper(const double *u21, double *A) {
   int k = 0;

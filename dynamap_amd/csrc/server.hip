@@ -96,6 +96,7 @@ static int apply_trace(Sess &S, const Trace &t) {
   for (int l = 0; l < 3; ++l) S.h.iters[l] = t.icp.lvl_iterations[l];
   S.h.icp_dmin = t.icp.min_depth; S.h.icp_dmax = t.icp.max_depth; S.h.dist_max = t.icp.dist_max;
   S.h.w_photo = t.icp.photo_weight; S.h.lambda = t.icp.regularization; S.h.max_inc = t.icp.max_increment;
+  S.h.min_inc = t.icp.min_increment;
   S.icp = t.icp;
   return DM_OK;
 }
@@ -337,7 +338,10 @@ int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int32_t nfr
     c = tr.d_rgb ? tr.d_rgb + P * 3 * (size_t)(first_frame + f) : nullptr;
   };
   int remaining = NT * nframes;
-  const Clock::time_point t0 = Clock::now();
+  // the clock (the origin of arrivals and completions, and of the wall time)
+  // starts after the set-up: graph capture and the pose log's allocation
+  Clock::time_point t0 = Clock::now();
+  double wall = 0.0;
 
   if (mode == DM_RUN_BATCHED) {
     int iters[3] = {0, 0, 0};
@@ -347,6 +351,8 @@ int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int32_t nfr
     if (r != DM_OK) return r;
     double *d_log = nullptr;
     if (poses) DM_HIP(hipMalloc(&d_log, sizeof(double) * 16 * (size_t)NT * nframes));
+    DM_HIP(hipStreamSynchronize(srv->stream));
+    t0 = Clock::now();
     hipEvent_t ev_start;
     DM_HIP(hipEventCreate(&ev_start));
     DM_HIP(hipEventRecord(ev_start, srv->stream));
@@ -474,6 +480,7 @@ int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int32_t nfr
     r = drain(true);
     if (r != DM_OK) return r;
     DM_HIP(hipStreamSynchronize(srv->stream));
+    wall = secs_since(t0);
     hipEventDestroy(ev_start);
     if (d_log) {
       DM_HIP(hipMemcpy(poses, d_log, sizeof(double) * 16 * (size_t)NT * nframes, hipMemcpyDeviceToHost));
@@ -520,7 +527,8 @@ int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int32_t nfr
   }
   for (int t = 0; t < NT; ++t)
     if (srv->traces[t].sess) srv->traces[t].next_frame = first_frame + next[t];
-  if (wall_seconds) *wall_seconds = secs_since(t0);
+  if (mode != DM_RUN_BATCHED) wall = secs_since(t0);
+  if (wall_seconds) *wall_seconds = wall;
   if (batches) *batches = nbatch;
   return DM_OK;
 }
@@ -590,11 +598,26 @@ int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage
     }
     return DM_OK;
   };
+  // the ICP stages time one full level-0 iteration: the per-level stop flag
+  // (which the step sets once the level converges and which makes further
+  // launches of the level return at once) is cleared before each repetition
+  auto clear_stop = [&]() -> int {
+    for (int t : act) {
+      Sess &S = *srv->traces[t].sess;
+      DM_HIP(hipMemsetAsync(S.h.ctr + C_ICP_STOP, 0, sizeof(int32_t), srv->stream));
+    }
+    return DM_OK;
+  };
   for (int st = 0; st < 7; ++st) {
     int r = reset_ctr();
     if (r != DM_OK) return r;
-    DM_HIP(hipEventRecord(e0, srv->stream));
+    double total_ms = 0.0;
+    const bool icp = st == 1 || st == 2;
     for (int k = 0; k < reps; ++k) {
+      if (icp || k == 0) {
+        if (icp && (r = clear_stop()) != DM_OK) return r;
+        DM_HIP(hipEventRecord(e0, srv->stream));
+      }
       switch (st) {
         case 0: launch_source_maps(L); break;
         case 1: launch_icp_reduce(L, 0, 1, true); break;
@@ -604,12 +627,15 @@ int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage
         case 5: launch_raycast(L); break;
         case 6: launch_alloc_rays(L); break;
       }
+      if (icp || k == reps - 1) {
+        DM_HIP(hipEventRecord(e1, srv->stream));
+        DM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        DM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        total_ms += ms;
+      }
     }
-    DM_HIP(hipEventRecord(e1, srv->stream));
-    DM_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    DM_HIP(hipEventElapsedTime(&ms, e0, e1));
-    stage_ms[st] = (double)ms / reps;
+    stage_ms[st] = total_ms / reps;
     if (work && (st == 4 || st == 6)) {
       int64_t sum_vis = 0, sum_visit = 0, sum_new = 0, sum_alloc = 0;
       for (int t : act) {

@@ -186,8 +186,6 @@ __device__ __forceinline__ void icp_pixel(const DmView &v, int level, const Intr
 // Tinc.  On the session's last iteration of the frame apply the reference's
 // guard (gnSolver.cu:426-428) and set the frame pose T = Tprev * Tinc.
 __device__ void icp_step_device(const DmView &v, int level, int it, const double *sums) {
-  bool last = it == v.iters[level];
-  for (int l = 0; l < level; ++l) last = last && v.iters[l] == 0;
   double A[36], b[6], xi[6], E[16];
   sym6_from_upper(sums, A);
   for (int k = 0; k < 6; ++k) b[k] = sums[21 + k];
@@ -196,6 +194,12 @@ __device__ void icp_step_device(const DmView &v, int level, int it, const double
   double *Tinc = v.posed + PD_TINC;
   mat4_mul(E, Tinc, Tinc);
   for (int k = 0; k < 6; ++k) v.posed[PD_ACC + k] += xi[k];
+  // gnSolver.cu:413-415: the level ends once the summed twist changed by at
+  // most min_increment since the last step that did not end a level
+  const bool stop = level_converged(v.posed + PD_ACC, v.posed + PD_PREVACC, (double)v.min_inc);
+  if (stop) v.ctr[C_ICP_STOP] = level + 1;
+  bool last = it == v.iters[level] || stop;
+  for (int l = 0; l < level; ++l) last = last && v.iters[l] == 0;
   if (last) {
     const double *acc = v.posed + PD_ACC;
     const double ssum = acc[0] + acc[1] + acc[2] + acc[3] + acc[4] + acc[5];
@@ -219,7 +223,7 @@ __global__ __launch_bounds__(256) void k_icp_reduce(const DmView *__restrict__ v
   const int slot_ = active[blockIdx.y];
   if (slot_ < 0) return;
   const DmView &v = views[slot_];
-  if (v.ctr[C_HAS_MODEL] == 0 || it > v.iters[level]) return;
+  if (v.ctr[C_HAS_MODEL] == 0 || it > v.iters[level] || v.ctr[C_ICP_STOP] == level + 1) return;
   const Intr s = level_intr(v, level);
   const int P = s.rows * s.cols;
   const int nwg = icp_wg_count(P);
@@ -261,7 +265,7 @@ __global__ __launch_bounds__(256) void k_icp_solve(const DmView *__restrict__ vi
   const int slot_ = active[blockIdx.x];
   if (slot_ < 0) return;
   const DmView &v = views[slot_];
-  if (v.ctr[C_HAS_MODEL] == 0 || it > v.iters[level]) return;
+  if (v.ctr[C_HAS_MODEL] == 0 || it > v.iters[level] || v.ctr[C_ICP_STOP] == level + 1) return;
   const int nwg = icp_wg_count((v.rows >> level) * (v.cols >> level));
   double acc[kVals];
 #pragma unroll
@@ -312,6 +316,8 @@ __global__ void k_pose_begin(const DmView *__restrict__ views, const int *__rest
   }
   rigid_inverse_ff(v.posef + PF_T, v.posef + PF_TINV);
   for (int k = 0; k < 6; ++k) P[PD_ACC + k] = 0.0;
+  for (int k = 0; k < 6; ++k) P[PD_PREVACC + k] = 0.0;
+  v.ctr[C_ICP_STOP] = 0;
   v.ctr[C_NVIS] = 0;
   v.ctr[C_NVISIT] = 0;
 }

@@ -98,14 +98,18 @@ __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r, unsigned
   const int i = r * v.cols + u;
   const float d = v.depth[i];
   if (d < v.min_depth || d > v.max_depth) {
-    v.surf[i] = __builtin_nanf("");
+    v.surf[i] = make_uint2(__float_as_uint(__builtin_nanf("")), 0u);
     return 0;
   }
   const Intr s = intr_of(v);
   const f3 p3 = get_point3d_uv(u, r, d, s);
-  // the integration's surface distance of this pixel (tsdf.cu:319-322)
+  // the integration's surface distance and colour of this pixel
+  // (tsdf.cu:319-322, :329-331)
   const float surf = norm(mk(0.f, 0.f, 0.f) - p3);
-  v.surf[i] = d < v.max_depth ? surf : -surf;
+  const uint32_t rgb = v.rgb ? (uint32_t)v.rgb[3 * i] | ((uint32_t)v.rgb[3 * i + 1] << 8) |
+                                   ((uint32_t)v.rgb[3 * i + 2] << 16)
+                             : 0u;
+  v.surf[i] = make_uint2(__float_as_uint(d < v.max_depth ? surf : -surf), rgb);
   const M4 T = load_m4(v.posef + PF_T);
   const float block_m = 8.0f * v.voxel_size;
   const f3 start = mk(T.m[3], T.m[7], T.m[11]);
@@ -324,7 +328,9 @@ __device__ __forceinline__ uint64_t integrate_voxel(const DmView &v, const Intr 
   // d == max_depth (no band update); NaN when d is out of [min_depth,
   // max_depth] or NaN, where the reference's comparisons all fail and the
   // voxel keeps its value
-  const float sv = ldg(v.surf + idx);
+  const uint64_t sp = ldg((const uint64_t *)(v.surf + idx));
+  const float sv = __uint_as_float((uint32_t)sp);
+  const uint32_t prgb = (uint32_t)(sp >> 32);
   if (sv != sv) return old;
   const float surf = fabsf(sv);
   const float vox = norm(mk(0.f, 0.f, 0.f) - vc);
@@ -332,10 +338,7 @@ __device__ __forceinline__ uint64_t integrate_voxel(const DmView &v, const Intr 
   if (vox > surf - tau && vox < surf + tau && !signbit(sv)) nsdf = surf - vox;
   else if (vox < surf - tau) nsdf = tau;
   else return old;
-  uint32_t nr = 0, ng = 0, nb = 0;
-  if (v.rgb) {
-    nr = v.rgb[3 * idx]; ng = v.rgb[3 * idx + 1]; nb = v.rgb[3 * idx + 2];
-  }
+  const uint32_t nr = prgb & 0xff, ng = (prgb >> 8) & 0xff, nb = (prgb >> 16) & 0xff;
   const uint32_t rgbw = vox_rgbw(old);
   const uint32_t wi = rgbw >> 24;
   const float w = (float)wi;

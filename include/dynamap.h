@@ -67,6 +67,9 @@ typedef struct dm_icp_props {
   float photo_weight;    /* weight of the photometric term (0 = geometric only) */
   float regularization;  /* Levenberg damping factor (reference 0.02) */
   float max_increment;   /* reset guard on the summed twist (reference 0.3) */
+  float min_increment;   /* a level ends once its step changes the summed twist by at
+                            most this (reference 0.001, gnSolver.cu:414; 0 = every
+                            iteration runs) */
 } dm_icp_props;
 
 DM_API const char *dm_last_error(void);

@@ -260,11 +260,13 @@ struct Properties {
   float photoWeight = 0.0f;
   float regularization = 0.02f;
   float maxIncrement = 0.3f;
+  float minIncrement = 0.001f;  // src/dynaMap.cpp:156
   dm_icp_props c() const {
     dm_icp_props p{};
     for (int l = 0; l < 3; ++l) p.lvl_iterations[l] = lvlIterationNum[l];
     p.min_depth = minDepth; p.max_depth = maxDepth; p.dist_max = distMax;
     p.photo_weight = photoWeight; p.regularization = regularization; p.max_increment = maxIncrement;
+    p.min_increment = minIncrement;
     return p;
   }
 };

@@ -865,9 +865,11 @@ OR_EXPORT int or_icp_step(const double *A_in, const double *b_in, int it, double
 
 /*
  * Full multi-level frame-to-model ICP (gnSolver.cu:317-438 restated).
- * Levels run coarse to fine with iters[l] Gauss-Newton steps each (no early
- * exit); Tinc <- exp(xi) Tinc.  K: the final guard resets the increment when
- * the summed twist is NaN/Inf or its component sum is >= max_inc.
+ * Levels run coarse to fine with at most iters[l] Gauss-Newton steps each;
+ * Tinc <- exp(xi) Tinc.  K: a level ends once the summed twist changed by at
+ * most min_inc since the last step that did not end a level
+ * (gnSolver.cu:413-415); the final guard resets the increment when the
+ * summed twist is NaN/Inf or its component sum is >= max_inc.
  * src/tgt_*: per level l = 0..2 maps of (rows >> l) x (cols >> l).
  * Output Tinc (row-major, source camera -> target camera).
  */
@@ -875,9 +877,9 @@ OR_EXPORT void or_icp_track(const float *const *sv, const float *const *sn, cons
                             const float *const *tv, const float *const *tn, const float *const *tint, int rows,
                             int cols, double fx, double fy, double cx, double cy, const int iters[3],
                             double dmin, double dmax, double dist_max, double w_photo, double lambda,
-                            double max_inc, double Tinc[16]) {
+                            double max_inc, double min_inc, double Tinc[16]) {
   double T[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-  double acc[6] = {0};
+  double acc[6] = {0}, prev[6] = {0};
   for (int l = 2; l >= 0; --l) {
     double sc = (double)(1 << l);
     int r = rows >> l, c = cols >> l;
@@ -889,6 +891,13 @@ OR_EXPORT void or_icp_track(const float *const *sv, const float *const *sn, cons
       or_se3_exp(xi, E);
       mat4_mul(E, T, T);
       for (int k = 0; k < 6; ++k) acc[k] += xi[k];
+      double c2 = 0.0;
+      for (int k = 0; k < 6; ++k) {
+        const double c = acc[k] - prev[k];
+        c2 += c * c;
+      }
+      if (sqrt(c2) <= min_inc) break;
+      memcpy(prev, acc, sizeof acc);
     }
   }
   double s = acc[0] + acc[1] + acc[2] + acc[3] + acc[4] + acc[5];

@@ -41,7 +41,7 @@ _PROTOS = {
     "or_se3_exp": (None, [_vp, _vp]),
     "or_icp_step": (_i, [_vp, _vp, _i, _d, _vp]),
     "or_icp_track": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _d, _d, _d, _d, _d, _d,
-                            _vp]),
+                            _d, _vp]),
 }
 
 _libs = {}
@@ -208,7 +208,8 @@ def icp_track(src, tgt, sensor, icp, bits=64):
     iters = np.array(icp["lvl_iterations"], np.int32)
     T = np.zeros(16)
     L.or_icp_track(sv, sn, si, tv, tn, ti, *_intr(sensor), P(iters), icp["min_depth"], icp["max_depth"],
-                   icp["dist_max"], icp["photo_weight"], icp["regularization"], icp["max_increment"], P(T))
+                   icp["dist_max"], icp["photo_weight"], icp["regularization"], icp["max_increment"],
+                   float(np.float32(icp.get("min_increment", 0.0))), P(T))
     return T.reshape(4, 4)
 
 

@@ -103,3 +103,19 @@ def test_continued_runs_and_their_checks(traces):
     b = srv.run(N.DM_RUN_PER_FRAME_DEV, 2, NF - 2, want_poses=True)["poses"]
     srv.close()
     assert np.array_equal(np.concatenate([a, b], 1), ref)
+
+
+def test_server_session_equals_standalone_session(traces):
+    """A served trace and a standalone Session with the same parameters
+    (including the ICP's per-level stop threshold) give bit-equal poses."""
+    k = 4
+    name, d, c, g = traces[k]
+    w, t = small(name)
+    assert w["icp"].get("min_increment", 0.0) > 0.0
+    srv = server(traces, sel=[k])
+    pb = srv.run(N.DM_RUN_BATCHED, 0, NF, want_poses=True)["poses"][0]
+    srv.close()
+    s = dm.Session(t, w["icp"], w["sensor"], g[0])
+    ps = np.stack([s.process_frame(d[f], None if c is None else c[f]) for f in range(NF)])
+    s.close()
+    assert np.array_equal(pb, ps)
