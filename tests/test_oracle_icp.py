@@ -102,3 +102,50 @@ def test_icp_recovers_known_pose(bits):
     _, b0, s0 = O.icp_system(src, model, SENS, 0, np.eye(4), icp, bits)
     _, b1, s1 = O.icp_system(src, model, SENS, 0, true, icp, bits)
     assert np.sqrt(s1[1] / s1[0]) < 1e-4 < np.sqrt(s0[1] / s0[0])
+
+
+def test_level_stop_rule():
+    """gnSolver.cu:413-415: a level ends after the step that changes the
+    summed twist by at most minIncrement, the change being measured from the
+    sum after the last step that did not end a level (so the step that ended
+    a level also counts towards the next level's first change).  Restated
+    here step by step from the oracle's own linearisation and step; the
+    oracle's or_icp_track must end every level at the same iteration and
+    give the same increment, and a zero threshold must run every iteration."""
+    A = pose([-0.35, 0.2, 0.0], [0.1, -0.1, 0.0])
+    B = pose([-0.34, 0.21, 0.005], [0.11, -0.103, 0.006])
+    dA, vA, nA = render_planes(A)
+    dB, _, _ = render_planes(B)
+    model = O.model_maps(vA, nA, np.zeros(dA.shape, np.float32))
+    src = O.source_maps(dB, None, SENS, 64)
+    L = O.load(64)
+    for min_inc in (0.0, 1e-3, 5e-3):
+        icp = dict(lvl_iterations=(10, 5, 4), min_depth=0.1, max_depth=10.0, dist_max=0.2, photo_weight=0.0,
+                   regularization=0.02, max_increment=0.3, min_increment=min_inc)
+        thr = float(np.float32(min_inc))
+        T = np.eye(4)
+        acc, prev = np.zeros(6), np.zeros(6)
+        ran = []
+        for lvl in (2, 1, 0):
+            n = 0
+            for it in range(1, icp["lvl_iterations"][lvl] + 1):
+                Am, b, _ = O.icp_system(src, model, SENS, lvl, T, icp, 64)
+                xi = np.zeros(6)
+                L.or_icp_step(O.P(np.ascontiguousarray(Am.ravel())), O.P(np.ascontiguousarray(b)), it,
+                              icp["regularization"], O.P(xi))
+                E = np.zeros(16)
+                L.or_se3_exp(O.P(xi), O.P(E))
+                T = E.reshape(4, 4) @ T
+                acc = acc + xi
+                n += 1
+                if np.linalg.norm(acc - prev) <= thr:
+                    break
+                prev = acc.copy()
+            ran.append(n)
+        To = O.icp_track(src, model, SENS, icp, 64)
+        print("threshold", min_inc, "iterations per level (2, 1, 0):", ran)
+        if min_inc == 0.0:
+            assert ran == [4, 5, 10]
+        else:
+            assert sum(ran) < 19
+        assert np.allclose(To, T, rtol=0, atol=1e-12)
