@@ -552,14 +552,27 @@ struct AxisSample {
   int c, i0, i1;
   float w;
 };
+// The corner voxels world_to_voxel(q - vs/2) and world_to_voxel(q - vs/2 +
+// vs) are floor(q/vs) and floor(q/vs) + 1 in exact arithmetic (the +-1/2 of
+// the dual offset and of the rounding cancel, on either sign); with the
+// float roundings they still are unless q/vs lies within a few ulp of an
+// integer, so they are taken from vq = q / vs when its fraction is more than
+// |vq| * 1e-6 + 1e-6 from 0 and 1, and evaluated otherwise.
 __device__ __forceinline__ AxisSample axis_sample(float q, float vs, float inv_vs) {
   AxisSample a;
   const float vq = q / vs;
   a.c = to_i32(vq + copysignf(0.5f, q));  // = signf_(q) * 0.5f but at q = +-0, where both give 0
-  const float dual = q - vs / 2.0f;
-  a.i0 = world_to_voxel_fast(dual + 0.0f, vs, inv_vs);
-  a.i1 = world_to_voxel_fast(dual + vs, vs, inv_vs);
-  a.w = vq - floorf(vq);
+  const float fl = floorf(vq);
+  a.w = vq - fl;
+  const float tol = fabsf(vq) * 1.0e-6f + 1.0e-6f;
+  if (a.w > tol && a.w < 1.0f - tol && fabsf(vq) < 1.0e8f) {
+    a.i0 = (int)fl;
+    a.i1 = a.i0 + 1;
+  } else {
+    const float dual = q - vs / 2.0f;
+    a.i0 = world_to_voxel_fast(dual + 0.0f, vs, inv_vs);
+    a.i1 = world_to_voxel_fast(dual + vs, vs, inv_vs);
+  }
   return a;
 }
 
