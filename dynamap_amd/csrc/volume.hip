@@ -144,13 +144,20 @@ __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r, unsigned
     visit_block<kCount>(v, bx, by, bz, seen);
     ++nvisit;
   }
-  const long long bound = llabs((long long)ex - bx) + llabs((long long)ey - by) + llabs((long long)ez - bz);
-  for (long long it = 0; it < bound && (bx != ex || by != ey || bz != ez); ++it) {
-    if (mtx < mty) {
-      if (mtx < mtz) { bx += stx; mtx += dtx; } else { bz += stz; mtz += dtz; }
-    } else {
-      if (mty < mtz) { by += sty; mty += dty; } else { bz += stz; mtz += dtz; }
-    }
+  const long long bound64 = llabs((long long)ex - bx) + llabs((long long)ey - by) + llabs((long long)ez - bz);
+  const int bound = (int)min(bound64, (long long)0x7fffffff);
+  for (int it = 0; it < bound && (bx != ex || by != ey || bz != ez); ++it) {
+    // the reference's nested comparisons, as selects: x when mtx is below
+    // both, else y when mty < mtz, else z
+    const bool sx = mtx < mty && mtx < mtz;
+    const bool sy = !(mtx < mty) && mty < mtz;
+    const bool sz = !sx && !sy;
+    bx += sx ? stx : 0;
+    mtx += sx ? dtx : 0.0f;
+    by += sy ? sty : 0;
+    mty += sy ? dty : 0.0f;
+    bz += sz ? stz : 0;
+    mtz += sz ? dtz : 0.0f;
     visit_block<kCount>(v, bx, by, bz, seen);
     ++nvisit;
   }
