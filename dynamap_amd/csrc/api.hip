@@ -232,11 +232,9 @@ static int total_iters(const DmView &h) { return h.iters[0] + h.iters[1] + h.ite
 static int run_icp(Sess &S, bool host_solve) {
   const Launch L = S.launch();
   const int total = total_iters(S.h);
-  int done = 0;
   if (!host_solve) {
     for (int l = kLevels - 1; l >= 0; --l)
       for (int it = 1; it <= S.h.iters[l]; ++it) {
-        ++done;
         launch_icp_reduce(L, l, it, true);
       }
     DM_HIP(hipGetLastError());
@@ -255,7 +253,6 @@ static int run_icp(Sess &S, bool host_solve) {
     const int nwg = icp_wg_count(P);
     part.resize((size_t)nwg * kIcpVals);
     for (int it = 1; it <= S.h.iters[l]; ++it) {
-      ++done;
       launch_icp_reduce(L, l, it, false);
       DM_HIP(hipMemcpyAsync(part.data(), S.h.partials, part.size() * sizeof(float), hipMemcpyDeviceToHost,
                             S.stream));

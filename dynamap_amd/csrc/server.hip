@@ -637,11 +637,11 @@ int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage
     }
     stage_ms[st] = total_ms / reps;
     if (work && (st == 4 || st == 6)) {
-      int64_t sum_vis = 0, sum_visit = 0, sum_new = 0, sum_alloc = 0;
+      int64_t sum_vis = 0, sum_visit = 0, sum_alloc = 0;
       for (int t : act) {
         int32_t c[C_COUNT];
         DM_HIP(hipMemcpy(c, srv->traces[t].sess->h.ctr, sizeof c, hipMemcpyDeviceToHost));
-        sum_vis += c[C_NVIS]; sum_visit += c[C_NVISIT]; sum_new += c[C_NNEW]; sum_alloc += c[C_NUM_ALLOC];
+        sum_vis += c[C_NVIS]; sum_visit += c[C_NVISIT]; sum_alloc += c[C_NUM_ALLOC];
       }
       if (st == 4) { work[1] = sum_vis / reps; work[5] = sum_alloc; }
       if (st == 6) { work[2] = sum_visit / reps; }
