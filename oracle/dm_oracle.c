@@ -544,7 +544,7 @@ static inline uint8_t to_u8(real x) {
 /*
  * Per pixel, march the z-depth parameter from min_depth towards max_depth.
  * K: samples are GetInterpolatedVoxel at T * getPoint3d(i, z).
- * D (BASELINE raycast): the step is 0.5*trunc through unobserved space and
+ * D (raycast of these workloads): the step is 0.5*trunc through unobserved space and
  * min(sdf, 0.5*trunc) (at least half a voxel) through observed space; a hit
  * is the first +/- zero crossing, located by linear interpolation between the
  * bracketing samples (the reference stops at sdf < voxel_size, up to one

@@ -16,7 +16,7 @@ def pytest_configure(config):
 
 
 def _gpu_present():
-    # decided from the driver node, not from the library: on a GPU box a
+    # decided from the kernel device node, not from the library: on a GPU box a
     # missing or broken library must fail the GPU tests, not skip them
     return os.path.exists("/dev/kfd")
 
