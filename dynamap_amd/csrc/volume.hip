@@ -739,10 +739,24 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, f
     x0 = floordiv8((int)floorf(px - e) - 1); x1 = floordiv8((int)floorf(px + 1.0f + e));
     y0 = floordiv8((int)floorf(py - e) - 1); y1 = floordiv8((int)floorf(py + 1.0f + e));
     z0 = floordiv8((int)floorf(pz - e) - 1); z1 = floordiv8((int)floorf(pz + 1.0f + e));
-    for (int qx = x0; qx <= x1; ++qx)
-      for (int qy = y0; qy <= y1; ++qy)
-        for (int qz = z0; qz <= z1; ++qz)
-          if (!block_far_any(v, qx, qy, qz)) return false;
+    // x-adjacent blocks are adjacent bits of the window: one word (two at a
+    // word boundary) covers the x range of a (y, z) row; blocks outside the
+    // window go through the hash table
+    const bool xin = x0 >= -kWin / 2 && x1 < kWin / 2 && x1 - x0 <= 1;
+    for (int qz = z0; qz <= z1; ++qz)
+      for (int qy = y0; qy <= y1; ++qy) {
+        uint32_t i0;
+        if (xin && win_bit(x0, qy, qz, i0)) {
+          const uint32_t i1 = i0 + (uint32_t)(x1 - x0);
+          const bool split = (i1 >> 5) != (i0 >> 5);
+          uint32_t hit = (ldg(v.nearmap + (i0 >> 5)) >> (i0 & 31)) & (split ? 1u : (2u << (i1 - i0)) - 1u);
+          if (split) hit |= ldg(v.nearmap + (i1 >> 5)) & 1u;
+          if (hit) return false;
+        } else {
+          for (int qx = x0; qx <= x1; ++qx)
+            if (!block_far_any(v, qx, qy, qz)) return false;
+        }
+      }
   }
   float zm = 3.0e38f;
   const float m0 = 1.0f + 2.0f * e, m1 = 7.0f - 2.0f * e;
