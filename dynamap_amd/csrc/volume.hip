@@ -721,7 +721,7 @@ __device__ __forceinline__ bool block_far_any(const DmView &v, int bx, int by, i
 //    and the run extends over those blocks.
 // Runs end e + 1 voxel inside the region, so the exact positions up to zmax
 // keep their voxel range inside it.
-__device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, float z, float &zmax) {
+__device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, float z, float &zmax) {
   const float e = 0.05f;
   const float px = po.x + pd.x * z, py = po.y + pd.y * z, pz = po.z + pd.z * z;
   if (!(fabsf(px) < 1.0e6f && fabsf(py) < 1.0e6f && fabsf(pz) < 1.0e6f)) return false;
@@ -759,6 +759,9 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, f3 ipd, f
       }
   }
   float zm = 3.0e38f;
+  // the hardware reciprocal (1 ulp): zmax only needs to be accurate to well
+  // within e, and the direction is recomputed here rather than kept live
+  const f3 ipd = mk(__builtin_amdgcn_rcpf(pd.x), __builtin_amdgcn_rcpf(pd.y), __builtin_amdgcn_rcpf(pd.z));
   const float m0 = 1.0f + 2.0f * e, m1 = 7.0f - 2.0f * e;
   // reciprocal direction: zmax only needs to be accurate to well within e
   if (pd.x > 0.f) zm = fminf(zm, ((float)(8 * x1) + m1 - po.x) * ipd.x);
@@ -796,7 +799,6 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
   // ray in voxel units for the approximate empty-space test
   const f3 po = mk(T.m[3], T.m[7], T.m[11]) * inv_vs;
   const f3 pd = rot(T, mk(((float)u - s.cx) / s.fx, ((float)r - s.cy) / s.fy, 1.0f)) * inv_vs;
-  const f3 ipd = mk(pd.x != 0.f ? 1.0f / pd.x : 0.f, pd.y != 0.f ? 1.0f / pd.y : 0.f, pd.z != 0.f ? 1.0f / pd.z : 0.f);
   __shared__ int win_lds[8 * 256];
   BlockWin c;
   c.wx = 0x3fffffff; c.wy = 0; c.wz = 0;
@@ -810,7 +812,7 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
     // far samples: the step is 0.5*trunc whether they are observed or not;
     // the run of far samples up to zmax is stepped over without lookups (the
     // last one's interpolation is recomputed below if the next sample needs it)
-    if (far_run(v, po, pd, ipd, z, zmax)) {
+    if (far_run(v, po, pd, z, zmax)) {
       prev_fast = true;
       zprev = z;
       z = z + big;
