@@ -539,8 +539,8 @@ __device__ __forceinline__ float intensity_of(uint32_t r, uint32_t g, uint32_t b
 // through rounding at huge coordinates) looks its voxels up one by one.
 struct BlockWin {
   int wx, wy, wz;      // window base block
-  unsigned resolved;   // bit k: LDS slot k holds the pool index (or -1)
-  int dx, dy, dz;      // 1 where the ray runs towards -axis (the window extends that way)
+  unsigned resolved;   // bit k < 8: LDS slot k holds the pool index; bits 8, 9, 10: the ray
+                       // runs towards -x, -y, -z (the window extends that way)
   int *slot;           // this thread's slots: slot[k * 256], k = x + 2y + 4z
 };
 
@@ -622,10 +622,10 @@ __device__ __forceinline__ bool interp_axes(const DmView &v, const AxisSample &X
     return true;
   }
   if (lox < W.wx || hix > W.wx + 1 || loy < W.wy || hiy > W.wy + 1 || loz < W.wz || hiz > W.wz + 1) {
-    W.wx = lox - (hix == lox ? W.dx : 0);
-    W.wy = loy - (hiy == loy ? W.dy : 0);
-    W.wz = loz - (hiz == loz ? W.dz : 0);
-    W.resolved = 0u;
+    W.wx = lox - (hix == lox ? (int)((W.resolved >> 8) & 1u) : 0);
+    W.wy = loy - (hiy == loy ? (int)((W.resolved >> 9) & 1u) : 0);
+    W.wz = loz - (hiz == loz ? (int)((W.resolved >> 10) & 1u) : 0);
+    W.resolved &= 0x700u;
   }
   const int kx[2] = {(ix[0] >> 3) - W.wx, (ix[1] >> 3) - W.wx};
   const int ky[2] = {2 * ((iy[0] >> 3) - W.wy), 2 * ((iy[1] >> 3) - W.wy)};
@@ -802,8 +802,7 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
   __shared__ int win_lds[8 * 256];
   BlockWin c;
   c.wx = 0x3fffffff; c.wy = 0; c.wz = 0;
-  c.resolved = 0u;
-  c.dx = pd.x < 0.f; c.dy = pd.y < 0.f; c.dz = pd.z < 0.f;
+  c.resolved = (pd.x < 0.f ? 0x100u : 0u) | (pd.y < 0.f ? 0x200u : 0u) | (pd.z < 0.f ? 0x400u : 0u);
   c.slot = win_lds + threadIdx.x;
   float z = v.min_depth, pz = 0.f, ps = 0.f, zh = 0.f, zprev = 0.f;
   bool pobs = false, hit = false, prev_fast = false;
