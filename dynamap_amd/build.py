@@ -85,22 +85,10 @@ def build_cpp_tests():
     return out
 
 
-def build_probe():
-    """hbm_probe: plain fill and copy kernels of known byte counts, the
-    reference for scaling the FETCH_SIZE / WRITE_SIZE counters."""
-    os.makedirs(LIB, exist_ok=True)
-    out = os.path.join(LIB, "hbm_probe")
-    _run(["hipcc", *[f for f in HIP_FLAGS if f not in ("-fPIC", "-fvisibility=hidden")],
-          os.path.join(CSRC, "hbm_probe.hip"), "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
-    return out
-
-
 def build_all():
     build_synth()
     build_hip()
     build_cpp_tests()
-    build_probe()
 
 
 if __name__ == "__main__":

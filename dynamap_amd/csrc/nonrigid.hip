@@ -638,7 +638,9 @@ __global__ __launch_bounds__(256) void k_nr_invdiag(const double *__restrict__ H
 struct PcgState {
   double *x, *r, *z, *p, *q, *s;  // 6 N each; z = u = M r, q = w = A u, s = A p
   double *part_pq;            // per-workgroup partials of w.u (written by spmv, read by update)
-  double *part_rz;            // per-workgroup partials of r.u (written by init/update, read by update)
+  double *part_rz;            // per-workgroup partials of r.u, two halves of nwg: init writes
+                              // half 0, update k reads half k & 1 and writes half (k + 1) & 1, so
+                              // no workgroup overwrites a partial another one may still read
   double *rz;                 // rz[k] = r_k . u_k (k = 0 .. max)
   double *al;                 // al[k]: step length of iteration k
   int *flag;                  // [0] converged, [1] iterations run
@@ -664,6 +666,7 @@ __device__ double sum_partials(const double *part, int n, double *lds1) {
 
 __device__ void wg_partial(double x, double *lds, double *part) {
   const double w = wave_sum(x);
+  __syncthreads();  // lds may still be read by an earlier sum of this workgroup
   const int wid = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) lds[wid] = w;
   __syncthreads();
@@ -761,7 +764,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const do
                                                            int it, const NrFrame *Fp) {
   if (S.flag[0]) return;
   __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1], l1[1];
-  const double gamma = sum_partials(S.part_rz, S.nwg, l1);
+  const double gamma = sum_partials(S.part_rz + (size_t)(it & 1) * S.nwg, S.nwg, l1);
   const double delta = sum_partials(S.part_pq, S.nwg_s, l1);
   const double tol = Fp->pcg_tol;
   const bool stop = it == 0 ? !(gamma > 0.0) : (!(gamma > tol * tol * S.rz[0]) || !(gamma > 0.0));
@@ -797,8 +800,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const do
     z = precond_row(Minv, rl + (threadIdx.x - row), node, row);
     S.z[t] = z;
   }
-  __syncthreads();
-  wg_partial(r * z, lds, S.part_rz);
+  wg_partial(r * z, lds, S.part_rz + (size_t)((it + 1) & 1) * S.nwg);
 }
 
 // T_i <- exp(x_i) T_i, then the node's dual quaternion
@@ -1059,7 +1061,7 @@ static int graph_build(dm_graph *g, const float *h_verts, const float *h_normals
   DM_TRY(g->dalloc(&S.q, nn * 6));
   DM_TRY(g->dalloc(&S.s, nn * 6));
   DM_TRY(g->dalloc(&S.part_pq, S.nwg_s));
-  DM_TRY(g->dalloc(&S.part_rz, S.nwg));
+  DM_TRY(g->dalloc(&S.part_rz, 2 * (size_t)S.nwg));
   DM_TRY(g->dalloc(&S.flag, 2));
   DM_TRY(g->dalloc(&g->d_frame, 1));
   DM_HIP(hipHostMalloc((void **)&g->h_frame, sizeof(NrFrame), hipHostMallocDefault));

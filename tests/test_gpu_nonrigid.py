@@ -85,6 +85,23 @@ def test_solve_matches_fp64_oracle(bend_case):
     assert bend_err(gv, 10)[vis].mean() < 3e-4 and bend_err(ov, 10)[vis].mean() < 3e-4
 
 
+def test_solve_is_deterministic(bend_case):
+    """Two graphs built from the same mesh and solved on the same frames give
+    bit-equal node transforms: every PCG dot product is a fixed-order sum of
+    workgroup partials, and no workgroup overwrites a partial that another
+    one of the same launch still reads."""
+    V, Nn, pose, depth = bend_case
+    out = []
+    for _ in range(2):
+        g = dm.DefGraph(V, Nn, 0.025, 0.05)
+        solver = dm.NonRigidICP()
+        for f in (4, 8, 12):
+            solver.solve(g, cyl_depth(f, cyl_pose(f)), SENS, cyl_pose(f))
+        out.append(g.transforms())
+        g.close()
+    assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
+
+
 def test_identity_target_keeps_surface():
     V, Nn = cylinder(200, 100)
     pose = cyl_pose(0)
