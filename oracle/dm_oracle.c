@@ -21,10 +21,13 @@
  *
  * Pinning (see tests/): hash vs an independent Python formula and a
  * hand-computed table; DDA vs brute-force segment sampling; integration vs the
- * closed form of a fronto-parallel wall; raycast vs analytic ray-plane /
- * ray-sphere depth; LDLT vs numpy.linalg.solve; SE(3) exp vs scipy expm; ICP
- * vs known-pose convergence and an FP64 brute-force normal-equation sum; MC
- * tables vs a digest of the reference's tables, MC vs sphere topology.
+ * closed form of a fronto-parallel wall; trilinear sampling exact on a linear
+ * field; raycast vs analytic ray-plane / ray-sphere depth; normal map vs a
+ * plane's normal and a cubic surface's closed form; LDLT vs numpy.linalg.solve;
+ * SE(3) exp vs scipy expm; ICP normal equations (geometric and photometric) vs
+ * an FP64 per-pixel sum with finite-difference Jacobians, known-pose
+ * convergence, photometric recovery of in-plane motion; MC tables vs a digest
+ * of the reference's tables, MC vs sphere topology.
  */
 #include <math.h>
 #include <stdint.h>
@@ -530,6 +533,17 @@ static int interp_voxel(const or_volume *v, r3 p, real *sdf_out, real col[3]) {
   *sdf_out = dist;
   if (col) { col[0] = c0; col[1] = c1; col[2] = c2; }
   return 1;
+}
+
+/* Test hook: interp_voxel at n world points (x, y, z triples); ok[k] = 0
+ * where the centre voxel is unobserved (sdf and colour then left at 0). */
+OR_EXPORT void or_interp(const or_volume *v, int n, const double *pts, double *sdf, double *col, int *ok) {
+  for (int k = 0; k < n; ++k) {
+    real s = 0, c[3] = {0, 0, 0};
+    ok[k] = interp_voxel(v, mk3((real)pts[3 * k], (real)pts[3 * k + 1], (real)pts[3 * k + 2]), &s, c);
+    sdf[k] = (double)s;
+    for (int a = 0; a < 3; ++a) col[3 * k + a] = (double)c[a];
+  }
 }
 
 static inline uint8_t to_u8(real x) {

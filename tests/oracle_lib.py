@@ -11,7 +11,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-BUILD = os.path.join(ORACLE_DIR, "_build")
+# DM_ORACLE_BUILD: another build of the oracle (a copy with a planted mistake)
+BUILD = os.environ.get("DM_ORACLE_BUILD", os.path.join(ORACLE_DIR, "_build"))
 _vp = C.c_void_p
 _d = C.c_double
 _i = C.c_int
@@ -281,6 +282,22 @@ def _set_blocks(self, xyz, sdf, w, rgb=None):
 
 
 OracleVolume.set_blocks = _set_blocks
+
+
+def _interp(self, pts):
+    """Trilinear samples of the volume at world points (n, 3) (test hook of
+    GetInterpolatedVoxel): (sdf (n,), colour (n, 3), observed (n,) bool)."""
+    L = self.L
+    L.or_interp.restype = None
+    L.or_interp.argtypes = [_vp, _i, _vp, _vp, _vp, _vp]
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    n = len(pts)
+    sdf, col, ok = np.zeros(n), np.zeros((n, 3)), np.zeros(n, np.int32)
+    L.or_interp(self.h, n, P(pts), P(sdf), P(col), P(ok))
+    return sdf, col, ok.astype(bool)
+
+
+OracleVolume.interp = _interp
 
 
 # ---------------------------------------------------------------- non-rigid

@@ -118,3 +118,99 @@ def test_raycast_wall_and_sphere(bits):
     v_, u_ = np.mgrid[0:S["rows"], 0:S["cols"]]
     np.testing.assert_allclose(vert[..., 2][hit], d[hit], rtol=0, atol=0)
     np.testing.assert_allclose(vert[..., 0][hit], ((u_ - S["cx"]) * d / S["fx"])[hit], atol=1e-6)
+
+
+def linear_volume(bits, a, hole=None):
+    """Blocks -2..1 on each axis, every voxel observed, voxel g (global index
+    triple) holding sdf = a[:3].(g vs) + a[3] and colour 100 + g, so that both
+    are linear functions of the voxel centre g vs (tsdf.cu:189-239 puts voxel
+    g's centre at g vs: WorldToGlobalVoxelHashing rounds half away from zero).
+    `hole`: a global voxel index left unobserved (weight 0, sdf 5)."""
+    vs = PROPS["voxel_size"]
+    vol = O.OracleVolume(PROPS, bits)
+    rng = range(-2, 2)
+    bl = np.array([(x, y, z) for x in rng for y in rng for z in rng], np.int32)
+    loc = np.arange(512)
+    lxyz = np.stack([loc // 64, (loc // 8) % 8, loc % 8], -1)  # x-major, z fastest
+    g = bl[:, None, :] * 8 + lxyz[None]
+    sdf = (g * vs) @ np.asarray(a[:3]) + a[3]
+    w = np.ones(g.shape[:2], np.uint8)
+    if hole is not None:
+        m = (g == np.asarray(hole)).all(-1)
+        w[m] = 0
+        sdf[m] = 5.0
+    assert vol.set_blocks(bl, sdf, w, (100 + g).astype(np.uint8)) == 0
+    return vol
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_trilinear_exact_on_linear_field(bits):
+    """GetInterpolatedVoxel (tsdf.cu:47-175) reproduces a linear field
+    exactly wherever no coordinate is a whole number of voxels: its corners
+    are the voxels at floor(p / vs) + {0, 1} per axis and its weights
+    frac(p / vs), so sum(w * f(corner)) = f(p).  A swapped weight (1 - f for
+    f), corners offset by half a voxel, or axes exchanged each miss by up to
+    a[k] * vs.
+
+    K (reference quirk): the corners come from rounding p - vs/2 half away
+    from zero (WorldToGlobalVoxelHashing, tsdf.cu:189-193) while the weights
+    use floor(p / vs).  On an axis with p / vs = k <= 0 exactly, the rounding
+    gives corner k - 1 and the weight puts all of that axis on it, so the
+    sample is f(p) - a[axis] * vs per such axis; k = 0 is exact in both
+    builds."""
+    vs = PROPS["voxel_size"]
+    a = np.array([0.9, -0.45, 0.3, 0.02])
+    vol = linear_volume(bits, a)
+    pts = np.random.default_rng(11).uniform(-0.14, 0.14, (4000, 3))
+    sdf, col, ok = vol.interp(pts)
+    assert ok.all()
+    pf = pts.astype(np.float32).astype(np.float64) if bits == 32 else pts
+    want = pf @ a[:3] + a[3]
+    tol = 1e-12 if bits == 64 else 2e-6
+    assert np.abs(sdf - want).max() < tol
+    want_c = 100 + pf / vs
+    assert np.abs(col - want_c).max() < (1e-9 if bits == 64 else 2e-4)
+    # coordinates exactly 0 on one, two or three axes
+    q = np.random.default_rng(13).uniform(-0.13, 0.13, (600, 3))
+    zero = np.zeros((600, 3), bool)
+    zero[:200, 0] = True
+    zero[200:400, 1:] = True
+    zero[400:] = True
+    q[zero] = 0.0
+    sdf, col, ok = vol.interp(q)
+    assert ok.all()
+    qf = q.astype(np.float32).astype(np.float64) if bits == 32 else q
+    want = qf @ a[:3] + a[3] - (zero * a[:3] * vs).sum(-1)
+    assert np.abs(sdf - want).max() < tol
+    assert np.abs(col - (100 + qf / vs - zero)).max() < (1e-9 if bits == 64 else 2e-4)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_trilinear_unobserved_corner_takes_centre_voxel(bits):
+    """tsdf.cu:47-175: a corner voxel with weight 0 contributes the values of
+    the centre voxel (the voxel containing p) instead of its own; an
+    unobserved centre voxel makes the sample unobserved.  On a linear field
+    the result is therefore f(p) + c_h (f(centre) - f(hole)) with c_h the
+    trilinear weight of the hole's corner."""
+    vs = PROPS["voxel_size"]
+    a = np.array([0.9, -0.45, 0.3, 0.02])
+    hole = (3, -2, 1)
+    vol = linear_volume(bits, a, hole=hole)
+    rng = np.random.default_rng(12)
+    # points whose 8 corners include the hole but whose centre voxel is another one
+    pts = (np.asarray(hole) + rng.uniform(-0.98, 0.98, (2000, 3))) * vs
+    centre = np.floor(pts / vs + 0.5).astype(int)
+    corner0 = np.floor(pts / vs).astype(int)
+    sdf, col, ok = vol.interp(pts)
+    at_hole = (centre == hole).all(-1)
+    assert not ok[at_hole].any() and ok[~at_hole].all()
+    f = lambda g: (g * vs) @ a[:3] + a[3]
+    wt = pts / vs - corner0
+    d = np.asarray(hole) - corner0
+    in_cell = ((d == 0) | (d == 1)).all(-1)
+    c_h = np.prod(np.where(d == 1, wt, 1 - wt), -1) * in_cell
+    want = pts @ a[:3] + a[3] + c_h * (f(centre) - f(np.asarray(hole)))
+    sel = ~at_hole
+    tol = 1e-12 if bits == 64 else 5e-6
+    assert np.abs(sdf[sel] - want[sel]).max() < tol
+    assert (c_h[sel] > 0.05).sum() > 200  # the rule was exercised
