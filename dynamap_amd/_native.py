@@ -88,6 +88,11 @@ PROTOTYPES = {
     "dm_graph_get_transforms": (C.c_int, [_vp, _vp]),
     "dm_blend": (C.c_int, [_vp, _vp, _vp]),
     "dm_nonrigid_solve": (C.c_int, [_vp, _vp, _P(dm_sensor), _vp, _P(dm_nonrigid_props), _vp]),
+    "dm_nonrigid_reserve": (C.c_int, [_vp, C.c_int64, C.c_int32]),
+    "dm_nonrigid_solve_pair": (C.c_int, [_vp, _vp, _vp, _P(dm_sensor), _vp, _P(dm_nonrigid_props), _vp]),
+    "dm_nonrigid_solve_mesh": (C.c_int, [_vp, _vp, _vp, _vp, _P(dm_nonrigid_props), _vp]),
+    "dm_graph_create_nodes": (C.c_int, [_vp, _vp, C.c_int64, _vp, C.c_int32, C.c_float, _P(_vp)]),
+    "dm_graph_update_nodes": (C.c_int, [_vp]),
     "dm_mesh_extract": (C.c_int, [_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _P(C.c_int64)]),
     "dm_session_create": (C.c_int, [_P(dm_tsdf_props), _P(dm_icp_props), _P(dm_sensor), _vp, _P(_vp)]),
     "dm_session_destroy": (C.c_int, [_vp]),
@@ -100,6 +105,7 @@ PROTOTYPES = {
     "dm_session_set_model_maps": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp]),
     "dm_session_icp_system": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp]),
     "dm_session_icp_track": (C.c_int, [_vp, C.c_int32, _vp]),
+    "dm_session_icp_increment": (C.c_int, [_vp, _vp]),
     "dm_server_create": (C.c_int, [_P(dm_server_config), _P(_vp)]),
     "dm_server_destroy": (C.c_int, [_vp]),
     "dm_server_add_trace": (C.c_int, [_vp, _P(dm_tsdf_props), _P(dm_icp_props), _P(dm_sensor), _vp, C.c_int32,

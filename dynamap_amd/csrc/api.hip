@@ -766,4 +766,16 @@ int dm_session_icp_track(dm_session *s, int32_t solve_on_host, double Tinc_out[1
   return DM_OK;
 }
 
+int dm_session_icp_increment(dm_session *s, double twist[6]) {
+  if (!s || !twist) { set_error("null argument"); return DM_E_INVALID; }
+  double acc[6];
+  DM_HIP(hipMemcpyAsync(acc, s->h.posed + PD_ACC, sizeof acc, hipMemcpyDeviceToHost, s->stream));
+  DM_HIP(hipStreamSynchronize(s->stream));
+  // gnSolver.cu:426-428: a NaN, infinite or too large sum is replaced by zero
+  const double ssum = acc[0] + acc[1] + acc[2] + acc[3] + acc[4] + acc[5];
+  const bool reset = ssum != ssum || std::isinf(ssum) || ssum >= (double)s->h.max_inc;
+  for (int k = 0; k < 6; ++k) twist[k] = reset ? 0.0 : acc[k];
+  return DM_OK;
+}
+
 }  // extern "C"

@@ -378,6 +378,7 @@ struct NrFrame {
   double fx, fy, cx, cy;
   double pose[16];  // camera-to-world
   double min_depth, max_depth, dist_max, normal_cos, lambda, damping, pcg_tol;
+  int use_src;      // gate the depth target by the source (predicted) depth's range
 };
 
 // target vertex (camera) per pixel, components rounded to float (w = 1 valid)
@@ -421,10 +422,30 @@ __global__ __launch_bounds__(256) void k_nr_target_norm(const float4 *__restrict
 
 // per-vertex data term: g = [p' x n, n] (world), residual r = n . (p' - q);
 // out[8 * v] = g0..g5, r, valid
+__device__ __forceinline__ void nr_data_record(const double p[3], const double nt[3], const double q[3],
+                                               const double *pose, float *o) {
+  double nW[3], qW[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    nW[r] = pose[4 * r] * nt[0] + pose[4 * r + 1] * nt[1] + pose[4 * r + 2] * nt[2];
+    qW[r] = pose[4 * r] * q[0] + pose[4 * r + 1] * q[1] + pose[4 * r + 2] * q[2] + pose[4 * r + 3];
+  }
+  const double res = nW[0] * (p[0] - qW[0]) + nW[1] * (p[1] - qW[1]) + nW[2] * (p[2] - qW[2]);
+  o[0] = (float)(p[1] * nW[2] - p[2] * nW[1]);
+  o[1] = (float)(p[2] * nW[0] - p[0] * nW[2]);
+  o[2] = (float)(p[0] * nW[1] - p[1] * nW[0]);
+  o[3] = (float)nW[0]; o[4] = (float)nW[1]; o[5] = (float)nW[2];
+  o[6] = (float)res;
+  o[7] = 1.f;
+}
+
+// source depth (sdepth) gates a pixel when F.use_src: the reference's range
+// test on the source image (nonRigidICP.cu:291-292)
 __global__ __launch_bounds__(256) void k_nr_data(const float *__restrict__ pv, const float *__restrict__ pn,
                                                 int64_t nv, const float4 *__restrict__ tv,
                                                 const float4 *__restrict__ tn, const float *__restrict__ depth,
-                                                const NrFrame *Fp, float *__restrict__ out) {
+                                                const float *__restrict__ sdepth, const NrFrame *Fp,
+                                                float *__restrict__ out) {
   const NrFrame &F = *Fp;
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (v >= nv) return;
@@ -447,25 +468,47 @@ __global__ __launch_bounds__(256) void k_nr_data(const float *__restrict__ pv, c
   const int pix = vv * F.cols + u;
   const float d = depth[pix];
   if (!(d >= (float)F.min_depth && d <= (float)F.max_depth)) return;
+  if (F.use_src) {
+    const float ds = sdepth[pix];
+    if (!(ds >= (float)F.min_depth && ds <= (float)F.max_depth)) return;
+  }
   const float4 qv = tv[pix], qn = tn[pix];
   if (qn.w == 0.f) return;
   const double q[3] = {qv.x, qv.y, qv.z}, nt[3] = {qn.x, qn.y, qn.z};
   const double dx = pc[0] - q[0], dy = pc[1] - q[1], dz = pc[2] - q[2];
   if (dx * dx + dy * dy + dz * dz > F.dist_max * F.dist_max) return;
   if (fabs(ncam[0] * nt[0] + ncam[1] * nt[1] + ncam[2] * nt[2]) < F.normal_cos) return;
-  double nW[3], qW[3];
+  nr_data_record(p, nt, q, pose, o);
+}
+
+// mesh target (nonRigidICP.cu:354-376): target vertex v and its normal, in
+// camera coordinates, correspond to canonical vertex v; same gates and
+// record as the depth target
+__global__ __launch_bounds__(256) void k_nr_data_mesh(const float *__restrict__ pv, const float *__restrict__ pn,
+                                                     int64_t nv, const float *__restrict__ mv,
+                                                     const float *__restrict__ mn, const NrFrame *Fp,
+                                                     float *__restrict__ out) {
+  const NrFrame &F = *Fp;
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nv) return;
+  float *o = out + 8 * v;
+  o[7] = 0.f;
+  const double *pose = F.pose;
+  const double p[3] = {pv[3 * v], pv[3 * v + 1], pv[3 * v + 2]};
+  const double nw[3] = {pn[3 * v], pn[3 * v + 1], pn[3 * v + 2]};
+  double pc[3], ncam[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    nW[r] = pose[4 * r] * nt[0] + pose[4 * r + 1] * nt[1] + pose[4 * r + 2] * nt[2];
-    qW[r] = pose[4 * r] * q[0] + pose[4 * r + 1] * q[1] + pose[4 * r + 2] * q[2] + pose[4 * r + 3];
+    pc[r] = pose[r] * (p[0] - pose[3]) + pose[4 + r] * (p[1] - pose[7]) + pose[8 + r] * (p[2] - pose[11]);
+    ncam[r] = pose[r] * nw[0] + pose[4 + r] * nw[1] + pose[8 + r] * nw[2];
   }
-  const double res = nW[0] * (p[0] - qW[0]) + nW[1] * (p[1] - qW[1]) + nW[2] * (p[2] - qW[2]);
-  o[0] = (float)(p[1] * nW[2] - p[2] * nW[1]);
-  o[1] = (float)(p[2] * nW[0] - p[0] * nW[2]);
-  o[2] = (float)(p[0] * nW[1] - p[1] * nW[0]);
-  o[3] = (float)nW[0]; o[4] = (float)nW[1]; o[5] = (float)nW[2];
-  o[6] = (float)res;
-  o[7] = 1.f;
+  const double q[3] = {mv[3 * v], mv[3 * v + 1], mv[3 * v + 2]}, nt[3] = {mn[3 * v], mn[3 * v + 1], mn[3 * v + 2]};
+  const double n2 = nt[0] * nt[0] + nt[1] * nt[1] + nt[2] * nt[2];
+  if (!(n2 > 0) || !(q[0] == q[0] && q[1] == q[1] && q[2] == q[2])) return;
+  const double dx = pc[0] - q[0], dy = pc[1] - q[1], dz = pc[2] - q[2];
+  if (dx * dx + dy * dy + dz * dz > F.dist_max * F.dist_max) return;
+  if (fabs(ncam[0] * nt[0] + ncam[1] * nt[1] + ncam[2] * nt[2]) < F.normal_cos) return;
+  nr_data_record(p, nt, q, pose, o);
 }
 
 // ARAP edge e = i * K + k (j = k-th neighbour of i): a = T_i g_j, b = T_j g_j,
@@ -887,13 +930,14 @@ struct dm_graph {
   PcgState pcg{};
   int pcg_cap = 0;
   // frame staging
-  float *depth = nullptr;
+  float *depth = nullptr, *sdepth = nullptr;
   float4 *tv = nullptr, *tn = nullptr;
   size_t pixels_cap = 0;
+  float *mesh_v = nullptr, *mesh_n = nullptr;  // mesh target (nv x 3 each), allocated on first use
   NrFrame *d_frame = nullptr, *h_frame = nullptr;  // device record and its pinned host copy
   // the captured solve and what it was captured for
   hipGraphExec_t exec = nullptr;
-  int key_gn = -1, key_pcg = -1, key_rows = -1, key_cols = -1;
+  int key_gn = -1, key_pcg = -1, key_rows = -1, key_cols = -1, key_mode = -1;
 
   template <class T> int dalloc(T **p, size_t n) {
     void *q = nullptr;
@@ -928,6 +972,8 @@ static int sort_pairs(const uint32_t *ki, uint32_t *ko, const uint32_t *vi, uint
   DM_HIP(hipStreamSynchronize(st));  // tmp is released on return
   return DM_OK;
 }
+
+static int graph_finish(dm_graph *g);
 
 static int graph_build(dm_graph *g, const float *h_verts, const float *h_normals) {
   hipStream_t st = g->stream;
@@ -966,17 +1012,62 @@ static int graph_build(dm_graph *g, const float *h_verts, const float *h_normals
   DM_TRY(g->dalloc(&g->node_of_vertex, nv));
   hipLaunchKernelGGL(k_nr_nodes, dim3(grid_of(nv)), dim3(256), 0, st, isnode.as<int>(), rank.as<long long>(), nv,
                      g->node_vid, g->node_of_vertex);
-  // neighbours and weights
+  return graph_finish(g);
+}
+
+// defGraph::init with the nodes given (defGraph.cu:15-69, sampleDownMeshHost
+// :673-702): node k is vertex ids[k]
+static int graph_build_nodes(dm_graph *g, const float *h_verts, const float *h_normals, const int32_t *ids, int nn) {
+  hipStream_t st = g->stream;
+  const int64_t nv = g->nv;
+  std::vector<int32_t> nov((size_t)nv, -1);
+  for (int k = 0; k < nn; ++k) {
+    if (ids[k] < 0 || ids[k] >= nv || nov[ids[k]] >= 0) {
+      set_error("deformation graph: node vertex id out of range or repeated");
+      return DM_E_INVALID;
+    }
+    nov[ids[k]] = k;
+  }
+  DM_TRY(g->dalloc(&g->verts, (size_t)nv * 3));
+  DM_TRY(g->dalloc(&g->normals, (size_t)nv * 3));
+  DM_HIP(hipMemcpyAsync(g->verts, h_verts, (size_t)nv * 12, hipMemcpyHostToDevice, st));
+  DM_HIP(hipMemcpyAsync(g->normals, h_normals, (size_t)nv * 12, hipMemcpyHostToDevice, st));
+  g->nn = nn;
+  DM_TRY(g->dalloc(&g->node_vid, nn));
+  DM_TRY(g->dalloc(&g->node_of_vertex, nv));
+  DM_HIP(hipMemcpyAsync(g->node_vid, ids, (size_t)nn * 4, hipMemcpyHostToDevice, st));
+  DM_HIP(hipMemcpyAsync(g->node_of_vertex, nov.data(), (size_t)nv * 4, hipMemcpyHostToDevice, st));
+  DM_HIP(hipStreamSynchronize(st));  // nov is released on return
+  return graph_finish(g);
+}
+
+// defGraphUpdateNodes (defGraph.cu:483-545) and the vertex neighbours
+// (updateActiveNeighbourNodes, :243-302): the K nearest nodes of every node
+// and of every vertex, with weights exp(-d^2/dgw^2)
+static int graph_links(dm_graph *g) {
+  hipStream_t st = g->stream;
+  const int nn = g->nn;
+  const int64_t nv = g->nv;
+  hipLaunchKernelGGL(k_nr_knn, dim3(grid_of(nn)), dim3(256), 0, st, g->verts, g->node_vid, nn, g->node_vid,
+                     (int64_t)nn, g->props.dgw, g->nbr, g->nbr_w, (float *)nullptr);
+  hipLaunchKernelGGL(k_nr_knn, dim3(grid_of(nv)), dim3(256), 0, st, g->verts, g->node_vid, nn,
+                     (const int *)nullptr, nv, g->props.dgw, g->vnn, g->vnn_w, g->vnn_wn);
+  DM_HIP(hipGetLastError());
+  return DM_OK;
+}
+
+// neighbours, the block pattern of the normal equations, node state and the
+// per-iteration buffers of a graph whose nodes are set
+static int graph_finish(dm_graph *g) {
+  hipStream_t st = g->stream;
+  const int64_t nv = g->nv;
+  const long long nn = g->nn;
   DM_TRY(g->dalloc(&g->nbr, nn * kNK));
   DM_TRY(g->dalloc(&g->nbr_w, nn * kNK));
   DM_TRY(g->dalloc(&g->vnn, (size_t)nv * kNK));
   DM_TRY(g->dalloc(&g->vnn_w, (size_t)nv * kNK));
   DM_TRY(g->dalloc(&g->vnn_wn, (size_t)nv * kNK));
-  hipLaunchKernelGGL(k_nr_knn, dim3(grid_of(nn)), dim3(256), 0, st, g->verts, g->node_vid, (int)nn, g->node_vid,
-                     (int64_t)nn, g->props.dgw, g->nbr, g->nbr_w, (float *)nullptr);
-  hipLaunchKernelGGL(k_nr_knn, dim3(grid_of(nv)), dim3(256), 0, st, g->verts, g->node_vid, (int)nn,
-                     (const int *)nullptr, nv, g->props.dgw, g->vnn, g->vnn_w, g->vnn_wn);
-  DM_HIP(hipGetLastError());
+  DM_TRY(graph_links(g));
   // block pattern: contributions sorted by block key (stable: vertex order, then edges)
   DevBuf ccnt, coff, wsum2;
   DM_TRY(dev_alloc(ccnt, (size_t)nv * 4));
@@ -1073,20 +1164,27 @@ static int graph_build(dm_graph *g, const float *h_verts, const float *h_normals
 // iterations of (blend, data, ARAP, assembly, PCG, update).  Everything
 // that changes between calls (pose, gates, depth) is read from device
 // memory, so the sequence is captured once and replayed.
-static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its) {
+// mode 0: depth target (optionally gated by a source depth); 1: mesh target
+static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its, int mode) {
   hipStream_t st = g->stream;
   const size_t P = (size_t)rows * cols;
   const int nn = g->nn, ne = nn * kNK;
   const PcgState S = g->pcg;
   const unsigned pg = (unsigned)S.nwg;
   const NrFrame *F = g->d_frame;
-  hipLaunchKernelGGL(k_nr_target_vert, dim3(grid_of(P)), dim3(256), 0, st, g->depth, F, g->tv);
-  hipLaunchKernelGGL(k_nr_target_norm, dim3(grid_of(P)), dim3(256), 0, st, g->tv, F, g->tn);
+  if (mode == 0) {
+    hipLaunchKernelGGL(k_nr_target_vert, dim3(grid_of(P)), dim3(256), 0, st, g->depth, F, g->tv);
+    hipLaunchKernelGGL(k_nr_target_norm, dim3(grid_of(P)), dim3(256), 0, st, g->tv, F, g->tn);
+  }
   for (int it = 0; it < gn; ++it) {
     hipLaunchKernelGGL(k_nr_blend, dim3(grid_of(g->nv)), dim3(256), 0, st, g->verts, g->normals, g->nv,
                        g->node_of_vertex, g->vnn, g->vnn_w, g->dq, g->pv, g->pn);
-    hipLaunchKernelGGL(k_nr_data, dim3(grid_of(g->nv)), dim3(256), 0, st, g->pv, g->pn, g->nv, g->tv, g->tn,
-                       g->depth, F, g->vdata);
+    if (mode == 0)
+      hipLaunchKernelGGL(k_nr_data, dim3(grid_of(g->nv)), dim3(256), 0, st, g->pv, g->pn, g->nv, g->tv, g->tn,
+                         g->depth, g->sdepth, F, g->vdata);
+    else
+      hipLaunchKernelGGL(k_nr_data_mesh, dim3(grid_of(g->nv)), dim3(256), 0, st, g->pv, g->pn, g->nv, g->mesh_v,
+                         g->mesh_n, F, g->vdata);
     hipLaunchKernelGGL(k_nr_arap, dim3(grid_of(ne)), dim3(256), 0, st, g->T, g->verts, g->node_vid, g->nbr,
                        g->nbr_w, nn, g->edata);
     hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 4)), dim3(256), 0, st, g->blk_a, g->blk_b, g->cstart,
@@ -1106,6 +1204,85 @@ static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its) {
     }
     hipLaunchKernelGGL(k_nr_update, dim3(grid_of(nn)), dim3(256), 0, st, S.x, g->T, nn, g->dq);
   }
+}
+
+static int nr_reserve(dm_graph *g, size_t P, int pcg_its) {
+  if (P > g->pixels_cap) {
+    DM_TRY(g->dalloc(&g->depth, P));
+    DM_TRY(g->dalloc(&g->sdepth, P));
+    DM_TRY(g->dalloc(&g->tv, P));
+    DM_TRY(g->dalloc(&g->tn, P));
+    g->pixels_cap = P;
+    g->key_gn = -1;  // buffers moved: capture again
+  }
+  if (pcg_its + 1 > g->pcg_cap) {
+    DM_TRY(g->dalloc(&g->pcg.rz, pcg_its + 1));
+    DM_TRY(g->dalloc(&g->pcg.al, pcg_its + 1));
+    g->pcg_cap = pcg_its + 1;
+    g->key_gn = -1;
+  }
+  return DM_OK;
+}
+
+// One solve call: mode 0 = depth target (src may gate it), 1 = mesh target.
+static int nr_solve(dm_graph *g, int mode, const float *depth, const float *src, const float *mv, const float *mn,
+                    const dm_sensor *sensor, const double pose[16], const dm_nonrigid_props *pr, double stats[4]) {
+  if (!g || !pose || !pr || (mode == 0 && (!sensor || sensor->rows == 0 || sensor->cols == 0)) ||
+      pr->gn_iterations < 0 || pr->pcg_iterations < 0 || pr->pcg_iterations > 1000) {
+    set_error("dm_nonrigid_solve: invalid argument");
+    return DM_E_INVALID;
+  }
+  hipStream_t st = g->stream;
+  const size_t P = mode == 0 ? (size_t)sensor->rows * sensor->cols : 0;
+  DM_TRY(nr_reserve(g, P, pr->pcg_iterations));
+  if (mode == 1 && !g->mesh_v) {
+    DM_TRY(g->dalloc(&g->mesh_v, (size_t)g->nv * 3));
+    DM_TRY(g->dalloc(&g->mesh_n, (size_t)g->nv * 3));
+  }
+  DM_HIP(hipStreamSynchronize(st));  // the pinned frame record may still be read by the last call
+  NrFrame &F = *g->h_frame;
+  F = NrFrame{};
+  if (mode == 0) {
+    F.rows = (int)sensor->rows; F.cols = (int)sensor->cols;
+    F.fx = sensor->fx; F.fy = sensor->fy; F.cx = sensor->cx; F.cy = sensor->cy;
+  }
+  for (int k = 0; k < 16; ++k) F.pose[k] = pose[k];
+  F.min_depth = pr->min_depth; F.max_depth = pr->max_depth; F.dist_max = pr->dist_max;
+  F.normal_cos = pr->normal_cos; F.lambda = pr->lambda_reg; F.damping = pr->damping; F.pcg_tol = pr->pcg_tol;
+  F.use_src = src != nullptr;
+  DM_HIP(hipMemcpyAsync(g->d_frame, g->h_frame, sizeof(NrFrame), hipMemcpyHostToDevice, st));
+  if (mode == 0) {
+    DM_HIP(hipMemcpyAsync(g->depth, depth, P * 4, hipMemcpyHostToDevice, st));
+    if (src) DM_HIP(hipMemcpyAsync(g->sdepth, src, P * 4, hipMemcpyHostToDevice, st));
+  } else {
+    DM_HIP(hipMemcpyAsync(g->mesh_v, mv, (size_t)g->nv * 12, hipMemcpyHostToDevice, st));
+    DM_HIP(hipMemcpyAsync(g->mesh_n, mn, (size_t)g->nv * 12, hipMemcpyHostToDevice, st));
+  }
+  if (!g->exec || g->key_gn != pr->gn_iterations || g->key_pcg != pr->pcg_iterations || g->key_rows != F.rows ||
+      g->key_cols != F.cols || g->key_mode != mode) {
+    if (g->exec) { (void)hipGraphExecDestroy(g->exec); g->exec = nullptr; }
+    hipGraph_t graph = nullptr;
+    DM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    nr_enqueue(g, F.rows, F.cols, pr->gn_iterations, pr->pcg_iterations, mode);
+    hipError_t ce = hipStreamEndCapture(st, &graph);
+    if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+    ce = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ce != hipSuccess) { g->exec = nullptr; return hip_fail(ce, "hipGraphInstantiate"); }
+    g->key_gn = pr->gn_iterations; g->key_pcg = pr->pcg_iterations;
+    g->key_rows = F.rows; g->key_cols = F.cols; g->key_mode = mode;
+  }
+  DM_HIP(hipGraphLaunch(g->exec, st));
+  const PcgState S = g->pcg;
+  double hs[3] = {0, 0, 0};
+  int flags[2] = {0, 0};
+  if (pr->gn_iterations > 0) {
+    DM_HIP(hipMemcpyAsync(hs, g->stats, 24, hipMemcpyDeviceToHost, st));
+    DM_HIP(hipMemcpyAsync(flags, S.flag, 8, hipMemcpyDeviceToHost, st));
+  }
+  DM_HIP(hipStreamSynchronize(st));
+  if (stats) { stats[0] = hs[0]; stats[1] = hs[1]; stats[2] = hs[2]; stats[3] = flags[1]; }
+  return DM_OK;
 }
 
 extern "C" {
@@ -1203,62 +1380,67 @@ int dm_blend(dm_graph *g, float *out_vertices, float *out_normals) {
   return DM_OK;
 }
 
-int dm_nonrigid_solve(dm_graph *g, const float *depth, const dm_sensor *sensor, const double pose[16],
-                      const dm_nonrigid_props *pr, double stats[4]) {
-  if (!g || !depth || !sensor || !pose || !pr || sensor->rows == 0 || sensor->cols == 0 || pr->gn_iterations < 0 ||
-      pr->pcg_iterations < 0 || pr->pcg_iterations > 1000) {
-    set_error("dm_nonrigid_solve: invalid argument");
+int dm_nonrigid_reserve(dm_graph *g, int64_t observations, int32_t pcg_iterations) {
+  if (!g || pcg_iterations < 0 || pcg_iterations > 1000 || observations < 0 || observations > (1LL << 28)) {
+    set_error("dm_nonrigid_reserve: invalid argument");
     return DM_E_INVALID;
   }
-  hipStream_t st = g->stream;
-  const size_t P = (size_t)sensor->rows * sensor->cols;
-  if (P > g->pixels_cap) {
-    DM_TRY(g->dalloc(&g->depth, P));
-    DM_TRY(g->dalloc(&g->tv, P));
-    DM_TRY(g->dalloc(&g->tn, P));
-    g->pixels_cap = P;
-    g->key_gn = -1;  // buffers moved: capture again
+  DM_TRY(nr_reserve(g, (size_t)observations, pcg_iterations));
+  DM_HIP(hipStreamSynchronize(g->stream));
+  return DM_OK;
+}
+
+int dm_nonrigid_solve(dm_graph *g, const float *depth, const dm_sensor *sensor, const double pose[16],
+                      const dm_nonrigid_props *pr, double stats[4]) {
+  if (!depth) { set_error("dm_nonrigid_solve: invalid argument"); return DM_E_INVALID; }
+  return nr_solve(g, 0, depth, nullptr, nullptr, nullptr, sensor, pose, pr, stats);
+}
+
+int dm_nonrigid_solve_pair(dm_graph *g, const float *target_depth, const float *source_depth,
+                           const dm_sensor *sensor, const double pose[16], const dm_nonrigid_props *pr,
+                           double stats[4]) {
+  if (!target_depth || !source_depth) { set_error("dm_nonrigid_solve_pair: invalid argument"); return DM_E_INVALID; }
+  return nr_solve(g, 0, target_depth, source_depth, nullptr, nullptr, sensor, pose, pr, stats);
+}
+
+int dm_nonrigid_solve_mesh(dm_graph *g, const float *target_vertices, const float *target_normals,
+                           const double pose[16], const dm_nonrigid_props *pr, double stats[4]) {
+  if (!target_vertices || !target_normals) {
+    set_error("dm_nonrigid_solve_mesh: invalid argument");
+    return DM_E_INVALID;
   }
-  if (pr->pcg_iterations + 1 > g->pcg_cap) {
-    DM_TRY(g->dalloc(&g->pcg.rz, pr->pcg_iterations + 1));
-    DM_TRY(g->dalloc(&g->pcg.al, pr->pcg_iterations + 1));
-    g->pcg_cap = pr->pcg_iterations + 1;
-    g->key_gn = -1;
+  return nr_solve(g, 1, nullptr, nullptr, target_vertices, target_normals, nullptr, pose, pr, stats);
+}
+
+int dm_graph_create_nodes(const float *vertices, const float *normals, int64_t nv, const int32_t *node_vertex_ids,
+                          int32_t num_nodes, float dgw, dm_graph **out) {
+  if (!vertices || !normals || !node_vertex_ids || !out || nv <= 0 || nv >= (1LL << 27) || !(dgw > 0.f) ||
+      num_nodes < kNK + 1 || num_nodes > kMaxNodes || num_nodes > nv) {
+    set_error("dm_graph_create_nodes: invalid argument (5 <= nodes <= min(32768, vertices), dgw > 0)");
+    return DM_E_INVALID;
   }
-  DM_HIP(hipStreamSynchronize(st));  // the pinned frame record may still be read by the last call
-  NrFrame &F = *g->h_frame;
-  F = NrFrame{};
-  F.rows = (int)sensor->rows; F.cols = (int)sensor->cols;
-  F.fx = sensor->fx; F.fy = sensor->fy; F.cx = sensor->cx; F.cy = sensor->cy;
-  for (int k = 0; k < 16; ++k) F.pose[k] = pose[k];
-  F.min_depth = pr->min_depth; F.max_depth = pr->max_depth; F.dist_max = pr->dist_max;
-  F.normal_cos = pr->normal_cos; F.lambda = pr->lambda_reg; F.damping = pr->damping; F.pcg_tol = pr->pcg_tol;
-  DM_HIP(hipMemcpyAsync(g->d_frame, g->h_frame, sizeof(NrFrame), hipMemcpyHostToDevice, st));
-  DM_HIP(hipMemcpyAsync(g->depth, depth, P * 4, hipMemcpyHostToDevice, st));
-  if (!g->exec || g->key_gn != pr->gn_iterations || g->key_pcg != pr->pcg_iterations || g->key_rows != F.rows ||
-      g->key_cols != F.cols) {
-    if (g->exec) { (void)hipGraphExecDestroy(g->exec); g->exec = nullptr; }
-    hipGraph_t graph = nullptr;
-    DM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    nr_enqueue(g, F.rows, F.cols, pr->gn_iterations, pr->pcg_iterations);
-    hipError_t ce = hipStreamEndCapture(st, &graph);
-    if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
-    ce = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ce != hipSuccess) { g->exec = nullptr; return hip_fail(ce, "hipGraphInstantiate"); }
-    g->key_gn = pr->gn_iterations; g->key_pcg = pr->pcg_iterations;
-    g->key_rows = F.rows; g->key_cols = F.cols;
+  dm_graph *g = new dm_graph();
+  g->nv = nv;
+  g->props = dm_graph_props{0.f, dgw, num_nodes};
+  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete g;
+    return hip_fail(e, "hipStreamCreate");
   }
-  DM_HIP(hipGraphLaunch(g->exec, st));
-  const PcgState S = g->pcg;
-  double hs[3] = {0, 0, 0};
-  int flags[2] = {0, 0};
-  if (pr->gn_iterations > 0) {
-    DM_HIP(hipMemcpyAsync(hs, g->stats, 24, hipMemcpyDeviceToHost, st));
-    DM_HIP(hipMemcpyAsync(flags, S.flag, 8, hipMemcpyDeviceToHost, st));
+  int rc = graph_build_nodes(g, vertices, normals, node_vertex_ids, num_nodes);
+  if (rc != DM_OK) {
+    g->release();
+    delete g;
+    return rc;
   }
-  DM_HIP(hipStreamSynchronize(st));
-  if (stats) { stats[0] = hs[0]; stats[1] = hs[1]; stats[2] = hs[2]; stats[3] = flags[1]; }
+  *out = g;
+  return DM_OK;
+}
+
+int dm_graph_update_nodes(dm_graph *g) {
+  if (!g) { set_error("null argument"); return DM_E_INVALID; }
+  DM_TRY(graph_links(g));
+  DM_HIP(hipStreamSynchronize(g->stream));
   return DM_OK;
 }
 

@@ -247,14 +247,22 @@ class DefGraph:
     vertex with weights exp(-d^2/dgw^2), node transforms (canonical ->
     current).  `init` = defGraph::init + defGraphUpdateNodes."""
 
-    def __init__(self, vertices, normals, node_radius=0.025, dgw=0.05, max_nodes=4096):
+    def __init__(self, vertices, normals, node_radius=0.025, dgw=0.05, max_nodes=4096, node_ids=None):
+        """Nodes sampled per grid cell of edge node_radius, or, with
+        node_ids, node k = vertex node_ids[k] (defGraph::init's own choice of
+        the first or random vertices, defGraph.cu:673-702)."""
         self._lib = N.lib()
         self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         self.normals = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
-        pr = N.dm_graph_props(node_radius=node_radius, dgw=dgw, max_nodes=max_nodes)
         h = C.c_void_p()
-        N.check(self._lib.dm_graph_create(N.ptr(self.vertices), N.ptr(self.normals), len(self.vertices),
-                                          C.byref(pr), C.byref(h)))
+        if node_ids is None:
+            pr = N.dm_graph_props(node_radius=node_radius, dgw=dgw, max_nodes=max_nodes)
+            N.check(self._lib.dm_graph_create(N.ptr(self.vertices), N.ptr(self.normals), len(self.vertices),
+                                              C.byref(pr), C.byref(h)))
+        else:
+            ids = np.ascontiguousarray(node_ids, np.int32)
+            N.check(self._lib.dm_graph_create_nodes(N.ptr(self.vertices), N.ptr(self.normals), len(self.vertices),
+                                                    N.ptr(ids), len(ids), float(dgw), C.byref(h)))
         self._h = h
         n = C.c_int32()
         N.check(self._lib.dm_graph_num_nodes(h, C.byref(n)))
@@ -289,6 +297,10 @@ class DefGraph:
         n = C.c_int32()
         N.check(self._lib.dm_graph_num_blocks(self._h, C.byref(n)))
         return n.value
+
+    # defGraph::defGraphUpdateNodes (defGraph.cu:483-545)
+    def defGraphUpdateNodes(self):
+        N.check(self._lib.dm_graph_update_nodes(self._h))
 
     # defGraph::updateNodesDQ (defGraph.cu:704-730)
     def set_transforms(self, T):
@@ -331,17 +343,53 @@ class NonRigidICP:
     def init(self, graph=None, **props):
         self.props.update(props)
 
-    def solve(self, graph, depth, sensor, pose, **override):
+    def reserve(self, graph, observations):
+        """nonRigidICP::init's sizing (nonRigidICP.cu:18-40): frame buffers for
+        `observations` pixels (rows * cols), so later solves of that size
+        allocate no frame buffers."""
+        N.check(graph._lib.dm_nonrigid_reserve(graph._h, int(observations), int(self.props["pcg_iterations"])))
+
+    def _props(self, override):
         p = dict(self.props)
         p.update(override)
-        pr = N.dm_nonrigid_props(**p)
+        return N.dm_nonrigid_props(**p)
+
+    @staticmethod
+    def _stats(st):
+        return dict(residuals=int(st[0]), data_energy=float(st[1]), arap_energy=float(st[2]),
+                    pcg_iterations=int(st[3]))
+
+    def solve(self, graph, depth, sensor, pose, source_depth=None, **override):
+        """Against one depth frame seen from pose; with source_depth (the
+        predicted image of the same view) a pixel takes part only where both
+        depths lie in range (nonRigidICP.cu:291-292)."""
+        pr = self._props(override)
         depth = _f32(depth)
         pose = np.ascontiguousarray(pose, np.float64)
         st = np.zeros(4, np.float64)
-        N.check(graph._lib.dm_nonrigid_solve(graph._h, N.ptr(depth), C.byref(_sensor(sensor)), N.ptr(pose),
-                                            C.byref(pr), N.ptr(st)))
-        return dict(residuals=int(st[0]), data_energy=float(st[1]), arap_energy=float(st[2]),
-                    pcg_iterations=int(st[3]))
+        if source_depth is None:
+            N.check(graph._lib.dm_nonrigid_solve(graph._h, N.ptr(depth), C.byref(_sensor(sensor)), N.ptr(pose),
+                                                C.byref(pr), N.ptr(st)))
+        else:
+            src = _f32(source_depth)
+            N.check(graph._lib.dm_nonrigid_solve_pair(graph._h, N.ptr(depth), N.ptr(src), C.byref(_sensor(sensor)),
+                                                     N.ptr(pose), C.byref(pr), N.ptr(st)))
+        return self._stats(st)
+
+    def solve_mesh(self, graph, target_vertices, target_normals, pose, **override):
+        """nonRigidICP::solve(warpField, blender, targetMesh, pose)
+        (nonRigidICP.cu:706-852): target vertex v corresponds to canonical
+        vertex v; positions and normals in the camera frame at pose."""
+        pr = self._props(override)
+        mv = np.ascontiguousarray(target_vertices, np.float32).reshape(-1, 3)
+        mn = np.ascontiguousarray(target_normals, np.float32).reshape(-1, 3)
+        if len(mv) != len(graph.vertices) or len(mn) != len(mv):
+            raise ValueError("the target mesh needs one vertex and normal per canonical vertex")
+        pose = np.ascontiguousarray(pose, np.float64)
+        st = np.zeros(4, np.float64)
+        N.check(graph._lib.dm_nonrigid_solve_mesh(graph._h, N.ptr(mv), N.ptr(mn), N.ptr(pose), C.byref(pr),
+                                                 N.ptr(st)))
+        return self._stats(st)
 
 
 
@@ -419,6 +467,32 @@ class Session:
         p = np.zeros((4, 4), np.float64)
         N.check(self._lib.dm_session_get_pose(self._h, N.ptr(p)))
         return p
+
+    def icp_increment(self):
+        """gnSolver::solve's summed twist (omega, t) of the last tracking."""
+        xi = np.zeros(6, np.float64)
+        N.check(self._lib.dm_session_icp_increment(self._h, N.ptr(xi)))
+        return xi
+
+    def writePoseToTxt(self, filename, timestamp, pose=None):
+        """builder::writePoseToTxt (builder.cu:243-262): append one TUM line
+        "t tx ty tz qx qy qz qw" for the current (or the given) pose."""
+        from .io import write_trajectory
+        write_trajectory(filename, [timestamp], [self.pose() if pose is None else pose], append=True)
+
+    def run_dataset(self, dataset, nframes=None, trajectory=None, solve_on_host=False):
+        """builder::buildModelLS over the frames of a TUM RGB-D sequence
+        (io.TumDataset, the reference's dataSet::invokeNextFrame,
+        utils_Dataset.cpp:223-237), appending each frame's pose to the TUM
+        trajectory file `trajectory` when given.  Returns the poses (n,4,4)."""
+        n = len(dataset) if nframes is None else min(int(nframes), len(dataset))
+        poses = np.zeros((n, 4, 4), np.float64)
+        for k in range(n):
+            depth, rgb, _, stamp = dataset.frame(k)
+            poses[k] = self.process_frame(depth, rgb, solve_on_host)
+            if trajectory is not None:
+                self.writePoseToTxt(trajectory, stamp, poses[k])
+        return poses
 
     def source_maps(self, depth, rgb, level):
         r, c = self.sensor["rows"] >> level, self.sensor["cols"] >> level

@@ -20,6 +20,7 @@
 #pragma once
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <fstream>
 #include <stdexcept>
 #include <string>
@@ -57,12 +58,55 @@ struct float4x4 {
 struct float3 {
   float x, y, z;
 };
+struct uchar3 {
+  uint8_t x, y, z;
+};
+inline uchar3 make_uchar3(uint8_t x, uint8_t y, uint8_t z) { return uchar3{x, y, z}; }
+
+// Eigen::Matrix4f / Eigen::Matrix<double, 6, 1> of the reference's solver
+// signatures (gnSolver.h:72-76, nonRigidICP.h:112-122) as PODs: a row-major
+// camera-to-world pose and a twist (omega, t)
+using Matrix4d = std::array<double, 16>;
+using Vector6d = std::array<double, 6>;
+inline Matrix4d Identity4() { return Matrix4d{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}; }
 
 // rgbdImage (include/image/sensor/rgbdImage.h) over host buffers
 struct rgbdImage {
   const float *depth = nullptr;   // rows*cols metres
   const uint8_t *rgb = nullptr;   // rows*cols*3, may be null
   rgbdSensor sensor{};
+};
+
+// pyramid (include/image/pyramid/pyramid.h): a frame the caller fills in
+// place after init, as the reference's test programs do
+// (liveFrame.depth[i] = ..., liveFrame.rgb[i] = make_uchar3(...))
+class pyramid {
+ public:
+  rgbdSensor sensor{};
+  float *depth = nullptr;
+  uchar3 *rgb = nullptr;
+  pyramid() = default;
+  pyramid(const pyramid &o) { *this = o; }
+  pyramid &operator=(const pyramid &o) {
+    sensor = o.sensor;
+    depth_ = o.depth_;
+    rgb_ = o.rgb_;
+    depth = depth_.empty() ? nullptr : depth_.data();
+    rgb = rgb_.empty() ? nullptr : rgb_.data();
+    return *this;
+  }
+  void init(const rgbdSensor &s) {
+    sensor = s;
+    depth_.assign((size_t)s.rows * s.cols, 0.f);
+    rgb_.assign((size_t)s.rows * s.cols, uchar3{0, 0, 0});
+    depth = depth_.data();
+    rgb = rgb_.data();
+  }
+  rgbdImage image() const { return rgbdImage{depth, rgb ? &rgb->x : nullptr, sensor}; }
+
+ private:
+  std::vector<float> depth_;
+  std::vector<uchar3> rgb_;
 };
 
 namespace geometry {
@@ -74,6 +118,11 @@ struct Vertex {
 };
 struct Triangle {
   Vertex v0, v1, v2;
+};
+
+// PointCloudXYZ (geom_Types.h): a vertex map and its normals
+struct PointCloudXYZ {
+  std::vector<float3> points, normals;
 };
 
 // Mesh (mesh.h:25-66) as a host triangle soup.  edge_ids[3 * i + j] names
@@ -252,29 +301,61 @@ inline geometry::Mesh tsdfVolume::ExtractMesh() {
 
 namespace solver {
 
-// solver::Properties (gnSolver.h:47-55); levels 0 = full resolution
+// solver::Properties (gnSolver.h:47-55).  lvlIterationNum[p] iterations run
+// at the pyramid level of scale lvlScale[p] (1 = full resolution, 2 = half,
+// 4 = quarter; any order of the three, as the reference's src/dynaMap.cpp:
+// 150-155 gives {4, 2, 1}); levels run coarse to fine.  maxIteration and
+// huberConstant are kept for source compatibility: the reference computes
+// but never applies the robust weights (gnSolver.cu:116-119).
 struct Properties {
   int lvlIterationNum[3] = {10, 5, 4};
+  int lvlScale[3] = {1, 2, 4};
+  int maxIteration = 20;
+  float minIncrement = 0.001f;  // src/dynaMap.cpp:156
+  float maxIncrement = 0.3f;
+  float regularization = 0.02f;
+  float huberConstant = 0.3f;
+  // the build's gates and weights (the reference hard-codes DEPTH_MIN/MAX,
+  // pyramid.h:15-16, and sets the photometric weight to 0, gnSolver.cu:180)
   float minDepth = 0.3f, maxDepth = 5.0f;
   float distMax = 0.1f;
   float photoWeight = 0.0f;
-  float regularization = 0.02f;
-  float maxIncrement = 0.3f;
-  float minIncrement = 0.001f;  // src/dynaMap.cpp:156
   dm_icp_props c() const {
     dm_icp_props p{};
-    for (int l = 0; l < 3; ++l) p.lvl_iterations[l] = lvlIterationNum[l];
+    bool seen[3] = {false, false, false};
+    for (int k = 0; k < 3; ++k) {
+      const int l = lvlScale[k] == 1 ? 0 : lvlScale[k] == 2 ? 1 : lvlScale[k] == 4 ? 2 : -1;
+      if (l < 0 || seen[l]) throw Error(DM_E_INVALID, "dynamap: lvlScale must hold 1, 2 and 4");
+      seen[l] = true;
+      p.lvl_iterations[l] = lvlIterationNum[k];
+    }
     p.min_depth = minDepth; p.max_depth = maxDepth; p.dist_max = distMax;
     p.photo_weight = photoWeight; p.regularization = regularization; p.max_increment = maxIncrement;
     p.min_increment = minIncrement;
     return p;
   }
+  // iterations at full resolution (the level the non-rigid solve runs on,
+  // nonRigidICP.cu:580-583)
+  int fullResolutionIterations() const {
+    for (int k = 0; k < 3; ++k)
+      if (lvlScale[k] == 1) return lvlIterationNum[k];
+    return 0;
+  }
 };
 
+inline Matrix4d mul4(const Matrix4d &A, const Matrix4d &B) {
+  Matrix4d C{};
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      double s = 0.0;
+      for (int k = 0; k < 4; ++k) s += A[4 * r + k] * B[4 * k + c];
+      C[4 * r + c] = s;
+    }
+  return C;
+}
+
 // gnSolver (gnSolver.h:57-100): multi-level projective ICP of a source frame
-// against target maps, on the GPU.  The target is given as a depth frame
-// (its vertex/normal maps become the model) or left as the last raycast of
-// a session.
+// against target maps, on the GPU.
 class gnSolver {
  public:
   gnSolver() = default;
@@ -283,37 +364,77 @@ class gnSolver {
   ~gnSolver() {
     if (s_) dm_session_destroy(s_);
   }
-  // gnSolver::init (gnSolver.cu:287-315)
+  // gnSolver::init (gnSolver.cu:287-315).  The tracker needs no voxel
+  // volume: it is created with a minimal one.
+  void init(const Properties &prop, const rgbdSensor &sensor) {
+    const tsdf::tsdfProperties tp{0.01f, 1 << 10, 10, 1 << 10, 8, 64, 0.03f, prop.maxDepth, prop.minDepth};
+    init(prop, sensor, tp);
+  }
   void init(const Properties &prop, const rgbdSensor &sensor, const tsdf::tsdfProperties &tp) {
+    if (s_) dm_session_destroy(s_);
+    s_ = nullptr;
     const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
     dm_icp_props ip = prop.c();
     check(dm_session_create(&tp, &ip, &sensor, I, &s_));
     rows_ = sensor.rows;
     cols_ = sensor.cols;
   }
-  // gnSolver::solve (gnSolver.cu:317-438): increment source camera -> target
-  // camera; solve_on_host keeps the reference's host 6x6 steps
-  std::array<double, 16> solve(const rgbdImage &target, const rgbdImage &source, bool solve_on_host = false) {
+  // gnSolver::solve (gnSolver.cu:317-438), the reference's call shape
+  // (builder.cu:94-99): registers sourceImage to targetImage, updates pose
+  // with the increment (pose <- pose * T_inc, camera-to-world; T_inc maps
+  // source-camera points into the target camera) and returns the summed
+  // twist in prev_increment_.  targetCloud / sourceCloud receive the
+  // full-resolution vertex and normal maps (camera coordinates; NaN normal =
+  // none), as the reference's getPointCloudXYZ / getNormalsfromVertices fill
+  // them.
+  void solve(Matrix4d &pose, Vector6d &prev_increment_, const rgbdImage &targetImage, const rgbdImage &sourceImage,
+             geometry::PointCloudXYZ &targetCloud, geometry::PointCloudXYZ &sourceCloud) {
+    const Matrix4d Tinc = track(targetImage, sourceImage, &targetCloud, &sourceCloud);
+    check(dm_session_icp_increment(s_, prev_increment_.data()));
+    pose = mul4(pose, Tinc);
+  }
+  void solve(Matrix4d &pose, Vector6d &prev_increment_, pyramid &targetImage, pyramid &sourceImage,
+             geometry::PointCloudXYZ &targetCloud, geometry::PointCloudXYZ &sourceCloud) {
+    solve(pose, prev_increment_, targetImage.image(), sourceImage.image(), targetCloud, sourceCloud);
+  }
+  // the increment alone (source camera -> target camera); solve_on_host
+  // keeps the reference's host 6x6 steps
+  Matrix4d solve(const rgbdImage &target, const rgbdImage &source, bool solve_on_host = false) {
+    return track(target, source, nullptr, nullptr, solve_on_host);
+  }
+
+ private:
+  Matrix4d track(const rgbdImage &target, const rgbdImage &source, geometry::PointCloudXYZ *tc,
+                 geometry::PointCloudXYZ *sc, bool solve_on_host = false) {
     for (int l = 0; l < 3; ++l) {
       const size_t P = (size_t)(rows_ >> l) * (cols_ >> l);
       std::vector<float> v(P * 3), n(P * 3), in(P);
       check(dm_session_source_maps(s_, target.depth, target.rgb, l, v.data(), n.data(), in.data()));
       check(dm_session_set_model_maps(s_, l, v.data(), n.data(), in.data()));
+      if (l == 0 && tc) to_cloud(v, n, *tc);
     }
-    std::vector<float> tmp((size_t)rows_ * cols_ * 3);
-    check(dm_session_source_maps(s_, source.depth, source.rgb, 0, tmp.data(), nullptr, nullptr));
-    std::array<double, 16> T{};
+    const size_t P = (size_t)rows_ * cols_;
+    std::vector<float> v(P * 3), n(sc ? P * 3 : 0);
+    check(dm_session_source_maps(s_, source.depth, source.rgb, 0, v.data(), sc ? n.data() : nullptr, nullptr));
+    if (sc) to_cloud(v, n, *sc);
+    Matrix4d T{};
     check(dm_session_icp_track(s_, solve_on_host ? 1 : 0, T.data()));
     return T;
   }
-
- private:
+  static void to_cloud(const std::vector<float> &v, const std::vector<float> &n, geometry::PointCloudXYZ &c) {
+    c.points.resize(v.size() / 3);
+    c.normals.resize(v.size() / 3);
+    for (size_t i = 0; i < c.points.size(); ++i) {
+      c.points[i] = float3{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+      c.normals[i] = float3{n[3 * i], n[3 * i + 1], n[3 * i + 2]};
+    }
+  }
   dm_session *s_ = nullptr;
   uint32_t rows_ = 0, cols_ = 0;
 };
 
-// nonRigidICP properties (the build's formulation; the reference's
-// solver::Properties carries the iteration counts and regularisation)
+// nonRigidICP properties of the build's formulation (Gauss-Newton
+// iterations, PCG, ARAP weight, association gates)
 struct NonRigidProperties {
   int gnIterations = 5, pcgIterations = 40;
   double pcgTol = 1e-6, lambdaReg = 10.0, damping = 1e-3, distMax = 0.05, normalCos = 0.5;
@@ -333,41 +454,128 @@ struct MeshSTD {
   std::vector<float3> vertices, normals;
 };
 
-// defGraph (defGraph.h:45-87): nodes sampled from the canonical mesh, their
+// defGraph (defGraph.h:45-87): nodes of the canonical mesh, their
 // neighbours and weights, node transforms; on the GPU.
 class defGraph {
  public:
+  static constexpr int NODE_NUM = 273;  // include/DynaMap.h:22
+  static constexpr int KNN = 4;         // include/DynaMap.h:18
   defGraph() = default;
   defGraph(const defGraph &) = delete;
   defGraph &operator=(const defGraph &) = delete;
   ~defGraph() { Free(); }
-  // defGraph::init + defGraphUpdateNodes (defGraph.cu:15-69, :483-545)
-  void init(const MeshSTD &srcMesh, float nodeRadius = 0.025f, float dgw = 0.05f, int maxNodes = 4096) {
+  // defGraph::init(MeshSTD&, int ObservationNum, int mode) (defGraph.cu:15-69)
+  // with sampleDownMeshHost (:673-702): min(nodes, ObservationNum) nodes
+  // (nodes = NODE_NUM unless given), mode 1 the first mesh vertices in
+  // order, mode 0 distinct random vertices below ObservationNum (seeded by
+  // `seed`; the reference seeds rand() with the time).  Weights
+  // exp(-d^2/dgw^2) with dgw = weightScale (include/DynaMap.h:24: 0.015).
+  void init(const MeshSTD &srcMesh, int ObservationNum, int mode = 0, int nodes = NODE_NUM) {
+    Free();
+    const int nv = (int)srcMesh.vertices.size();
+    const int range = std::min(ObservationNum, nv);
+    const int n = std::min(nodes, range);
+    std::vector<int32_t> ids((size_t)std::max(n, 0));
+    if (mode == 1) {
+      for (int k = 0; k < n; ++k) ids[k] = k;
+    } else {
+      // a partial Fisher-Yates shuffle of [0, range) driven by splitmix64
+      std::vector<int32_t> pool((size_t)std::max(range, 0));
+      for (int k = 0; k < range; ++k) pool[k] = k;
+      uint64_t x = seed;
+      for (int k = 0; k < n; ++k) {
+        x += 0x9E3779B97F4A7C15ull;
+        uint64_t z = x;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const int j = k + (int)(z % (uint64_t)(range - k));
+        std::swap(pool[k], pool[j]);
+        ids[k] = pool[k];
+      }
+    }
+    check(dm_graph_create_nodes(&srcMesh.vertices[0].x, &srcMesh.normals[0].x, (int64_t)nv, ids.data(), n,
+                                weightScale, &g_));
+    finish(srcMesh);
+  }
+  // the build's own sampling: per grid cell of edge nodeRadius the vertex
+  // closest to the cell centre
+  void init(const MeshSTD &srcMesh, float nodeRadius, float dgw, int maxNodes = 4096) {
     Free();
     dm_graph_props p{nodeRadius, dgw, maxNodes};
     check(dm_graph_create(&srcMesh.vertices[0].x, &srcMesh.normals[0].x, (int64_t)srcMesh.vertices.size(), &p, &g_));
-    int32_t n = 0;
-    check(dm_graph_num_nodes(g_, &n));
-    nodeNum = n;
-    verticesNum = srcMesh.vertices.size();
+    finish(srcMesh);
   }
-  void defGraphUpdateNodes() {}  // neighbours and weights are built by init
+  // defGraph::defGraphUpdateNodes (defGraph.cu:483-545): node neighbours and
+  // weights (and the vertices' nearest nodes) from the canonical positions
+  void defGraphUpdateNodes() { check(dm_graph_update_nodes(g_)); }
+  // defGraph::updateActiveNeighbourNodes (defGraph.cu:243-302, :340-395):
+  // the nearest nodes of the observations.  Observations are the canonical
+  // vertices (a target mesh corresponds vertex for vertex; a frame is
+  // associated through the warped canonical mesh), so this recomputes the
+  // vertices' neighbours.
+  void updateActiveNeighbourNodes(const MeshSTD &targetMesh, const Matrix4d &) {
+    if (targetMesh.vertices.size() != verticesNum)
+      throw Error(DM_E_INVALID, "dynamap: the target mesh needs the canonical mesh's vertex count");
+    defGraphUpdateNodes();
+  }
+  void updateActiveNeighbourNodes(const pyramid &, const rgbdSensor &, const Matrix4d &) { defGraphUpdateNodes(); }
   void Free() {
     if (g_) dm_graph_destroy(g_);
     g_ = nullptr;
   }
-  // defGraph::updateNodesDQ (defGraph.cu:704-730): node transforms, nodeNum x 16
+  // defGraph::updateNodesDQ (defGraph.cu:704-730): per node 6 floats (roll,
+  // pitch, yaw, tx, ty, tz); the rotation of Quaternion(EulerAngles)
+  // (Quaternion.h:83-96: q = q_z(yaw) q_y(pitch) q_x(roll)), the translation t
+  void updateNodesDQ(const float *dqVector) {
+    std::vector<double> T((size_t)nodeNum * 16);
+    for (int i = 0; i < nodeNum; ++i) {
+      const double roll = dqVector[6 * i], pitch = dqVector[6 * i + 1], yaw = dqVector[6 * i + 2];
+      const double cr = std::cos(roll * 0.5), sr = std::sin(roll * 0.5), cp = std::cos(pitch * 0.5),
+                   sp = std::sin(pitch * 0.5), cy = std::cos(yaw * 0.5), sy = std::sin(yaw * 0.5);
+      const double w = cy * cp * cr + sy * sp * sr, x = cy * cp * sr - sy * sp * cr, y = sy * cp * sr + cy * sp * cr,
+                   z = sy * cp * cr - cy * sp * sr;
+      double *M = &T[16 * (size_t)i];
+      M[0] = 1 - 2 * (y * y + z * z); M[1] = 2 * (x * y - w * z);     M[2] = 2 * (x * z + w * y);
+      M[4] = 2 * (x * y + w * z);     M[5] = 1 - 2 * (x * x + z * z); M[6] = 2 * (y * z - w * x);
+      M[8] = 2 * (x * z - w * y);     M[9] = 2 * (y * z + w * x);     M[10] = 1 - 2 * (x * x + y * y);
+      M[3] = dqVector[6 * i + 3]; M[7] = dqVector[6 * i + 4]; M[11] = dqVector[6 * i + 5];
+      M[12] = M[13] = M[14] = 0.0; M[15] = 1.0;
+    }
+    updateNodesDQ(T);
+  }
+  // node transforms directly, nodeNum x 16 row-major
   void updateNodesDQ(const std::vector<double> &T) { check(dm_graph_set_transforms(g_, T.data())); }
   std::vector<double> nodeTransforms() {
     std::vector<double> T((size_t)nodeNum * 16);
     check(dm_graph_get_transforms(g_, T.data()));
     return T;
   }
+  // defGraphNode::vertexID and nIds (defGraph.h:35-43)
+  std::vector<int32_t> nodeVertexIds() {
+    std::vector<int32_t> v((size_t)nodeNum);
+    check(dm_graph_get_nodes(g_, v.data(), nullptr, nullptr));
+    return v;
+  }
+  std::vector<int32_t> nodeNeighbours() {
+    std::vector<int32_t> v((size_t)nodeNum * KNN);
+    check(dm_graph_get_nodes(g_, nullptr, v.data(), nullptr));
+    return v;
+  }
   dm_graph *handle() { return g_; }
   int nodeNum = 0;
+  int nNum = KNN;
   size_t verticesNum = 0;
+  float weightScale = 0.015f;
+  uint64_t seed = 0;
 
  private:
+  void finish(const MeshSTD &m) {
+    int32_t n = 0;
+    check(dm_graph_num_nodes(g_, &n));
+    nodeNum = n;
+    verticesNum = m.vertices.size();
+  }
   dm_graph *g_ = nullptr;
 };
 
@@ -385,7 +593,10 @@ class dqBlender {
     dst.normals.resize(graph.verticesNum);
     check(dm_blend(graph.handle(), &dst.vertices[0].x, &dst.normals[0].x));
   }
-  void blendMesh(geometry::MeshSTD &dst) { blendMesh(dst, *graph_); }
+  void blendMesh(geometry::MeshSTD &dst) {
+    if (!graph_) throw Error(DM_E_STATE, "dynamap: dqBlender::init was not called");
+    blendMesh(dst, *graph_);
+  }
 
  private:
   geometry::defGraph *graph_ = nullptr;
@@ -398,9 +609,45 @@ namespace solver {
 // nonRigidICP (nonRigidICP.h:59-183)
 class nonRigidICP {
  public:
-  void init(geometry::defGraph &, const NonRigidProperties &prop, int = 0) { prop_ = prop; }
-  // nonRigidICP::solve (nonRigidICP.cu:557-704): Gauss-Newton on the node
-  // transforms against one depth frame seen from pose (camera-to-world)
+  // nonRigidICP::init(warpField, Prop, problemSize) (nonRigidICP.cu:18-40):
+  // the Gauss-Newton iterations are those of the full-resolution level
+  // (the level the reference's solve runs), the frame buffers are sized for
+  // problemSize observations
+  void init(geometry::defGraph &warpField, const Properties &prop, int problemSize) {
+    NonRigidProperties p = prop_;
+    p.gnIterations = prop.fullResolutionIterations();
+    init(warpField, p, problemSize);
+  }
+  void init(geometry::defGraph &warpField, const NonRigidProperties &prop, int problemSize = 0) {
+    prop_ = prop;
+    check(dm_nonrigid_reserve(warpField.handle(), problemSize, prop.pcgIterations));
+  }
+  // nonRigidICP::solve(warpField, blender, targetImage, sourceImage, pose,
+  // sensor) (nonRigidICP.cu:557-704): the warped canonical mesh against the
+  // target (live) frame seen from pose (camera-to-world); a pixel takes part
+  // only where the source (predicted) image's depth is in range too
+  std::array<double, 4> solve(geometry::defGraph &warpField, blender::dqBlender &, pyramid &targetImage,
+                              pyramid &sourceImage, Matrix4d pose, rgbdSensor sensor) {
+    std::array<double, 4> st{};
+    dm_nonrigid_props p = prop_.c();
+    check(dm_nonrigid_solve_pair(warpField.handle(), targetImage.depth, sourceImage.depth, &sensor, pose.data(), &p,
+                                 st.data()));
+    return st;
+  }
+  // nonRigidICP::solve(warpField, blender, targetMesh, pose)
+  // (nonRigidICP.cu:706-852): target vertex v (camera coordinates) matches
+  // canonical vertex v
+  std::array<double, 4> solve(geometry::defGraph &warpField, blender::dqBlender &, const geometry::MeshSTD &targetMesh,
+                              Matrix4d pose) {
+    if (targetMesh.vertices.size() != warpField.verticesNum || targetMesh.normals.size() != warpField.verticesNum)
+      throw Error(DM_E_INVALID, "dynamap: the target mesh needs the canonical mesh's vertex count");
+    std::array<double, 4> st{};
+    dm_nonrigid_props p = prop_.c();
+    check(dm_nonrigid_solve_mesh(warpField.handle(), &targetMesh.vertices[0].x, &targetMesh.normals[0].x, pose.data(),
+                                 &p, st.data()));
+    return st;
+  }
+  // against one depth frame (no source image)
   std::array<double, 4> solve(geometry::defGraph &graph, blender::dqBlender &, const rgbdImage &target,
                               const double pose[16]) {
     std::array<double, 4> st{};

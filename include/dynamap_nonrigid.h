@@ -49,6 +49,18 @@ typedef struct dm_nonrigid_props {
  * vertices, normals: nv x 3 floats (host). */
 DM_API int dm_graph_create(const float *vertices, const float *normals, int64_t nv, const dm_graph_props *props,
                            dm_graph **out);
+/* defGraph::init(MeshSTD&, int ObservationNum, int mode) with the node
+ * choice made by the caller (defGraph.cu:15-69, sampleDownMeshHost
+ * :673-702: mode 1 takes the first NODE_NUM vertices, mode 0 random ones):
+ * node k is vertex node_vertex_ids[k] (distinct, 5 <= num_nodes <= 32768);
+ * neighbours and weights as dm_graph_create with weight scale dgw. */
+DM_API int dm_graph_create_nodes(const float *vertices, const float *normals, int64_t nv,
+                                 const int32_t *node_vertex_ids, int32_t num_nodes, float dgw, dm_graph **out);
+/* defGraph::defGraphUpdateNodes (defGraph.cu:483-545) and the vertex part of
+ * updateActiveNeighbourNodes (:243-302): recompute the 4 nearest nodes of
+ * every node and of every vertex and their weights from the canonical
+ * positions; node transforms are kept. */
+DM_API int dm_graph_update_nodes(dm_graph *g);
 /* defGraph::Free (defGraph.cu:86-101) */
 DM_API int dm_graph_destroy(dm_graph *g);
 /* defGraph::nodeNum */
@@ -76,6 +88,27 @@ DM_API int dm_blend(dm_graph *g, float *out_vertices, float *out_normals);
  * iterations of the last step. */
 DM_API int dm_nonrigid_solve(dm_graph *g, const float *depth, const dm_sensor *sensor, const double pose[16],
                              const dm_nonrigid_props *props, double stats[4]);
+/* nonRigidICP::init(warpField, Prop, problemSize) (nonRigidICP.cu:18-40):
+ * sizes the solver's frame buffers for `observations` pixels (the
+ * reference's problemSize, rows * cols) and pcg_iterations, so that later
+ * solves up to that size allocate no frame buffers. */
+DM_API int dm_nonrigid_reserve(dm_graph *g, int64_t observations, int32_t pcg_iterations);
+/* nonRigidICP::solve(warpField, blender, targetImage, sourceImage, pose,
+ * sensor) (nonRigidICP.cu:557-704): as dm_nonrigid_solve against
+ * target_depth, where a pixel takes part only if source_depth (the predicted
+ * image of the same view, rows*cols metres) also lies in [min_depth,
+ * max_depth] there (the range test of both images, nonRigidICP.cu:291-292). */
+DM_API int dm_nonrigid_solve_pair(dm_graph *g, const float *target_depth, const float *source_depth,
+                                  const dm_sensor *sensor, const double pose[16], const dm_nonrigid_props *props,
+                                  double stats[4]);
+/* nonRigidICP::solve(warpField, blender, targetMesh, pose)
+ * (nonRigidICP.cu:706-852): the target is a mesh with the canonical mesh's
+ * vertex order (target vertex v corresponds to canonical vertex v,
+ * nonRigidICP.cu:370-376), nv x 3 positions and unit normals in the
+ * coordinates of the camera at `pose`; same gates (dist_max, normal_cos),
+ * terms and stats as dm_nonrigid_solve.  Zero or NaN entries drop a vertex. */
+DM_API int dm_nonrigid_solve_mesh(dm_graph *g, const float *target_vertices, const float *target_normals,
+                                  const double pose[16], const dm_nonrigid_props *props, double stats[4]);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,11 @@ DM_API int dm_session_icp_system(dm_session *s, int32_t level, const double Tinc
 /* gnSolver::solve, gnSolver.cu:317-438: the multi-level increment (source
  * camera -> model camera) of the staged source frame against the model maps. */
 DM_API int dm_session_icp_track(dm_session *s, int32_t solve_on_host, double Tinc_out[16]);
+/* gnSolver::solve's `increment` (gnSolver.cu:353-430, returned through
+ * prev_increment_): the summed twist (omega, t) of the session's last
+ * tracking, dm_session_icp_track or a processed frame, zero where the
+ * reference's guard (NaN, infinite, or a sum >= max_increment) reset it. */
+DM_API int dm_session_icp_increment(dm_session *s, double twist[6]);
 
 /* ------------------------------------------------------------------ */
 /* Serving loop (no reference counterpart: the reference processes one  */

@@ -123,6 +123,8 @@ static int cmp_cell_vertex(const void *a, const void *b) {
   return p->v < q->v ? -1 : (p->v > q->v);
 }
 
+static void graph_links(or_graph *g);
+
 /*
  * defGraph::init + defGraphUpdateNodes (defGraph.cu:15-69, :483-545).
  * status: 0 ok, -1 fewer than K+1 nodes, -2 more than max_nodes.
@@ -167,13 +169,29 @@ OR_EXPORT or_graph *or_graph_create(const float *verts, const float *normals, in
     if (isnode[v]) g->node_vid[k++] = (int32_t)v;
   }
   free(isnode);
-  g->nbr = (int32_t *)malloc((size_t)(nn ? nn : 1) * NR_K * sizeof(int32_t));
-  g->nbr_w = (real *)malloc((size_t)(nn ? nn : 1) * NR_K * sizeof(real));
-  g->vnn = (int32_t *)malloc((size_t)nv * NR_K * sizeof(int32_t));
-  g->vnn_w = (real *)malloc((size_t)nv * NR_K * sizeof(real));
-  g->T = (double *)calloc((size_t)(nn ? nn : 1) * 16, sizeof(double));
-  for (int i = 0; i < nn; ++i) { g->T[16 * i] = g->T[16 * i + 5] = g->T[16 * i + 10] = g->T[16 * i + 15] = 1.0; }
-  if (*status != 0) return g;
+  if (*status != 0) {
+    g->T = (double *)calloc((size_t)(nn ? nn : 1) * 16, sizeof(double));
+    return g;
+  }
+  graph_links(g);
+  return g;
+}
+
+/* defGraph::defGraphUpdateNodes (defGraph.cu:483-545) and the vertex
+ * neighbours of updateActiveNeighbourNodes (:243-302): the K nearest nodes
+ * of every node and of every vertex with weights exp(-d^2/dgw^2).  The
+ * first call also sets every node transform to the identity. */
+static void graph_links(or_graph *g) {
+  const int nn = g->nn;
+  const int64_t nv = g->nv;
+  if (!g->nbr) {
+    g->nbr = (int32_t *)malloc((size_t)(nn ? nn : 1) * NR_K * sizeof(int32_t));
+    g->nbr_w = (real *)malloc((size_t)(nn ? nn : 1) * NR_K * sizeof(real));
+    g->vnn = (int32_t *)malloc((size_t)nv * NR_K * sizeof(int32_t));
+    g->vnn_w = (real *)malloc((size_t)nv * NR_K * sizeof(real));
+    g->T = (double *)calloc((size_t)(nn ? nn : 1) * 16, sizeof(double));
+    for (int i = 0; i < nn; ++i) { g->T[16 * i] = g->T[16 * i + 5] = g->T[16 * i + 10] = g->T[16 * i + 15] = 1.0; }
+  }
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < nn; ++i) {
     real d2[NR_K];
@@ -186,8 +204,46 @@ OR_EXPORT or_graph *or_graph_create(const float *verts, const float *normals, in
     knn_nodes(g, g->verts + 3 * v, -1, g->vnn + NR_K * v, d2);
     for (int q = 0; q < NR_K; ++q) g->vnn_w[NR_K * v + q] = weight_of(clamp_dist(d2[q]), g->dgw);
   }
+}
+
+/*
+ * defGraph::init with the nodes given (defGraph.cu:15-69 with
+ * sampleDownMeshHost, :673-702): node k is vertex node_ids[k] (mode 1 of the
+ * reference passes the first NODE_NUM vertices, mode 0 random ones).
+ * status: 0 ok, -1 fewer than K+1 nodes, -3 an id out of range or repeated.
+ */
+OR_EXPORT or_graph *or_graph_create_nodes(const float *verts, const float *normals, int64_t nv,
+                                          const int32_t *node_ids, int nn, double dgw, int *status) {
+  or_graph *g = (or_graph *)calloc(1, sizeof(or_graph));
+  g->nv = nv;
+  g->dgw = (real)dgw;
+  g->verts = (float *)malloc(nv * 3 * sizeof(float));
+  g->normals = (float *)malloc(nv * 3 * sizeof(float));
+  memcpy(g->verts, verts, nv * 3 * sizeof(float));
+  memcpy(g->normals, normals, nv * 3 * sizeof(float));
+  g->nn = nn;
+  g->node_vid = (int32_t *)malloc((nn > 0 ? nn : 1) * sizeof(int32_t));
+  g->node_of_vertex = (int32_t *)malloc(nv * sizeof(int32_t));
+  for (int64_t v = 0; v < nv; ++v) g->node_of_vertex[v] = -1;
+  *status = nn < NR_K + 1 ? -1 : 0;
+  for (int k = 0; k < nn && *status == 0; ++k) {
+    const int32_t id = node_ids[k];
+    if (id < 0 || id >= nv || g->node_of_vertex[id] >= 0) { *status = -3; break; }
+    g->node_of_vertex[id] = k;
+    g->node_vid[k] = id;
+  }
+  if (*status != 0) {
+    g->nn = 0;
+    g->T = (double *)calloc(16, sizeof(double));
+    return g;
+  }
+  graph_links(g);
   return g;
 }
+
+/* defGraph::defGraphUpdateNodes: neighbours and weights recomputed from the
+ * canonical positions; node transforms kept. */
+OR_EXPORT void or_graph_update_nodes(or_graph *g) { graph_links(g); }
 
 OR_EXPORT void or_graph_destroy(or_graph *g) {
   if (!g) return;
@@ -517,7 +573,48 @@ typedef struct {
   int rows, cols;
   double fx, fy, cx, cy;
   const double *pose;
+  /* optional: the source (predicted) depth of the same view; a pixel takes
+   * part only where it lies in [min_depth, max_depth] (the reference's
+   * range test on both images, nonRigidICP.cu:291-292) */
+  const float *src_depth;
+  /* optional mesh target in camera coordinates, vertex v corresponding to
+   * canonical vertex v (nonRigidICP.cu:354-376); NULL = the depth target */
+  const float *mesh_v, *mesh_n;
 } nr_frame;
+
+/* Correspondence of the warped vertex p (world) with normal nw: target point
+ * q and normal nt in camera coordinates, pc / ncam the warped vertex and
+ * normal in camera coordinates.  0 = no correspondence. */
+static int nr_associate(const nr_frame *F, const or_nr_props *pr, int64_t v, const double p[3], const double nw[3],
+                        double pc[3], double ncam[3], double q[3], double nt[3]) {
+  const double *pose = F->pose;
+  for (int r = 0; r < 3; ++r) {
+    pc[r] = pose[r] * (p[0] - pose[3]) + pose[4 + r] * (p[1] - pose[7]) + pose[8 + r] * (p[2] - pose[11]);
+    ncam[r] = pose[r] * nw[0] + pose[4 + r] * nw[1] + pose[8 + r] * nw[2];
+  }
+  if (F->mesh_v) {
+    for (int r = 0; r < 3; ++r) { q[r] = F->mesh_v[3 * v + r]; nt[r] = F->mesh_n[3 * v + r]; }
+    const double n2 = nt[0] * nt[0] + nt[1] * nt[1] + nt[2] * nt[2];
+    if (!(n2 > 0) || !(q[0] == q[0] && q[1] == q[1] && q[2] == q[2])) return 0;
+  } else {
+    if (!(pc[2] > 0)) return 0;
+    const double uf = F->fx * pc[0] / pc[2] + F->cx, vf = F->fy * pc[1] / pc[2] + F->cy;
+    if (!(uf > -1.0 && vf > -1.0 && uf < (double)F->cols && vf < (double)F->rows)) return 0;
+    const int u = (int)floor(uf + 0.5), vv = (int)floor(vf + 0.5);
+    if (u < 0 || vv < 0 || u >= F->cols || vv >= F->rows) return 0;
+    const float d = F->depth[vv * F->cols + u];
+    if (!(d >= (float)pr->min_depth && d <= (float)pr->max_depth)) return 0;
+    if (F->src_depth) {
+      const float ds = F->src_depth[vv * F->cols + u];
+      if (!(ds >= (float)pr->min_depth && ds <= (float)pr->max_depth)) return 0;
+    }
+    if (!target_vn(F->depth, F->rows, F->cols, F->fx, F->fy, F->cx, F->cy, vv * F->cols + u, q, nt)) return 0;
+  }
+  const double dx = pc[0] - q[0], dy = pc[1] - q[1], dz = pc[2] - q[2];
+  if (dx * dx + dy * dy + dz * dz > pr->dist_max * pr->dist_max) return 0;
+  if (fabs(ncam[0] * nt[0] + ncam[1] * nt[1] + ncam[2] * nt[2]) < pr->normal_cos) return 0;
+  return 1;
+}
 
 /* Gauss-Newton linearisation at the current node transforms: fills m (data
  * and ARAP blocks, rhs = -J^T r, damping on the diagonal) */
@@ -533,23 +630,8 @@ static void nr_assemble(const or_graph *g, const nr_frame *F, const or_nr_props 
     /* association in FP64 from the FP32 warped vertex */
     const double p[3] = {pv[3 * v], pv[3 * v + 1], pv[3 * v + 2]};
     const double nw[3] = {pn[3 * v], pn[3 * v + 1], pn[3 * v + 2]};
-    double pc[3], ncam[3];
-    for (int r = 0; r < 3; ++r) {
-      pc[r] = pose[r] * (p[0] - pose[3]) + pose[4 + r] * (p[1] - pose[7]) + pose[8 + r] * (p[2] - pose[11]);
-      ncam[r] = pose[r] * nw[0] + pose[4 + r] * nw[1] + pose[8 + r] * nw[2];
-    }
-    if (!(pc[2] > 0)) continue;
-    const double uf = F->fx * pc[0] / pc[2] + F->cx, vf = F->fy * pc[1] / pc[2] + F->cy;
-    if (!(uf > -1.0 && vf > -1.0 && uf < (double)F->cols && vf < (double)F->rows)) continue;
-    const int u = (int)floor(uf + 0.5), vv = (int)floor(vf + 0.5);
-    if (u < 0 || vv < 0 || u >= F->cols || vv >= F->rows) continue;
-    const float d = F->depth[vv * F->cols + u];
-    if (!(d >= (float)pr->min_depth && d <= (float)pr->max_depth)) continue;
-    double q[3], nt[3];
-    if (!target_vn(F->depth, F->rows, F->cols, F->fx, F->fy, F->cx, F->cy, vv * F->cols + u, q, nt)) continue;
-    const double dx = pc[0] - q[0], dy = pc[1] - q[1], dz = pc[2] - q[2];
-    if (dx * dx + dy * dy + dz * dz > pr->dist_max * pr->dist_max) continue;
-    if (fabs(ncam[0] * nt[0] + ncam[1] * nt[1] + ncam[2] * nt[2]) < pr->normal_cos) continue;
+    double pc[3], ncam[3], q[3], nt[3];
+    if (!nr_associate(F, pr, v, p, nw, pc, ncam, q, nt)) continue;
     /* world-frame target normal and point, residual n . (p' - q) */
     double nW[3], qW[3];
     for (int r = 0; r < 3; ++r) {
@@ -642,9 +724,8 @@ static void nr_update(or_graph *g, const double *x) {
  * [0] data residuals, [1] data energy, [2] ARAP energy (at the last
  * linearisation), [3] PCG iterations of the last step.
  */
-OR_EXPORT int or_nonrigid_solve(or_graph *g, const float *depth, int rows, int cols, double fx, double fy, double cx,
-                                double cy, const double pose[16], const or_nr_props *pr, double *stats) {
-  const nr_frame F = {depth, rows, cols, fx, fy, cx, cy, pose};
+static int nonrigid_solve(or_graph *g, const nr_frame *Fp, const or_nr_props *pr, double *stats) {
+  const nr_frame F = *Fp;
   bsm m;
   bsm_build(&m, g);
   float *pv = (float *)malloc(g->nv * 3 * sizeof(float)), *pn = (float *)malloc(g->nv * 3 * sizeof(float));
@@ -664,13 +745,38 @@ OR_EXPORT int or_nonrigid_solve(or_graph *g, const float *depth, int rows, int c
   return 0;
 }
 
+OR_EXPORT int or_nonrigid_solve(or_graph *g, const float *depth, int rows, int cols, double fx, double fy, double cx,
+                                double cy, const double pose[16], const or_nr_props *pr, double *stats) {
+  const nr_frame F = {depth, rows, cols, fx, fy, cx, cy, pose, NULL, NULL, NULL};
+  return nonrigid_solve(g, &F, pr, stats);
+}
+
+/* nonRigidICP::solve(warpField, blender, targetImage, sourceImage, pose,
+ * sensor) (nonRigidICP.cu:557-704): the depth target gated by the source
+ * image's depth range at the associated pixel (nonRigidICP.cu:291-292). */
+OR_EXPORT int or_nonrigid_solve_pair(or_graph *g, const float *depth, const float *src_depth, int rows, int cols,
+                                     double fx, double fy, double cx, double cy, const double pose[16],
+                                     const or_nr_props *pr, double *stats) {
+  const nr_frame F = {depth, rows, cols, fx, fy, cx, cy, pose, src_depth, NULL, NULL};
+  return nonrigid_solve(g, &F, pr, stats);
+}
+
+/* nonRigidICP::solve(warpField, blender, targetMesh, pose)
+ * (nonRigidICP.cu:706-852): target vertex v (camera coordinates, with its
+ * normal) corresponds to canonical vertex v (nonRigidICP.cu:370-376). */
+OR_EXPORT int or_nonrigid_solve_mesh(or_graph *g, const float *mesh_v, const float *mesh_n, const double pose[16],
+                                     const or_nr_props *pr, double *stats) {
+  const nr_frame F = {NULL, 0, 0, 0, 0, 0, 0, pose, NULL, mesh_v, mesh_n};
+  return nonrigid_solve(g, &F, pr, stats);
+}
+
 /* The linearisation at the current transforms, densified (tests, small
  * graphs): H (6N x 6N, damping included), rhs (6N), the PCG solution x and
  * the number of upper blocks. */
 OR_EXPORT int or_nonrigid_system(const or_graph *g, const float *depth, int rows, int cols, double fx, double fy,
                                  double cx, double cy, const double pose[16], const or_nr_props *pr, double *H,
                                  double *rhs, double *x, double *energy) {
-  const nr_frame F = {depth, rows, cols, fx, fy, cx, cy, pose};
+  const nr_frame F = {depth, rows, cols, fx, fy, cx, cy, pose, NULL, NULL, NULL};
   const int nn = g->nn, n = 6 * nn;
   bsm m;
   bsm_build(&m, g);

@@ -320,15 +320,22 @@ def nr_props(**kw):
 class OracleGraph:
     """Deformation graph + blender + non-rigid solver of the CPU oracle."""
 
-    def __init__(self, verts, normals, node_radius=0.025, dgw=0.05, max_nodes=4096, bits=64):
+    def __init__(self, verts, normals, node_radius=0.025, dgw=0.05, max_nodes=4096, bits=64, node_ids=None):
         L = self.L = load(bits)
         L.or_graph_create.restype = _vp
         L.or_graph_create.argtypes = [_vp, _vp, C.c_int64, _d, _d, _i, _vp]
+        L.or_graph_create_nodes.restype = _vp
+        L.or_graph_create_nodes.argtypes = [_vp, _vp, C.c_int64, _vp, _i, _d, _vp]
         self.verts = np.ascontiguousarray(verts, np.float32)
         self.normals = np.ascontiguousarray(normals, np.float32)
         st = np.zeros(1, np.int32)
-        self.h = L.or_graph_create(P(self.verts), P(self.normals), len(self.verts), node_radius, dgw, max_nodes,
-                                   P(st))
+        if node_ids is None:
+            self.h = L.or_graph_create(P(self.verts), P(self.normals), len(self.verts), node_radius, dgw,
+                                       max_nodes, P(st))
+        else:
+            ids = np.ascontiguousarray(node_ids, np.int32)
+            self.h = L.or_graph_create_nodes(P(self.verts), P(self.normals), len(self.verts), P(ids), len(ids),
+                                             dgw, P(st))
         self.status = int(st[0])
         L.or_graph_num_nodes.restype = _i
         L.or_graph_num_nodes.argtypes = [_vp]
@@ -336,11 +343,16 @@ class OracleGraph:
         for n, a in (("or_graph_nodes", [_vp, _vp, _vp, _vp]), ("or_graph_vertex_nn", [_vp, _vp, _vp]),
                      ("or_graph_set_transforms", [_vp, _vp]), ("or_graph_get_transforms", [_vp, _vp]),
                      ("or_blend", [_vp, _vp, _vp]), ("or_graph_destroy", [_vp]),
-                     ("or_dq_of_transform", [_vp, _vp])):
+                     ("or_dq_of_transform", [_vp, _vp]), ("or_graph_update_nodes", [_vp])):
             f = getattr(L, n)
             f.restype, f.argtypes = None, a
         L.or_nonrigid_solve.restype = _i
         L.or_nonrigid_solve.argtypes = [_vp, _vp, _i, _i, _d, _d, _d, _d, _vp, C.POINTER(or_nr_props), _vp]
+        L.or_nonrigid_solve_pair.restype = _i
+        L.or_nonrigid_solve_pair.argtypes = [_vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, C.POINTER(or_nr_props),
+                                             _vp]
+        L.or_nonrigid_solve_mesh.restype = _i
+        L.or_nonrigid_solve_mesh.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(or_nr_props), _vp]
         L.or_nonrigid_system.restype = _i
         L.or_nonrigid_system.argtypes = [_vp, _vp, _i, _i, _d, _d, _d, _d, _vp, C.POINTER(or_nr_props), _vp, _vp,
                                          _vp, _vp]
@@ -385,6 +397,27 @@ class OracleGraph:
         pr = nr_props(**kw)
         self.L.or_nonrigid_solve(self.h, P(depth), *_intr(sensor), P(pose), C.byref(pr), P(st))
         return st
+
+    def solve_pair(self, depth, src_depth, sensor, pose, **kw):
+        depth = np.ascontiguousarray(depth, np.float32)
+        src = np.ascontiguousarray(src_depth, np.float32)
+        pose = np.ascontiguousarray(pose, np.float64)
+        st = np.zeros(4)
+        pr = nr_props(**kw)
+        self.L.or_nonrigid_solve_pair(self.h, P(depth), P(src), *_intr(sensor), P(pose), C.byref(pr), P(st))
+        return st
+
+    def solve_mesh(self, mesh_v, mesh_n, pose, **kw):
+        mv = np.ascontiguousarray(mesh_v, np.float32)
+        mn = np.ascontiguousarray(mesh_n, np.float32)
+        pose = np.ascontiguousarray(pose, np.float64)
+        st = np.zeros(4)
+        pr = nr_props(**kw)
+        self.L.or_nonrigid_solve_mesh(self.h, P(mv), P(mn), P(pose), C.byref(pr), P(st))
+        return st
+
+    def update_nodes(self):
+        self.L.or_graph_update_nodes(self.h)
 
     def system(self, depth, sensor, pose, **kw):
         n = 6 * self.nn

@@ -156,3 +156,71 @@ def test_bending_full_length_120():
     print("surface error mean (mm) at frames 19, 39, ..., 119:", [round(1000 * e, 4) for e in errs])
     g.close()
     assert max(errs) < 5e-4
+
+
+def test_given_nodes_and_update_match_transcription():
+    """dm_graph_create_nodes (defGraph::init with the reference's own node
+    choice, defGraph.cu:673-702) and dm_graph_update_nodes
+    (defGraphUpdateNodes, :483-545): neighbour ids bit-exact vs the FP32
+    transcription for an unsorted random node list; the update keeps
+    transforms and links."""
+    V, Nn = cylinder(200, 100)
+    ids = np.random.default_rng(11).choice(len(V), 700, replace=False).astype(np.int32)
+    g = dm.DefGraph(V, Nn, dgw=0.05, node_ids=ids)
+    o = O.OracleGraph(V, Nn, dgw=0.05, node_ids=ids, bits=32)
+    assert g.nodeNum == o.nn == 700
+    vid, nbr, w = g.nodes()
+    ovid, onbr, ow = o.nodes()
+    assert np.array_equal(vid, ids) and np.array_equal(nbr, onbr)
+    assert np.allclose(w, ow, rtol=2e-7, atol=1e-12)
+    vids, vw = g.vertex_neighbours()
+    assert np.array_equal(vids, o.vertex_nn()[0])
+    T = random_transforms(g.nodeNum, 4, rot=0.05, trans=0.01)
+    g.set_transforms(T)
+    g.defGraphUpdateNodes()
+    assert np.array_equal(g.nodes()[1], nbr) and np.array_equal(g.vertex_neighbours()[0], vids)
+    assert np.array_equal(g.transforms(), T)
+    with pytest.raises(dm._native.DmError):
+        dm.DefGraph(V, Nn, dgw=0.05, node_ids=[1, 2, 3, 4, 4])
+
+
+def test_pair_and_mesh_solves_match_fp64_oracle(bend_case):
+    """dm_nonrigid_solve_pair (source depth gates the pixels,
+    nonRigidICP.cu:291-292) and dm_nonrigid_solve_mesh (vertex-for-vertex
+    mesh target, nonRigidICP.cu:706-852) against the FP64 oracle."""
+    V, Nn, pose, depth = bend_case
+    src = depth.copy()
+    src[:, : SENS["cols"] // 2] = 0.0
+    g = dm.DefGraph(V, Nn, 0.025, 0.05)
+    o = O.OracleGraph(V, Nn, 0.025, 0.05, bits=64)
+    solver = dm.NonRigidICP()
+    solver.reserve(g, SENS["rows"] * SENS["cols"])
+    sg = solver.solve(g, depth, SENS, pose, source_depth=src)
+    so = o.solve_pair(depth, src, SENS, pose)
+    full = O.OracleGraph(V, Nn, 0.025, 0.05, bits=64).solve(depth, SENS, pose)
+    print("pair: gpu", sg, "oracle", so, "ungated oracle residuals", full[0])
+    assert so[0] < 0.8 * full[0]
+    assert abs(sg["residuals"] - so[0]) <= 0.01 * so[0]
+    gv, _ = dm.DqBlender(g).blendMesh()
+    ov, on = o.blend()
+    vis = V[:, 2] < -0.07
+    dn = np.abs(np.sum((gv - ov) * on, 1))[vis]
+    assert dn.mean() < 1e-4
+    # mesh target: the analytic bend of frame 10 in the camera frame at pose
+    b = 0.05 * np.sin(2 * np.pi * 10 / 40) * ((0.3 - V[:, 1].astype(np.float64)) / 0.6) ** 2
+    Vt = V.astype(np.float64).copy()
+    Vt[:, 0] += b
+    Ti = np.linalg.inv(pose)
+    mv = (Vt @ Ti[:3, :3].T + Ti[:3, 3]).astype(np.float32)
+    mn = (Nn.astype(np.float64) @ Ti[:3, :3].T).astype(np.float32)
+    g2 = dm.DefGraph(V, Nn, 0.025, 0.05)
+    o2 = O.OracleGraph(V, Nn, 0.025, 0.05, bits=64)
+    sg2 = solver.solve_mesh(g2, mv, mn, pose)
+    so2 = o2.solve_mesh(mv, mn, pose)
+    print("mesh: gpu", sg2, "oracle", so2)
+    assert sg2["residuals"] == so2[0] > 0.9 * len(V)
+    gv2, _ = dm.DqBlender(g2).blendMesh()
+    ov2, on2 = o2.blend()
+    dn2 = np.abs(np.sum((gv2 - ov2) * on2, 1))
+    assert dn2.mean() < 1e-4
+    assert np.abs(np.sum((ov2 - Vt) * on2, 1)).mean() < 3e-4

@@ -303,3 +303,130 @@ def test_bend_recovered():
     vb, _ = g.blend()
     err = np.abs(np.sqrt((vb[:, 0] - b(vb[:, 1])) ** 2 + vb[:, 2] ** 2) - 0.15)[vis]
     assert err0 > 5e-3 and err.mean() < 5e-4
+
+
+# ------------------------------------------------ given nodes, update, source gate, mesh target
+def test_given_nodes_match_brute_force():
+    """or_graph_create_nodes (defGraph::init with the reference's own node
+    choice, defGraph.cu:673-702): neighbours of an arbitrary, unsorted node
+    list by brute force; the sampled node list reproduces the sampled graph;
+    defGraphUpdateNodes recomputes the same links and keeps the transforms."""
+    V, Nn = random_cloud(2000, 7)
+    rng = np.random.default_rng(3)
+    ids = rng.choice(len(V), 150, replace=False).astype(np.int32)  # unsorted, as mode 0's random picks
+    dgw = 0.05
+    g = O.OracleGraph(V, Nn, dgw=dgw, node_ids=ids)
+    assert g.status == 0 and g.nn == 150
+    vid, nbr, w = g.nodes()
+    assert np.array_equal(vid, ids)
+    P = V[ids].astype(np.float64)
+    D = np.linalg.norm(P[:, None] - P[None], axis=2)
+    np.fill_diagonal(D, np.inf)
+    assert np.array_equal(nbr, np.argsort(D, axis=1, kind="stable")[:, :4])
+    dn = np.take_along_axis(D, nbr.astype(np.int64), 1)
+    assert np.allclose(w, np.exp(-np.maximum(dn, 1e-4) ** 2 / dgw ** 2), rtol=1e-12, atol=1e-15)
+    vids, vw = g.vertex_nn()
+    Dv = np.linalg.norm(V[:, None].astype(np.float64) - P[None], axis=2)
+    assert np.array_equal(vids, np.argsort(Dv, axis=1, kind="stable")[:, :4])
+    # a node vertex blends with its own node only (the node-vertex rule)
+    T = np.tile(np.eye(4), (g.nn, 1, 1))
+    T[5, :3, 3] = [0.01, 0.02, -0.03]
+    g.set_transforms(T)
+    bv, _ = g.blend()
+    assert np.allclose(bv[ids[5]], V[ids[5]] + T[5, :3, 3], atol=1e-7)
+    g.update_nodes()
+    assert np.array_equal(g.nodes()[1], nbr) and np.array_equal(g.vertex_nn()[0], vids)
+    assert np.array_equal(g.transforms(), T)
+    # the grid-sampled node list given explicitly gives the same graph
+    gs = O.OracleGraph(V, Nn, 0.05, dgw)
+    svid, snbr, sw = gs.nodes()
+    ge = O.OracleGraph(V, Nn, dgw=dgw, node_ids=svid)
+    evid, enbr, ew = ge.nodes()
+    assert np.array_equal(evid, svid) and np.array_equal(enbr, snbr) and np.array_equal(ew, sw)
+    assert np.array_equal(ge.vertex_nn()[0], gs.vertex_nn()[0])
+    # repeated or out-of-range ids are rejected
+    assert O.OracleGraph(V, Nn, dgw=dgw, node_ids=[1, 2, 3, 4, 4]).status == -3
+    assert O.OracleGraph(V, Nn, dgw=dgw, node_ids=[1, 2, 3, 4, len(V)]).status == -3
+    assert O.OracleGraph(V, Nn, dgw=dgw, node_ids=[1, 2, 3, 4]).status == -1
+
+
+def plane_grid(n=60, z=1.0, step=0.01):
+    """A fronto-parallel vertex grid at depth z (camera = world), normals to the camera."""
+    xs = (np.arange(n) - n / 2 + 0.25) * step
+    X, Y = np.meshgrid(xs, xs)
+    V = np.stack([X.ravel(), Y.ravel(), np.full(X.size, z)], 1).astype(np.float32)
+    Nn = np.tile(np.array([0, 0, -1], np.float32), (len(V), 1))
+    return V, Nn
+
+
+def test_source_gate_selects_pixels():
+    """or_nonrigid_solve_pair: a vertex takes part only where the source depth
+    at its projected pixel is in range (nonRigidICP.cu:291-292).  On a wall
+    seen head on, the residual count equals the number of vertices whose
+    pixel (u = round(fx x / z + cx), v = round(fy y / z + cy)) the source
+    keeps; an all-valid source changes nothing."""
+    V, Nn = plane_grid()
+    s = SENS
+    depth = np.full((s["rows"], s["cols"]), 1.0, np.float32)
+    u = np.floor(s["fx"] * V[:, 0].astype(np.float64) / V[:, 2] + s["cx"] + 0.5).astype(int)
+    v = np.floor(s["fy"] * V[:, 1].astype(np.float64) / V[:, 2] + s["cy"] + 0.5).astype(int)
+    src = depth.copy()
+    src[:, : s["cols"] // 2] = 0.0     # left half of the source unobserved
+    src[: s["rows"] // 2 - 3, :] = 9.0  # upper rows beyond max_depth
+    keep = (src[v, u] >= 0.1) & (src[v, u] <= 5.0)
+    assert 0 < keep.sum() < len(V)
+    kw = dict(gn_iterations=1, pcg_iterations=5)
+    g0 = O.OracleGraph(V, Nn, 0.05, 0.1)
+    st0 = g0.solve(depth, s, np.eye(4), **kw)
+    assert st0[0] == len(V)
+    g1 = O.OracleGraph(V, Nn, 0.05, 0.1)
+    st1 = g1.solve_pair(depth, depth, s, np.eye(4), **kw)
+    assert np.array_equal(st1, st0) and np.array_equal(g1.transforms(), g0.transforms())
+    g2 = O.OracleGraph(V, Nn, 0.05, 0.1)
+    st2 = g2.solve_pair(depth, src, s, np.eye(4), **kw)
+    assert st2[0] == keep.sum()
+    # the gate reads the source, not the target: the swapped call keeps all
+    g3 = O.OracleGraph(V, Nn, 0.05, 0.1)
+    assert g3.solve_pair(depth, depth, s, np.eye(4), **kw)[0] == len(V)
+
+
+def test_mesh_target_recovers_rigid_motion():
+    """or_nonrigid_solve_mesh (nonRigidICP.cu:706-852, vertex v <-> v): the
+    canonical mesh itself is a zero-residual target (no step); a rigidly
+    moved copy is recovered by every node; vertices with a zero normal or a
+    NaN position drop out."""
+    V, Nn = cylinder(60, 40)
+    pose = cyl_pose(3)
+    Ti = np.linalg.inv(pose)
+    cam = lambda P, R=Ti: (P @ R[:3, :3].T + R[:3, 3]).astype(np.float32)
+    kw = dict(gn_iterations=5, pcg_iterations=60, pcg_tol=1e-10, dist_max=0.1, normal_cos=0.3, lambda_reg=1.0)
+    g = O.OracleGraph(V, Nn, 0.04, 0.08)
+    st = g.solve_mesh(cam(V.astype(np.float64)), (Nn.astype(np.float64) @ Ti[:3, :3].T).astype(np.float32), pose,
+                      **kw)
+    assert st[0] == len(V) and st[1] < 1e-12
+    # (the target is rounded to float32 in the camera frame: steps of that size only)
+    assert np.allclose(g.transforms(), np.eye(4), atol=1e-6)
+    # a rigid motion M of the target (world frame) with no part along the
+    # cylinder's own symmetries (rotation about and sliding along its y axis),
+    # which point-to-plane residuals cannot see
+    M = np.eye(4)
+    M[:3, :3] = Rotation.from_rotvec([0.02, 0.0, 0.015]).as_matrix()
+    M[:3, 3] = [0.01, 0.0, 0.012]
+    Vt = (V.astype(np.float64) @ M[:3, :3].T + M[:3, 3])
+    Nt = Nn.astype(np.float64) @ M[:3, :3].T
+    mv, mn = cam(Vt), (Nt @ Ti[:3, :3].T).astype(np.float32)
+    g = O.OracleGraph(V, Nn, 0.04, 0.08)
+    e0 = O.OracleGraph(V, Nn, 0.04, 0.08).solve_mesh(mv, mn, pose, **dict(kw, gn_iterations=1))[1]
+    for _ in range(3):
+        st = g.solve_mesh(mv, mn, pose, **kw)
+    assert st[1] < 1e-5 * e0
+    T = g.transforms()
+    assert np.abs(T - M).max() < 1e-3, np.abs(T - M).max()  # from a 1.5 cm, 1.4 degree motion
+    bv, _ = g.blend()
+    assert np.abs(bv - Vt).max() < 5e-4
+    # dropped vertices
+    mn2, mv2 = mn.copy(), mv.copy()
+    mn2[:10] = 0
+    mv2[10:15] = np.nan
+    g = O.OracleGraph(V, Nn, 0.04, 0.08)
+    assert g.solve_mesh(mv2, mn2, pose, gn_iterations=1)[0] == len(V) - 15
