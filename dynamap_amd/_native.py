@@ -94,6 +94,7 @@ PROTOTYPES = {
     "dm_graph_create_nodes": (C.c_int, [_vp, _vp, C.c_int64, _vp, C.c_int32, C.c_float, _P(_vp)]),
     "dm_graph_update_nodes": (C.c_int, [_vp]),
     "dm_mesh_extract": (C.c_int, [_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _P(C.c_int64)]),
+    "dm_mesh_extract_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _P(C.c_int64)]),
     "dm_session_create": (C.c_int, [_P(dm_tsdf_props), _P(dm_icp_props), _P(dm_sensor), _vp, _P(_vp)]),
     "dm_session_destroy": (C.c_int, [_vp]),
     "dm_session_volume": (_vp, [_vp]),

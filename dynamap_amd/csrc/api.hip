@@ -78,6 +78,7 @@ int Sess::init(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   props = p;
   max_rows = rows;
   max_cols = cols;
+  DM_HIP(hipGetDevice(&device));
   DM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   memset(&h, 0, sizeof(h));
   h.rows = (int)rows;

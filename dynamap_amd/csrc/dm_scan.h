@@ -56,5 +56,11 @@ int dev_alloc(DevBuf &b, size_t bytes);
 // out[i] = in[0] + ... + in[i-1] (int64); *d_total points at the device
 // total (valid until wsum is released).  Asynchronous on st.
 int scan_ints(const int *in, int n, long long *out, DevBuf &wsum, hipStream_t st, long long **d_total);
+// Workspace (int64 values) the scan of n items needs: one sum per workgroup
+// of kScanItems items, plus the total.
+inline int scan_wsum_count(int n) { return std::max(1, (n + kScanItems - 1) / kScanItems) + 1; }
+// The same scan with a caller-provided workspace of scan_wsum_count(n)
+// int64 values; *d_total = wsum + scan_wsum_count(n) - 1.
+int scan_ints_ws(const int *in, int n, long long *out, long long *wsum, hipStream_t st, long long **d_total);
 
 }  // namespace dm

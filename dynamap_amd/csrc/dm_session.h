@@ -35,6 +35,7 @@ struct Sess {
   DmView *d_view = nullptr;      // device copy (one-entry table)
   int *d_active = nullptr;       // device {0}
   hipStream_t stream = nullptr;
+  int device = 0;                // HIP device the state lives on (current at init)
   std::vector<void *> allocs;    // device allocations owned by the session
   float *d_depth = nullptr;      // frame staging
   uint8_t *d_rgb = nullptr;

@@ -24,6 +24,7 @@
 // those of the restatement in the same order, so positions are bit-identical.
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 #include "dm_device.h"
 #include "dm_session.h"
@@ -288,6 +289,69 @@ __global__ __launch_bounds__(kCellThreads) void k_mc_emit(const DmView *__restri
 
 // ---------------------------------------------------------------- host
 
+// Scratch of dm_mesh_extract_noalloc: one fixed array per device, resident
+// until the process ends.  Segments (bytes, each 256-aligned):
+//   counts  int32 per bucket            (kNaMaxBuckets)
+//   offs    int64 per bucket            (kNaMaxBuckets)
+//   wsum1   scan workspace of the buckets
+//   list    pool index per allocated block (kNaMaxBlocks)
+//   cubes   cube index per cell of a chunk of kNaChunk blocks
+//   btri    triangles per block of the chunk
+//   boff    triangle offset per block of the chunk
+//   wsum2   scan workspace of the chunk
+// Every segment is written before it is read within one call, so nothing
+// needs clearing between calls.
+constexpr int kNaMaxBuckets = 1 << 20;
+constexpr int kNaMaxBlocks = 1 << 21;
+constexpr int kNaChunk = 1 << 14;
+constexpr size_t na_align(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNaOffCounts = 0;
+constexpr size_t kNaOffOffs = kNaOffCounts + na_align((size_t)kNaMaxBuckets * 4);
+constexpr size_t kNaOffWsum1 = kNaOffOffs + na_align((size_t)kNaMaxBuckets * 8);
+constexpr size_t kNaOffList = kNaOffWsum1 + na_align((size_t)(kNaMaxBuckets / kScanItems + 2) * 8);
+constexpr size_t kNaOffCubes = kNaOffList + na_align((size_t)kNaMaxBlocks * 4);
+constexpr size_t kNaOffBtri = kNaOffCubes + na_align((size_t)kNaChunk * kBlockVox);
+constexpr size_t kNaOffBoff = kNaOffBtri + na_align((size_t)kNaChunk * 4);
+constexpr size_t kNaOffWsum2 = kNaOffBoff + na_align((size_t)kNaChunk * 8);
+constexpr size_t kNaScratchBytes = kNaOffWsum2 + na_align((size_t)(kNaChunk / kScanItems + 2) * 8);
+
+static __device__ __align__(256) uint8_t scratch[kNaScratchBytes];
+
+// Device address of `scratch` per device, taken once with that device
+// current.  The host-side name of the array is not a device address.
+constexpr int kNaMaxDevices = 64;
+static std::mutex g_na_mutex;
+static uint8_t *g_na_ptr[kNaMaxDevices];
+
+static int na_scratch(int device, uint8_t **out) {
+  if (device < 0 || device >= kNaMaxDevices) {
+    set_error("dm_mesh_extract_noalloc: device index out of range");
+    return DM_E_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(g_na_mutex);
+  if (!g_na_ptr[device]) {
+    void *p = nullptr;
+    DM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_na_ptr[device] = (uint8_t *)p;
+  }
+  *out = g_na_ptr[device];
+  return DM_OK;
+}
+
+// Makes `device` current for the scope and restores the previous one.
+struct DeviceScope {
+  int prev = -1;
+  int enter(int device) {
+    DM_HIP(hipGetDevice(&prev));
+    if (prev != device) DM_HIP(hipSetDevice(device));
+    return DM_OK;
+  }
+  ~DeviceScope() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
 }  // namespace dm
 
 using namespace dm;
@@ -356,6 +420,75 @@ int dm_mesh_extract(dm_volume *vol, const double *box_lo, const double *box_hi, 
   if (col) DM_HIP(hipMemcpyAsync(col, dcol.p, (size_t)m * 36, hipMemcpyDeviceToHost, st));
   if (edge_ids) DM_HIP(hipMemcpyAsync(edge_ids, deid.p, (size_t)m * 24, hipMemcpyDeviceToHost, st));
   DM_HIP(hipStreamSynchronize(st));
+  return DM_OK;
+}
+
+int dm_mesh_extract_noalloc(dm_volume *vol, const double *box_lo, const double *box_hi, int64_t cap, float *dev_pos,
+                            float *dev_col, uint64_t *dev_edge_ids, int64_t *ntri_out) {
+  if (!vol || !ntri_out || cap < 0 || (!box_lo) != (!box_hi)) {
+    set_error("dm_mesh_extract_noalloc: invalid argument");
+    return DM_E_INVALID;
+  }
+  const int nb = vol->h.num_buckets;
+  if (nb > kNaMaxBuckets) {
+    set_error("dm_mesh_extract_noalloc: the volume has more than 2^20 hash buckets (use dm_mesh_extract)");
+    return DM_E_CAPACITY;
+  }
+  DeviceScope scope;
+  DM_TRY(scope.enter(vol->device));
+  uint8_t *base = nullptr;
+  DM_TRY(na_scratch(vol->device, &base));
+  int *counts = (int *)(base + kNaOffCounts);
+  long long *offs = (long long *)(base + kNaOffOffs), *wsum1 = (long long *)(base + kNaOffWsum1);
+  int *list = (int *)(base + kNaOffList);
+  uint8_t *cubes = base + kNaOffCubes;
+  int *btri = (int *)(base + kNaOffBtri);
+  long long *boff = (long long *)(base + kNaOffBoff), *wsum2 = (long long *)(base + kNaOffWsum2);
+
+  hipStream_t st = vol->stream;
+  BoxArg box{};
+  box.use = box_lo != nullptr;
+  for (int k = 0; k < 3 && box.use; ++k) { box.lo[k] = box_lo[k]; box.hi[k] = box_hi[k]; }
+  hipLaunchKernelGGL(k_gather_counts, dim3((nb + 255) / 256), dim3(256), 0, st, vol->d_view, counts);
+  long long *d_total = nullptr;
+  DM_TRY(scan_ints_ws(counts, nb, offs, wsum1, st, &d_total));
+  long long nblk = 0;
+  DM_HIP(hipMemcpyAsync(&nblk, d_total, 8, hipMemcpyDeviceToHost, st));
+  DM_HIP(hipStreamSynchronize(st));
+  *ntri_out = 0;
+  if (nblk == 0) return DM_OK;
+  if (nblk > vol->h.num_blocks) {
+    set_error("dm_mesh_extract_noalloc: hash table holds more entries than the pool");
+    return DM_E_HIP;
+  }
+  if (nblk > kNaMaxBlocks) {
+    set_error("dm_mesh_extract_noalloc: more than 2^21 allocated blocks (use dm_mesh_extract)");
+    return DM_E_CAPACITY;
+  }
+  const int n = (int)nblk;
+  hipLaunchKernelGGL(k_block_list, dim3((nb + 255) / 256), dim3(256), 0, st, vol->d_view, offs, list);
+  const bool want = dev_pos || dev_col || dev_edge_ids;
+  long long T = 0;
+  for (int c0 = 0; c0 < n; c0 += kNaChunk) {
+    const int cn = std::min(kNaChunk, n - c0);
+    hipLaunchKernelGGL(k_mc_count, dim3(cn), dim3(kCellThreads), 0, st, vol->d_view, list + c0, box, cubes, btri);
+    DM_HIP(hipGetLastError());
+    DM_TRY(scan_ints_ws(btri, cn, boff, wsum2, st, &d_total));
+    long long Tc = 0;
+    DM_HIP(hipMemcpyAsync(&Tc, d_total, 8, hipMemcpyDeviceToHost, st));
+    DM_HIP(hipStreamSynchronize(st));
+    const long long m = std::min<long long>(Tc, cap - std::min<long long>(T, cap));
+    if (want && m > 0) {
+      hipLaunchKernelGGL(k_mc_emit, dim3(cn), dim3(kCellThreads), 0, st, vol->d_view, list + c0, box, cubes, btri,
+                         boff, m, dev_pos ? dev_pos + (size_t)T * 9 : nullptr,
+                         dev_col ? dev_col + (size_t)T * 9 : nullptr,
+                         dev_edge_ids ? dev_edge_ids + (size_t)T * 3 : nullptr);
+      DM_HIP(hipGetLastError());
+    }
+    T += Tc;
+  }
+  DM_HIP(hipStreamSynchronize(st));
+  *ntri_out = T;
   return DM_OK;
 }
 

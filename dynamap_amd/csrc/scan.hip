@@ -58,15 +58,19 @@ int dev_alloc(DevBuf &b, size_t bytes) {
   return DM_OK;
 }
 
-int scan_ints(const int *in, int n, long long *out, DevBuf &wsum, hipStream_t st, long long **d_total) {
-  const int nw = std::max(1, (n + kScanItems - 1) / kScanItems);
-  if (int rc = dev_alloc(wsum, (size_t)(nw + 1) * 8)) return rc;
-  hipLaunchKernelGGL(k_scan_up, dim3(nw), dim3(256), 0, st, in, n, wsum.as<long long>());
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, wsum.as<long long>(), nw);
-  hipLaunchKernelGGL(k_scan_down, dim3(nw), dim3(256), 0, st, in, n, wsum.as<long long>(), out);
+int scan_ints_ws(const int *in, int n, long long *out, long long *wsum, hipStream_t st, long long **d_total) {
+  const int nw = scan_wsum_count(n) - 1;
+  hipLaunchKernelGGL(k_scan_up, dim3(nw), dim3(256), 0, st, in, n, wsum);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, wsum, nw);
+  hipLaunchKernelGGL(k_scan_down, dim3(nw), dim3(256), 0, st, in, n, wsum, out);
   DM_HIP(hipGetLastError());
-  *d_total = wsum.as<long long>() + nw;
+  *d_total = wsum + nw;
   return DM_OK;
+}
+
+int scan_ints(const int *in, int n, long long *out, DevBuf &wsum, hipStream_t st, long long **d_total) {
+  if (int rc = dev_alloc(wsum, (size_t)scan_wsum_count(n) * 8)) return rc;
+  return scan_ints_ws(in, n, out, wsum.as<long long>(), st, d_total);
 }
 
 }  // namespace dm

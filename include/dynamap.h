@@ -163,6 +163,19 @@ DM_API int dm_mc_tri_table(uint64_t out[256]);
  * may be NULL; cap 0 counts only. */
 DM_API int dm_mesh_extract(dm_volume *vol, const double box_lo[3], const double box_hi[3], int64_t cap, float *pos,
                            float *col, uint64_t *edge_ids, int64_t *ntri);
+/* dm_mesh_extract without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * triangles, order and values as dm_mesh_extract; the outputs are DEVICE
+ * pointers (pos, col 9 floats and edge_ids 3 per triangle, any may be NULL)
+ * on the volume's device.  The kernels' scratch is one fixed array per
+ * device (about 29 MiB, held until the process ends); the volume may have at
+ * most 2^20 hash buckets and 2^21 allocated blocks (DM_E_CAPACITY
+ * otherwise).  Calls that overlap on one device share that array and race:
+ * serialise them.  Calls no hipMalloc or hipFree; it synchronises the
+ * volume's stream. */
+DM_API int dm_mesh_extract_noalloc(dm_volume *vol, const double box_lo[3], const double box_hi[3], int64_t cap,
+                                   float *dev_pos, float *dev_col, uint64_t *dev_edge_ids, int64_t *ntri);
 
 #ifdef __cplusplus
 }
