@@ -73,6 +73,8 @@ PROTOTYPES = {
     "dm_volume_get_entries": (C.c_int, [_vp, _vp]),
     "dm_volume_get_blocks": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp, _P(C.c_int32)]),
     "dm_volume_counters": (C.c_int, [_vp, _vp]),
+    "dm_volume_delete_blocks": (C.c_int, [_vp, _vp, C.c_int64, _P(C.c_int64)]),
+    "dm_volume_recycle": (C.c_int, [_vp, _vp, C.c_double, _P(C.c_int64)]),
     "dm_volume_snapshot": (C.c_int, [_vp, _vp, C.c_uint64, _P(C.c_uint64)]),
     "dm_volume_restore": (C.c_int, [_vp, _vp, C.c_uint64]),
     "dm_mc_tri_table": (C.c_int, [_vp]),

@@ -5,6 +5,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include "dm_scan.h"
 #include "dm_session.h"
 #include "dm_solve.h"
 
@@ -47,8 +48,8 @@ size_t session_bytes(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   const size_t P = (size_t)rows * cols;
   const size_t nc = (size_t)new_cap(p.num_blocks);
   size_t b = (size_t)p.num_buckets * bucket_stride(p.bucket_size);
-  b += ((size_t)p.num_blocks + 1) * kBlockVox * 8 + (size_t)p.num_blocks * (16 + 1);
-  b += ((size_t)kCoarseWords + 2 * kWinWords) * 4;
+  b += ((size_t)p.num_blocks + 1) * kBlockVox * 8 + (size_t)p.num_blocks * (16 + 1 + 4);
+  b += ((size_t)kCoarseWords + 3 * kWinWords) * 4;
   b += (size_t)pow2_at_least((int)(2 * nc)) * 8 + nc * (8 + 4 * 4) + (size_t)p.num_buckets * 4;
   size_t maps = 0;
   for (int l = 0; l < kLevels; ++l) maps += (size_t)(rows >> l) * (cols >> l) * 16 * 4;
@@ -101,9 +102,11 @@ int Sess::init(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   // for unallocated blocks in the raycast's gathers
   DM_TRY(dalloc(&h.vox, ((size_t)p.num_blocks + 1) * kBlockVox));
   DM_TRY(dalloc(&h.block_key, (size_t)p.num_blocks));
+  DM_TRY(dalloc(&h.free_list, (size_t)p.num_blocks, false));
   DM_TRY(dalloc(&h.blk_near, (size_t)p.num_blocks));
   DM_TRY(dalloc(&h.nearmap, kWinWords));
   DM_TRY(dalloc(&h.dilmap, kWinWords));
+  DM_TRY(dalloc(&h.allocmap, kWinWords));
   DM_TRY(dalloc(&h.coarse, (size_t)kCoarseWords));
   DM_TRY(dalloc(&h.dset, (size_t)dsz));
   DM_TRY(dalloc(&h.newkeys, (size_t)h.newcap, false));
@@ -322,16 +325,17 @@ struct SnapHeader {
   int32_t ctr[C_COUNT];
   uint64_t payload;
 };
-static const char kSnapMagic[8] = {'D', 'M', 'S', 'N', 'A', 'P', '0', '3'};
-static const uint32_t kSnapVersion = 3;
+static const char kSnapMagic[8] = {'D', 'M', 'S', 'N', 'A', 'P', '0', '4'};
+static const uint32_t kSnapVersion = 4;
 
 // device sections of a volume / session state, in file order; the raycast's
 // bitmaps are not stored (restore rebuilds them from blk_near and the keys)
-static std::vector<std::pair<void *, size_t>> snap_sections(const Sess &S, int nalloc) {
+static std::vector<std::pair<void *, size_t>> snap_sections(const Sess &S, int nalloc, int nfree) {
   const DmView &h = S.h;
   std::vector<std::pair<void *, size_t>> v = {
       {h.buckets, (size_t)h.num_buckets * h.bstride}, {h.block_key, (size_t)nalloc * sizeof(int4)},
       {h.vox, (size_t)nalloc * kBlockVox * 8},        {h.blk_near, (size_t)nalloc},
+      {h.free_list, (size_t)nfree * 4},
       {h.posed, (size_t)PD_PERSIST * 8},             {h.posef, (size_t)PF_COUNT * 4}};
   for (int l = 0; l < kLevels; ++l) {
     const size_t Pl = (size_t)(h.rows >> l) * (h.cols >> l);
@@ -431,9 +435,69 @@ int dm_volume_raycast(dm_volume *vol, const float pose[16], const dm_sensor *sen
 
 int dm_volume_num_allocated(dm_volume *vol, int32_t *n) {
   if (!vol || !n) { set_error("null argument"); return DM_E_INVALID; }
-  DM_HIP(hipMemcpyAsync(n, vol->h.ctr + C_NUM_ALLOC, 4, hipMemcpyDeviceToHost, vol->stream));
+  int32_t c[C_COUNT];
+  DM_HIP(hipMemcpyAsync(c, vol->h.ctr, sizeof c, hipMemcpyDeviceToHost, vol->stream));
   DM_HIP(hipStreamSynchronize(vol->stream));
+  *n = c[C_NUM_ALLOC] - c[C_NUM_FREE];
   return DM_OK;
+}
+
+// HashTable::DeleteBlock for a list of blocks (one device pass; see
+// launch_delete), then the raycast's bitmaps are rebuilt.
+static int delete_list(Sess &S, const int4 *d_req, int n, int64_t *deleted) {
+  DevBuf dmask, touched, counts;
+  DM_TRY(dev_alloc(dmask, (size_t)S.props.num_buckets * 8));
+  DM_TRY(dev_alloc(touched, (size_t)std::max(n, 1) * 4));
+  DM_TRY(dev_alloc(counts, 8));
+  DM_HIP(hipMemsetAsync(dmask.p, 0, (size_t)S.props.num_buckets * 8, S.stream));
+  DM_HIP(hipMemsetAsync(counts.p, 0, 8, S.stream));
+  launch_delete(S.d_view, d_req, n, dmask.as<unsigned long long>(), touched.as<int>(), counts.as<int>(), S.stream);
+  DM_HIP(hipGetLastError());
+  int c[2] = {0, 0};
+  DM_HIP(hipMemcpyAsync(c, counts.p, 8, hipMemcpyDeviceToHost, S.stream));
+  DM_HIP(hipStreamSynchronize(S.stream));
+  if (c[1] > 0) {
+    rebuild_far_maps(S.launch());
+    DM_HIP(hipGetLastError());
+    DM_HIP(hipStreamSynchronize(S.stream));
+  }
+  if (deleted) *deleted = c[1];
+  return DM_OK;
+}
+
+int dm_volume_delete_blocks(dm_volume *vol, const int32_t *xyz, int64_t n, int64_t *deleted) {
+  if (!vol || (n > 0 && !xyz) || n < 0 || n > (1LL << 28)) {
+    set_error("dm_volume_delete_blocks: invalid argument");
+    return DM_E_INVALID;
+  }
+  if (deleted) *deleted = 0;
+  if (n == 0) return DM_OK;
+  std::vector<int4> req((size_t)n);
+  for (int64_t i = 0; i < n; ++i) req[i] = make_int4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0);
+  DevBuf d_req;
+  DM_TRY(dev_alloc(d_req, (size_t)n * sizeof(int4)));
+  DM_HIP(hipMemcpyAsync(d_req.p, req.data(), (size_t)n * sizeof(int4), hipMemcpyHostToDevice, vol->stream));
+  return delete_list(*vol, d_req.as<int4>(), (int)n, deleted);
+}
+
+int dm_volume_recycle(dm_volume *vol, const double center[3], double radius, int64_t *deleted) {
+  if (!vol || !center || !(radius >= 0.0)) {
+    set_error("dm_volume_recycle: invalid argument");
+    return DM_E_INVALID;
+  }
+  Sess &S = *vol;
+  DevBuf d_req, d_n;
+  DM_TRY(dev_alloc(d_req, (size_t)S.props.num_blocks * sizeof(int4)));
+  DM_TRY(dev_alloc(d_n, 4));
+  DM_HIP(hipMemsetAsync(d_n.p, 0, 4, S.stream));
+  launch_select_far(S.d_view, center, radius, d_req.as<int4>(), d_n.as<int>(), S.stream);
+  DM_HIP(hipGetLastError());
+  int n = 0;
+  DM_HIP(hipMemcpyAsync(&n, d_n.p, 4, hipMemcpyDeviceToHost, S.stream));
+  DM_HIP(hipStreamSynchronize(S.stream));
+  if (deleted) *deleted = 0;
+  if (n == 0) return DM_OK;
+  return delete_list(S, d_req.as<int4>(), n, deleted);
 }
 
 int dm_volume_num_entries(dm_volume *vol, int64_t *n) {
@@ -472,27 +536,31 @@ int dm_volume_get_entries(dm_volume *vol, int32_t *out) {
 int dm_volume_get_blocks(dm_volume *vol, int32_t cap, int32_t *xyz, float *sdf, uint8_t *rgb, uint8_t *weight,
                          int32_t *n_out) {
   if (!vol) { set_error("null argument"); return DM_E_INVALID; }
-  int n = 0;
-  DM_HIP(hipMemcpyAsync(&n, vol->h.ctr + C_NUM_ALLOC, 4, hipMemcpyDeviceToHost, vol->stream));
+  int32_t c[C_COUNT];
+  DM_HIP(hipMemcpyAsync(c, vol->h.ctr, sizeof c, hipMemcpyDeviceToHost, vol->stream));
   DM_HIP(hipStreamSynchronize(vol->stream));
+  const int hw = c[C_NUM_ALLOC], n = c[C_NUM_ALLOC] - c[C_NUM_FREE];
   if (n_out) *n_out = n;
   const int m = std::min(n, cap);
   if (m <= 0) return DM_OK;
-  std::vector<int4> keys(m);
-  std::vector<uint64_t> vox((size_t)m * kBlockVox);
-  DM_HIP(hipMemcpyAsync(keys.data(), vol->h.block_key, (size_t)m * 16, hipMemcpyDeviceToHost, vol->stream));
+  std::vector<int4> keys(hw);
+  std::vector<uint64_t> vox((size_t)hw * kBlockVox);
+  DM_HIP(hipMemcpyAsync(keys.data(), vol->h.block_key, (size_t)hw * 16, hipMemcpyDeviceToHost, vol->stream));
   if (sdf || rgb || weight)
     DM_HIP(hipMemcpyAsync(vox.data(), vol->h.vox, vox.size() * 8, hipMemcpyDeviceToHost, vol->stream));
   DM_HIP(hipStreamSynchronize(vol->stream));
-  for (int b = 0; b < m; ++b) {
-    if (xyz) { xyz[3 * b] = keys[b].x; xyz[3 * b + 1] = keys[b].y; xyz[3 * b + 2] = keys[b].z; }
+  // live blocks in pool order (freed pool indices skipped)
+  for (int p = 0, b = 0; p < hw && b < m; ++p) {
+    if (keys[p].w == 0) continue;
+    if (xyz) { xyz[3 * b] = keys[p].x; xyz[3 * b + 1] = keys[p].y; xyz[3 * b + 2] = keys[p].z; }
     for (int i = 0; i < kBlockVox; ++i) {
       const size_t vi = (size_t)b * kBlockVox + i;
-      const uint64_t w = vox[vi];
+      const uint64_t w = vox[(size_t)p * kBlockVox + i];
       if (sdf) { uint32_t lo = (uint32_t)w; memcpy(&sdf[vi], &lo, 4); }
       if (rgb) { rgb[3 * vi] = (uint8_t)(w >> 32); rgb[3 * vi + 1] = (uint8_t)(w >> 40); rgb[3 * vi + 2] = (uint8_t)(w >> 48); }
       if (weight) weight[vi] = (uint8_t)(w >> 56);
     }
+    ++b;
   }
   return DM_OK;
 }
@@ -502,7 +570,7 @@ int dm_volume_counters(dm_volume *vol, int64_t out[5]) {
   int32_t c[C_COUNT];
   DM_HIP(hipMemcpyAsync(c, vol->h.ctr, sizeof c, hipMemcpyDeviceToHost, vol->stream));
   DM_HIP(hipStreamSynchronize(vol->stream));
-  out[0] = c[C_NUM_ALLOC]; out[1] = c[C_DROP_BUCKET]; out[2] = c[C_DROP_POOL];
+  out[0] = c[C_NUM_ALLOC] - c[C_NUM_FREE]; out[1] = c[C_DROP_BUCKET]; out[2] = c[C_DROP_POOL];
   out[3] = c[C_DROP_RANGE]; out[4] = c[C_DSET_OVF];
   return DM_OK;
 }
@@ -526,7 +594,7 @@ int dm_volume_snapshot(dm_volume *vol, void *buf, uint64_t cap, uint64_t *size) 
   const float f7[7] = {S.h.icp_dmin, S.h.icp_dmax, S.h.dist_max, S.h.w_photo, S.h.lambda, S.h.max_inc,
                        S.h.min_inc};
   memcpy(H.icp_f, f7, sizeof f7);
-  const auto secs = snap_sections(S, H.num_alloc);
+  const auto secs = snap_sections(S, H.num_alloc, H.ctr[C_NUM_FREE]);
   uint64_t total = sizeof(SnapHeader);
   for (const auto &x : secs) total += x.second;
   H.payload = total - sizeof(SnapHeader);
@@ -557,13 +625,13 @@ int dm_volume_restore(dm_volume *vol, const void *buf, uint64_t size) {
   }
   if (memcmp(&H.props, &S.props, sizeof(dm_tsdf_props)) != 0 || H.max_rows != S.max_rows ||
       H.max_cols != S.max_cols || H.num_alloc < 0 || H.num_alloc > S.props.num_blocks ||
-      H.rows > S.max_rows || H.cols > S.max_cols) {
+      H.ctr[C_NUM_FREE] < 0 || H.ctr[C_NUM_FREE] > H.num_alloc || H.rows > S.max_rows || H.cols > S.max_cols) {
     set_error("dm_volume_restore: properties or frame size differ from the snapshot's");
     return DM_E_INVALID;
   }
   S.h.rows = (int)H.rows; S.h.cols = (int)H.cols;
   S.h.fx = H.fx; S.h.fy = H.fy; S.h.cx = H.cx; S.h.cy = H.cy;
-  const auto secs = snap_sections(S, H.num_alloc);
+  const auto secs = snap_sections(S, H.num_alloc, H.ctr[C_NUM_FREE]);
   uint64_t total = sizeof(SnapHeader);
   for (const auto &x : secs) total += x.second;
   if (size != total || H.payload != total - sizeof(SnapHeader)) {

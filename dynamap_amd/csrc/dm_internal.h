@@ -29,7 +29,8 @@ constexpr size_t kWinWords = (size_t)kWin * kWin * kWin / 32;
 
 // per-volume counters, int32 each
 enum Ctr : int {
-  C_NUM_ALLOC = 0,   // allocated blocks (pool indices 0..n-1 are in use)
+  C_NUM_ALLOC = 0,   // pool high-water mark: indices 0..n-1 have been used (live, or freed
+                     // and listed in free_list); live blocks = C_NUM_ALLOC - C_NUM_FREE
   C_N_NEW = 1,       // new keys collected this frame
   C_N_INS = 2,       // of which inserted
   C_CURSOR = 3,      // pool cursor within the frame
@@ -43,6 +44,7 @@ enum Ctr : int {
   C_NVISIT = 11,     // block visits of the last allocation's ray traversal
   C_NNEW = 12,       // keys inserted by the last allocation
   C_ICP_STOP = 13,   // level + 1 of the ICP level that ended early this frame, 0 = none
+  C_NUM_FREE = 14,   // freed pool indices waiting in free_list (a stack; reused first)
   C_COUNT = 16
 };
 
@@ -71,11 +73,15 @@ struct DmView {
   // volume arrays
   uint8_t *buckets;     // num_buckets * bstride: u64 key[S] | i32 ptr[S] | i32 count
   uint64_t *vox;        // num_blocks * 512 voxel words (sdf bits | r | g<<8 | b<<16 | w<<24 <<32)
-  int4 *block_key;      // num_blocks block coordinates
+  int4 *block_key;      // num_blocks block coordinates, w = 1 live, 0 freed
+  int32_t *free_list;   // num_blocks: pool indices returned by deletion (HashTable::DeleteBlock
+                        // + Heap::Append), taken back by later insertions
   uint8_t *blk_near;    // num_blocks: 1 when an observed voxel lies below the raycast far threshold
   uint32_t *nearmap;    // kWinWords: window bit of every block with blk_near set
   uint32_t *dilmap;     // kWinWords: window bit of every block within one block of a near block
   uint32_t *coarse;     // kCoarse^3 bits: cell of 8^3 blocks holds a block with its dilmap bit set
+  uint32_t *allocmap;   // kWinWords: window bit of every allocated block (the allocation's
+                        // table test; blocks outside the window go through the hash table)
   uint64_t *dset;       // per-frame new-key set, dset_mask + 1 slots, 0 = empty
   uint64_t *newkeys;    // newcap
   int32_t *newdslot;    // newcap
@@ -131,7 +137,17 @@ void launch_alloc(const Launch &L);
 void launch_alloc_rays(const Launch &L, bool count_unique = false);
 void launch_integrate(const Launch &L);
 void launch_raycast(const Launch &L);
-void rebuild_far_maps(const Launch &L);  // clear and rebuild nearmap/dilmap/coarse (not for graph capture)
+// HashTable::DeleteBlock over a list of n block coordinates (one session,
+// view = its one-entry table): dmask (num_buckets 64-bit words, one bit per
+// slot) must be zero and is left zero; touched holds n ints; counts =
+// {touched buckets, freed blocks}, zero on entry.  Rebuild the raycast's
+// bitmaps afterwards.
+void launch_delete(const DmView *view, const int4 *req, int n, unsigned long long *dmask, int *touched, int *counts,
+                   hipStream_t st);
+// live blocks whose centre lies farther than radius from c, appended to req
+// (capacity: the pool size) with *nreq counting them
+void launch_select_far(const DmView *view, const double c[3], double radius, int4 *req, int *nreq, hipStream_t st);
+void rebuild_far_maps(const Launch &L);  // clear and rebuild nearmap/dilmap/coarse/allocmap (not for graph capture)
 void launch_source_maps(const Launch &L);
 // ICP linearisation at one level (one FP32 record per workgroup); with solve,
 // a second kernel sums the records and takes the Gauss-Newton step on the

@@ -71,6 +71,7 @@ int sess_reset(Sess &S, const double init_pose[16]) {
   DM_HIP(hipMemsetAsync(S.h.blk_near, 0, (size_t)n, S.stream));
   DM_HIP(hipMemsetAsync(S.h.nearmap, 0, kWinWords * 4, S.stream));
   DM_HIP(hipMemsetAsync(S.h.dilmap, 0, kWinWords * 4, S.stream));
+  DM_HIP(hipMemsetAsync(S.h.allocmap, 0, kWinWords * 4, S.stream));
   DM_HIP(hipMemsetAsync(S.h.coarse, 0, (size_t)kCoarseWords * 4, S.stream));
   DM_HIP(hipMemsetAsync(S.h.ctr, 0, C_COUNT * 4, S.stream));
   double pd[PD_COUNT] = {0};
@@ -641,7 +642,7 @@ int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage
       for (int t : act) {
         int32_t c[C_COUNT];
         DM_HIP(hipMemcpy(c, srv->traces[t].sess->h.ctr, sizeof c, hipMemcpyDeviceToHost));
-        sum_vis += c[C_NVIS]; sum_visit += c[C_NVISIT]; sum_alloc += c[C_NUM_ALLOC];
+        sum_vis += c[C_NVIS]; sum_visit += c[C_NVISIT]; sum_alloc += c[C_NUM_ALLOC] - c[C_NUM_FREE];
       }
       if (st == 4) { work[1] = sum_vis / reps; work[5] = sum_alloc; }
       if (st == 6) { work[2] = sum_visit / reps; }

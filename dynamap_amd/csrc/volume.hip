@@ -24,6 +24,16 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
   return x;
 }
 
+// Bit of block (x, y, z) in the dense window bitmaps (kWin^3 blocks centred
+// on block (0, 0, 0), x fastest); false outside the window.
+__device__ __forceinline__ bool win_bit(int x, int y, int z, uint32_t &idx) {
+  const unsigned X = (unsigned)(x + kWin / 2), Y = (unsigned)(y + kWin / 2), Z = (unsigned)(z + kWin / 2);
+  if (X >= (unsigned)kWin || Y >= (unsigned)kWin || Z >= (unsigned)kWin) return false;
+  idx = (Z * (unsigned)kWin + Y) * (unsigned)kWin + X;
+  return true;
+}
+__device__ __forceinline__ bool test_bit(const uint32_t *map, uint32_t idx) { return (ldg(map + (idx >> 5)) >> (idx & 31)) & 1u; }
+
 // Record a block that is missing from the table in the per-frame key set.
 // A plain read first: once a key is in the set, later visitors return
 // without an atomic (a missing free-space block near the camera is visited
@@ -64,9 +74,16 @@ constexpr int kSeenProbes = 8;
 // kCount (profiling only): also mark the pool index of every block found in
 // the table in v.visited, so that the distinct blocks of a traversal can be
 // counted.
+// Inside the window the allocated-block bitmap answers "is it in the
+// table" with one (mostly cached) word read, and only blocks missing from it
+// reach the workgroup set and the per-frame key set; outside the window the
+// hash table is probed.
 template <bool kCount>
 __device__ __forceinline__ void visit_block(const DmView &v, int x, int y, int z, unsigned long long *seen) {
   if (!in_key_range(x, y, z)) { atomicAdd(&v.ctr[C_DROP_RANGE], 1); return; }
+  uint32_t widx;
+  const bool in_win = !kCount && win_bit(x, y, z, widx);
+  if (in_win && test_bit(v.allocmap, widx)) return;
   const uint64_t key = pack_key(x, y, z);
   uint32_t h = (uint32_t)mix64(key) & (kSeenSlots - 1);
   for (int probe = 0; probe < kSeenProbes; ++probe) {
@@ -79,10 +96,12 @@ __device__ __forceinline__ void visit_block(const DmView &v, int x, int y, int z
     }
     h = (h + 1u) & (kSeenSlots - 1);
   }
-  const int ptr = find_block(v, x, y, z);
-  if (ptr >= 0) {
-    if (kCount) atomicOr(&v.visited[ptr >> 5], 1u << (ptr & 31));
-    return;
+  if (!in_win) {
+    const int ptr = find_block(v, x, y, z);
+    if (ptr >= 0) {
+      if (kCount) atomicOr(&v.visited[ptr >> 5], 1u << (ptr & 31));
+      return;
+    }
   }
   note_new_key(v, key);
 }
@@ -240,7 +259,8 @@ __global__ __launch_bounds__(256) void k_alloc_commit(const DmView *__restrict__
   const int n = min(v.ctr[C_N_NEW], v.newcap);
   const int nalloc = v.ctr[C_NUM_ALLOC];
   const int nins = v.ctr[C_N_INS];
-  if (nalloc + nins > v.num_blocks) return;
+  const int nfree = v.ctr[C_NUM_FREE];
+  if (nalloc + max(0, nins - nfree) > v.num_blocks) return;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     const int rank = v.newrank[i];
     if (rank < 0) continue;
@@ -250,10 +270,15 @@ __global__ __launch_bounds__(256) void k_alloc_commit(const DmView *__restrict__
     const int b = hash_bucket(x, y, z, v.num_buckets);
     uint8_t *bk = v.buckets + (size_t)b * (size_t)v.bstride;
     const int slot = *(const int32_t *)(bk + 12 * v.bucket_size) + rank;
-    const int ptr = nalloc + atomicAdd(&v.ctr[C_CURSOR], 1);
+    // freed pool indices first, top of the stack down (Heap::Consume,
+    // heap.cu:14-17), then fresh ones
+    const int q = atomicAdd(&v.ctr[C_CURSOR], 1);
+    const int ptr = q < nfree ? v.free_list[nfree - 1 - q] : nalloc + (q - nfree);
     ((uint64_t *)bk)[slot] = key;
     ((int32_t *)(bk + 8 * v.bucket_size))[slot] = ptr;
     v.block_key[ptr] = make_int4(x, y, z, 1);
+    uint32_t widx;
+    if (win_bit(x, y, z, widx)) atomicOr(&v.allocmap[widx >> 5], 1u << (widx & 31));
   }
 }
 
@@ -263,7 +288,7 @@ __global__ __launch_bounds__(256) void k_alloc_finish(const DmView *__restrict__
   if (slot_ < 0) return;
   const DmView &v = views[slot_];
   const int n = min(v.ctr[C_N_NEW], v.newcap);
-  const bool pool_ok = v.ctr[C_NUM_ALLOC] + v.ctr[C_N_INS] <= v.num_blocks;
+  const bool pool_ok = v.ctr[C_NUM_ALLOC] + max(0, v.ctr[C_N_INS] - v.ctr[C_NUM_FREE]) <= v.num_blocks;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     const uint64_t key = v.newkeys[i];
     const int b = key_bucket(v, key);
@@ -286,8 +311,10 @@ __global__ void k_alloc_close(const DmView *__restrict__ views, const int *__res
   const int n_all = v.ctr[C_N_NEW];
   const int n = min(n_all, v.newcap);
   const int nins = v.ctr[C_N_INS];
-  if (v.ctr[C_NUM_ALLOC] + nins <= v.num_blocks) {
-    v.ctr[C_NUM_ALLOC] += nins;
+  const int nfree = v.ctr[C_NUM_FREE];
+  if (v.ctr[C_NUM_ALLOC] + max(0, nins - nfree) <= v.num_blocks) {
+    v.ctr[C_NUM_ALLOC] += max(0, nins - nfree);
+    v.ctr[C_NUM_FREE] = max(0, nfree - nins);
     v.ctr[C_DROP_BUCKET] += n - nins;
     v.ctr[C_NNEW] = nins;
   } else {
@@ -382,13 +409,6 @@ __device__ __forceinline__ bool coarse_bit(int cx, int cy, int cz, int &word, in
 }
 
 // Bit of block (x, y, z) in the block window, false outside it.
-__device__ __forceinline__ bool win_bit(int x, int y, int z, uint32_t &idx) {
-  const unsigned X = (unsigned)(x + kWin / 2), Y = (unsigned)(y + kWin / 2), Z = (unsigned)(z + kWin / 2);
-  if (X >= (unsigned)kWin || Y >= (unsigned)kWin || Z >= (unsigned)kWin) return false;
-  idx = (Z * (unsigned)kWin + Y) * (unsigned)kWin + X;
-  return true;
-}
-__device__ __forceinline__ bool test_bit(const uint32_t *map, uint32_t idx) { return (ldg(map + (idx >> 5)) >> (idx & 31)) & 1u; }
 
 // Row (dy, dz) = (row % 3 - 1, row / 3 - 1) of the 3^3 neighbourhood of block
 // (x, y, z) in the window: bits x-1..x+1 (clipped to the window) lie in words
@@ -430,6 +450,7 @@ __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ vi
   int nvis = 0;
   for (int blk = blockIdx.x; blk < nalloc; blk += gridDim.x) {
     const int4 key = v.block_key[blk];
+    if (key.w == 0) continue;  // freed pool index (uniform per workgroup)
     if (threadIdx.x < 10 && v.blk_near[blk]) {
       if (threadIdx.x < 9) {
         uint32_t w0, w1, m0, m1;
@@ -499,6 +520,125 @@ __global__ __launch_bounds__(256) void k_dilate_near(const DmView *__restrict__ 
   }
 }
 
+// Window bit of every allocated block in the allocation bitmap (rebuild).
+__global__ __launch_bounds__(256) void k_mark_allocated(const DmView *__restrict__ views,
+                                                       const int *__restrict__ active) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  const int nalloc = v.ctr[C_NUM_ALLOC];
+  for (int blk = blockIdx.x * 256 + threadIdx.x; blk < nalloc; blk += gridDim.x * 256) {
+    const int4 key = v.block_key[blk];
+    uint32_t idx;
+    if (key.w != 0 && win_bit(key.x, key.y, key.z, idx)) atomicOr(&v.allocmap[idx >> 5], 1u << (idx & 31));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block deletion (HashTable::DeleteBlock, hashTable.cu:112-134, with
+// Heap::Append, heap.cu:19-22).  D: a bucket keeps its occupied slots as a
+// prefix, so the survivors of a bucket close up in their order (a stable
+// compaction), which keeps the table canonical; freed pool indices go on a
+// stack that later insertions take first.  Three kernels per request list:
+// mark (one thread per requested block), compact (one thread per touched
+// bucket), clear (one workgroup per freed block).
+
+// One requested block: mark its slot for removal.  dmask (one word per
+// bucket, zero outside a request) collects the slots; the first mark of a
+// bucket lists it in touched.  Freed pool indices are pushed on free_list.
+__global__ __launch_bounds__(256) void k_del_mark(const DmView *__restrict__ views, const int4 *__restrict__ req,
+                                                 int n, unsigned long long *__restrict__ dmask,
+                                                 int *__restrict__ touched, int *__restrict__ counts) {
+  const DmView &v = views[0];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int4 q = req[i];
+  if (!in_key_range(q.x, q.y, q.z)) return;
+  const uint64_t key = pack_key(q.x, q.y, q.z);
+  const int b = hash_bucket(q.x, q.y, q.z, v.num_buckets);
+  const uint8_t *bk = bucket_ptr(v, b);
+  const int cnt = bucket_count(v, bk);
+  for (int s = 0; s < cnt; ++s) {
+    if (((const uint64_t *)bk)[s] != key) continue;
+    const unsigned long long bit = 1ull << s;
+    const unsigned long long old = atomicOr(&dmask[b], bit);
+    if (old & bit) return;  // the same block listed twice
+    if (old == 0ull) touched[atomicAdd(&counts[0], 1)] = b;
+    const int ptr = ((const int32_t *)(bk + 8 * v.bucket_size))[s];
+    v.block_key[ptr].w = 0;
+    v.free_list[v.ctr[C_NUM_FREE] + atomicAdd(&counts[1], 1)] = ptr;
+    return;
+  }
+}
+
+// Stable compaction of each touched bucket's occupied prefix.
+__global__ __launch_bounds__(256) void k_del_compact(const DmView *__restrict__ views,
+                                                    unsigned long long *__restrict__ dmask,
+                                                    const int *__restrict__ touched, const int *__restrict__ counts) {
+  const DmView &v = views[0];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= counts[0]) return;
+  const int b = touched[i];
+  uint8_t *bk = v.buckets + (size_t)b * (size_t)v.bstride;
+  uint64_t *keys = (uint64_t *)bk;
+  int32_t *ptrs = (int32_t *)(bk + 8 * v.bucket_size);
+  int32_t *cnt = (int32_t *)(bk + 12 * v.bucket_size);
+  const unsigned long long m = dmask[b];
+  const int c = *cnt;
+  int o = 0;
+  for (int s = 0; s < c; ++s) {
+    if ((m >> s) & 1ull) continue;
+    keys[o] = keys[s];
+    ptrs[o] = ptrs[s];
+    ++o;
+  }
+  for (int s = o; s < c; ++s) { keys[s] = 0ull; ptrs[s] = 0; }
+  *cnt = o;
+  dmask[b] = 0ull;
+}
+
+// Zero the freed blocks' voxels and flags (DeleteBlock's voxel reset) and
+// clear their allocation bits; then the counters.
+__global__ __launch_bounds__(256) void k_del_clear(const DmView *__restrict__ views, const int *__restrict__ counts) {
+  const DmView &v = views[0];
+  const int nfreed = counts[1];
+  const int base = v.ctr[C_NUM_FREE];
+  for (int k = blockIdx.x; k < nfreed; k += gridDim.x) {
+    const int ptr = v.free_list[base + k];
+    uint64_t *bv = v.vox + (size_t)ptr * kBlockVox;
+    bv[threadIdx.x] = 0ull;
+    bv[threadIdx.x + 256] = 0ull;
+    if (threadIdx.x == 0) {
+      v.blk_near[ptr] = 0;
+      const int4 key = v.block_key[ptr];
+      uint32_t idx;
+      if (win_bit(key.x, key.y, key.z, idx)) atomicAnd(&v.allocmap[idx >> 5], ~(1u << (idx & 31)));
+    }
+  }
+}
+
+__global__ void k_del_close(const DmView *__restrict__ views, int *__restrict__ counts) {
+  const DmView &v = views[0];
+  if (threadIdx.x != 0) return;
+  v.ctr[C_NUM_FREE] += counts[1];
+}
+
+// Recycling selection: every live block whose centre ((key + 1/2) * 8 * vs)
+// lies farther than radius from center, listed in req.
+__global__ __launch_bounds__(256) void k_select_far(const DmView *__restrict__ views, double cx, double cy, double cz,
+                                                   double r2, int4 *__restrict__ req, int *__restrict__ nreq) {
+  const DmView &v = views[0];
+  const int nalloc = v.ctr[C_NUM_ALLOC];
+  const double bm = 8.0 * (double)v.voxel_size;
+  for (int blk = blockIdx.x * 256 + threadIdx.x; blk < nalloc; blk += gridDim.x * 256) {
+    const int4 key = v.block_key[blk];
+    if (key.w == 0) continue;
+    const double dx = ((double)key.x + 0.5) * bm - cx, dy = ((double)key.y + 0.5) * bm - cy,
+                 dz = ((double)key.z + 0.5) * bm - cz;
+    if (dx * dx + dy * dy + dz * dz > r2) req[atomicAdd(nreq, 1)] = key;
+  }
+}
+
 // Clears every bitmap word (grid-stride) before a full rebuild.
 __global__ __launch_bounds__(256) void k_clear_far_maps(const DmView *__restrict__ views,
                                                         const int *__restrict__ active) {
@@ -509,6 +649,7 @@ __global__ __launch_bounds__(256) void k_clear_far_maps(const DmView *__restrict
   for (size_t w = (size_t)blockIdx.x * 256 + threadIdx.x; w < kWinWords; w += stride) {
     v.nearmap[w] = 0u;
     v.dilmap[w] = 0u;
+    v.allocmap[w] = 0u;
   }
   for (size_t w = (size_t)blockIdx.x * 256 + threadIdx.x; w < (size_t)kCoarseWords; w += stride) v.coarse[w] = 0u;
 }
@@ -563,12 +704,15 @@ struct AxisSample {
 // vs) are floor(q/vs) and floor(q/vs) + 1 in exact arithmetic (the +-1/2 of
 // the dual offset and of the rounding cancel, on either sign); with the
 // float roundings they still are unless q/vs lies within a few ulp of an
-// integer, so they are taken from vq = q / vs when its fraction is more than
-// |vq| * 1e-6 + 1e-6 from 0 and 1, and evaluated otherwise.
+// integer.  vq = q * (1/vs) is within 1.5 ulp of q / vs, far inside the
+// margin |vq| * 1e-6 + 1e-6, so the corners are taken from floor(vq) when its
+// fraction is further than that from 0 and 1 and evaluated exactly
+// otherwise; the centre voxel is world_to_voxel_fast, exact by the same
+// argument.  Only the weight carries vq's rounding (real-valued outputs).
 __device__ __forceinline__ AxisSample axis_sample(float q, float vs, float inv_vs) {
   AxisSample a;
-  const float vq = q / vs;
-  a.c = to_i32(vq + copysignf(0.5f, q));  // = signf_(q) * 0.5f but at q = +-0, where both give 0
+  const float vq = q * inv_vs;
+  a.c = world_to_voxel_fast(q, vs, inv_vs);
   const float fl = floorf(vq);
   a.w = vq - fl;
   const float tol = fabsf(vq) * 1.0e-6f + 1.0e-6f;
@@ -581,6 +725,17 @@ __device__ __forceinline__ AxisSample axis_sample(float q, float vs, float inv_v
     a.i1 = world_to_voxel_fast(dual + vs, vs, inv_vs);
   }
   return a;
+}
+
+// getPoint3d of pixel (u, v) at depth z with the focal lengths' reciprocals
+// (the raycast's sample and hit positions: a few ulp from the quotients,
+// only real-valued outputs depend on them)
+__device__ __forceinline__ f3 point3d_rc(int u, int v, float z, const Intr &s, float ifx, float ify) {
+  f3 p;
+  p.z = z;
+  p.x = (((float)u - s.cx) * z) * ifx;
+  p.y = (((float)v - s.cy) * z) * ify;
+  return p;
 }
 
 __device__ __forceinline__ bool interp_axes(const DmView &v, const AxisSample &X, const AxisSample &Y,
@@ -682,9 +837,9 @@ __device__ __forceinline__ bool interp_axes(const DmView &v, const AxisSample &X
   return true;
 }
 
-__device__ __forceinline__ bool interp_voxel_win(const DmView &v, f3 p, BlockWin &W, float &sdf_out, float *col) {
+__device__ __forceinline__ bool interp_voxel_win(const DmView &v, f3 p, float inv_vs, BlockWin &W, float &sdf_out,
+                                                 float *col) {
   const float vs = v.voxel_size;
-  const float inv_vs = 1.0f / vs;
   return interp_axes(v, axis_sample(p.x, vs, inv_vs), axis_sample(p.y, vs, inv_vs), axis_sample(p.z, vs, inv_vs), W,
                      sdf_out, col);
 }
@@ -796,9 +951,10 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
   const int it_last = max_steps - 1;
   const float zpre_max = nextafterf(v.max_depth, -INFINITY);  // largest float below max_depth
   const float inv_vs = 1.0f / vs;
+  const float ifx = 1.0f / s.fx, ify = 1.0f / s.fy;
   // ray in voxel units for the approximate empty-space test
   const f3 po = mk(T.m[3], T.m[7], T.m[11]) * inv_vs;
-  const f3 pd = rot(T, mk(((float)u - s.cx) / s.fx, ((float)r - s.cy) / s.fy, 1.0f)) * inv_vs;
+  const f3 pd = rot(T, mk(((float)u - s.cx) * ifx, ((float)r - s.cy) * ify, 1.0f)) * inv_vs;
   __shared__ int win_lds[8 * 256];
   BlockWin c;
   c.wx = 0x3fffffff; c.wy = 0; c.wz = 0;
@@ -823,16 +979,16 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
       }
       continue;
     }
-    const f3 p = xform(T, get_point3d_uv(u, r, z, s));
-    if (!interp_voxel_win(v, p, c, sdf, nullptr)) { pobs = false; prev_fast = false; zprev = z; z = z + big; continue; }
+    const f3 p = xform(T, point3d_rc(u, r, z, s, ifx, ify));
+    if (!interp_voxel_win(v, p, inv_vs, c, sdf, nullptr)) { pobs = false; prev_fast = false; zprev = z; z = z + big; continue; }
     if (sdf <= 0.f) {
       if (prev_fast) {
         float s2;
-        pobs = interp_voxel_win(v, xform(T, get_point3d_uv(u, r, zprev, s)), c, s2, nullptr);
+        pobs = interp_voxel_win(v, xform(T, point3d_rc(u, r, zprev, s, ifx, ify)), inv_vs, c, s2, nullptr);
         ps = s2;
         pz = zprev;
       }
-      if (pobs && ps > 0.f) { zh = pz + (z - pz) * (ps / (ps - sdf)); hit = true; }
+      if (pobs && ps > 0.f) { zh = pz + (z - pz) * (ps * __builtin_amdgcn_rcpf(ps - sdf)); hit = true; }
       break;
     }
     pobs = true; ps = sdf; pz = z; prev_fast = false;
@@ -846,7 +1002,7 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
   uint32_t cr = 0, cg = 0, cb = 0;
   float dout = 0.f;
   if (hit) {
-    const f3 pc = get_point3d_uv(u, r, zh, s);
+    const f3 pc = point3d_rc(u, r, zh, s, ifx, ify);
     const f3 pw = xform(T, pc);
     dout = zh;
     // The six central-difference samples pw +- vs e_a and the colour sample
@@ -854,7 +1010,6 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
     // without offset (pw.y + 0.0f for the x samples) gives the same axis
     // sample as pw itself, so 9 axis samples serve the 7 interpolations.
     // One (not unrolled) copy of the gather keeps the kernel's code small.
-    const float inv_vs = 1.0f / vs;
     const AxisSample X0 = axis_sample(pw.x, vs, inv_vs), Y0 = axis_sample(pw.y, vs, inv_vs),
                      Z0 = axis_sample(pw.z, vs, inv_vs);
     float gx = 0.f, gy = 0.f, gz = 0.f;
@@ -875,7 +1030,10 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
     }
     if (ok) {
       const M4 Ti = load_m4(v.posef + PF_TINV);
-      const f3 n = rot(Ti, normalize(mk(gx, gy, gz)));
+      // unit gradient through the hardware reciprocal square root (a zero
+      // gradient gives NaN, as the exact quotient does)
+      const float rn = __builtin_amdgcn_rsqf(gx * gx + gy * gy + gz * gz);
+      const f3 n = rot(Ti, mk(gx * rn, gy * rn, gz * rn));
       nrm = make_float4(n.x, n.y, n.z, 0.f);
     }
     vert = make_float4(pc.x, pc.y, pc.z, intensity_of(cr, cg, cb));
@@ -937,6 +1095,22 @@ void rebuild_far_maps(const Launch &L) {
   if (L.nact <= 0) return;
   hipLaunchKernelGGL(k_clear_far_maps, dim3(1024, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
   hipLaunchKernelGGL(k_dilate_near, dim3(256, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
+  hipLaunchKernelGGL(k_mark_allocated, dim3(256, (unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active);
+}
+
+void launch_delete(const DmView *view, const int4 *req, int n, unsigned long long *dmask, int *touched, int *counts,
+                   hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_del_mark, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, view, req, n, dmask, touched,
+                     counts);
+  hipLaunchKernelGGL(k_del_compact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, view, dmask, touched,
+                     counts);
+  hipLaunchKernelGGL(k_del_clear, dim3(1024), dim3(256), 0, st, view, counts);
+  hipLaunchKernelGGL(k_del_close, dim3(1), dim3(64), 0, st, view, counts);
+}
+
+void launch_select_far(const DmView *view, const double c[3], double radius, int4 *req, int *nreq, hipStream_t st) {
+  hipLaunchKernelGGL(k_select_far, dim3(1024), dim3(256), 0, st, view, c[0], c[1], c[2], radius * radius, req, nreq);
 }
 
 void launch_raycast(const Launch &L) {

@@ -135,6 +135,24 @@ class TsdfVolume:
     def ExtractMesh(self, lower_corner=None, upper_corner=None, max_triangles=200000):
         return MeshExtractor(max_triangles).ExtractMesh(self, lower_corner, upper_corner)
 
+    # HashTable::DeleteBlock (hashTable.cu:112-134) for each block of xyz (n, 3)
+    def DeleteBlocks(self, xyz):
+        k = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        n = C.c_int64()
+        N.check(self._lib.dm_volume_delete_blocks(self._h, N.ptr(k), len(k), C.byref(n)))
+        return n.value
+
+    def DeleteBlock(self, x, y, z):
+        return self.DeleteBlocks([[x, y, z]]) == 1
+
+    def Recycle(self, center, radius):
+        """Delete every allocated block whose centre lies farther than radius
+        (metres) from center; returns the number deleted."""
+        c = np.ascontiguousarray(center, np.float64).reshape(3)
+        n = C.c_int64()
+        N.check(self._lib.dm_volume_recycle(self._h, N.ptr(c), float(radius), C.byref(n)))
+        return n.value
+
     def counters(self):
         out = np.zeros(5, np.int64)
         N.check(self._lib.dm_volume_counters(self._h, N.ptr(out)))

@@ -98,7 +98,7 @@ DM_API int dm_volume_allocate(dm_volume *vol, const float *host_depth, const dm_
  * where undefined; depth 0 = no surface. */
 DM_API int dm_volume_raycast(dm_volume *vol, const float pose[16], const dm_sensor *sensor, float *host_depth,
                              float *host_vertex, float *host_normal, uint8_t *host_rgb);
-/* HashTable::GetNumAllocatedBlocks, hashTable.cu:165-167 */
+/* HashTable::GetNumAllocatedBlocks, hashTable.cu:165-167 (live blocks) */
 DM_API int dm_volume_num_allocated(dm_volume *vol, int32_t *n);
 /* HashTable::GetNumEntries, hashTable.cu:169-171 */
 DM_API int dm_volume_num_entries(dm_volume *vol, int64_t *n);
@@ -111,6 +111,18 @@ DM_API int dm_volume_get_entries(dm_volume *vol, int32_t *out);
  * block, voxel index x*64 + y*8 + z (geom_Types.h:116-119). */
 DM_API int dm_volume_get_blocks(dm_volume *vol, int32_t cap, int32_t *xyz, float *sdf, uint8_t *rgb,
                                 uint8_t *weight, int32_t *n_out);
+/* HashTable::DeleteBlock (hashTable.cu:112-134) for n blocks (xyz: n x 3
+ * block coordinates; absent or repeated ones are skipped): each block's
+ * voxels are reset, its pool index goes back on the free stack (Heap::Append,
+ * heap.cu:19-22) that later insertions take first, and its bucket's remaining
+ * entries close up in their order (occupied slots stay a prefix).  deleted
+ * (may be NULL) receives the number removed. */
+DM_API int dm_volume_delete_blocks(dm_volume *vol, const int32_t *xyz, int64_t n, int64_t *deleted);
+/* Pool recycling for long-running reconstructions (no reference
+ * counterpart; the reference never calls DeleteBlock): delete every
+ * allocated block whose centre ((x + 1/2) * 8 * voxel_size, per axis) lies
+ * farther than radius metres from center. */
+DM_API int dm_volume_recycle(dm_volume *vol, const double center[3], double radius, int64_t *deleted);
 /* Capacity counters (the reference drops silently, hashTable.cu:109):
  * out[0] allocated blocks, [1] dropped (bucket full), [2] dropped (pool
  * exhausted), [3] block visits outside the key range, [4] new keys lost to a

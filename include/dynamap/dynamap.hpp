@@ -211,6 +211,20 @@ class tsdfVolume {
     check(dm_volume_get_entries(v_, e.data()));
     return HashEntry{e[4 * i], e[4 * i + 1], e[4 * i + 2], e[4 * i + 3]};
   }
+  // HashTable::DeleteBlock (hashTable.cu:112-134): true when the block was allocated
+  bool DeleteBlock(int x, int y, int z) {
+    const int32_t xyz[3] = {x, y, z};
+    int64_t n = 0;
+    check(dm_volume_delete_blocks(v_, xyz, 1, &n));
+    return n == 1;
+  }
+  // pool recycling: delete every block whose centre lies farther than radius from center
+  int64_t Recycle(const float3 &center, double radius) {
+    const double c[3] = {center.x, center.y, center.z};
+    int64_t n = 0;
+    check(dm_volume_recycle(v_, c, radius, &n));
+    return n;
+  }
   tsdfProperties GetProperties() const { return Properties; }
   dm_volume *handle() { return v_; }
   // whole reconstruction state (no reference counterpart: resumable sessions)

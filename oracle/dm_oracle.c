@@ -142,7 +142,10 @@ typedef struct {
   real *sdf;           /* [num_blocks * 512] */
   uint8_t *rgb;        /* [num_blocks * 512 * 3] */
   uint8_t *w;          /* [num_blocks * 512] */
-  int num_alloc;
+  uint8_t *live;       /* [num_blocks] 1 = pool index in use */
+  int32_t *free_list;  /* [num_blocks] freed pool indices (a stack) */
+  int num_alloc;       /* pool high-water mark */
+  int num_free;
   int64_t dropped_bucket, dropped_pool, dropped_range;
 } or_volume;
 
@@ -181,16 +184,19 @@ OR_EXPORT or_volume *or_volume_create(double voxel_size, int num_buckets, int bu
   v->sdf = (real *)calloc((size_t)num_blocks * 512, sizeof(real));
   v->rgb = (uint8_t *)calloc((size_t)num_blocks * 512 * 3, 1);
   v->w = (uint8_t *)calloc((size_t)num_blocks * 512, 1);
+  v->live = (uint8_t *)calloc((size_t)num_blocks, 1);
+  v->free_list = (int32_t *)malloc((size_t)num_blocks * sizeof(int32_t));
   return v;
 }
 
 OR_EXPORT void or_volume_destroy(or_volume *v) {
   if (!v) return;
   free(v->keys); free(v->ptrs); free(v->count); free(v->block_key);
-  free(v->sdf); free(v->rgb); free(v->w); free(v);
+  free(v->sdf); free(v->rgb); free(v->w); free(v->live); free(v->free_list); free(v);
 }
 
-OR_EXPORT int or_volume_num_alloc(const or_volume *v) { return v->num_alloc; }
+/* live blocks */
+OR_EXPORT int or_volume_num_alloc(const or_volume *v) { return v->num_alloc - v->num_free; }
 OR_EXPORT void or_volume_dropped(const or_volume *v, int64_t out[3]) {
   out[0] = v->dropped_bucket; out[1] = v->dropped_pool; out[2] = v->dropped_range;
 }
@@ -372,7 +378,9 @@ OR_EXPORT void or_allocate(or_volume *v, const float *depth, int rows, int cols,
     n_ins += take;
     j = e;
   }
-  if ((size_t)v->num_alloc + n_ins > (size_t)v->num_blocks) {
+  /* freed pool indices are reused first (Heap::Consume, heap.cu:14-17) */
+  const size_t grow = n_ins > (size_t)v->num_free ? n_ins - (size_t)v->num_free : 0;
+  if ((size_t)v->num_alloc + grow > (size_t)v->num_blocks) {
     v->dropped_pool += (int64_t)u;
     free(nk);
     return;
@@ -384,7 +392,8 @@ OR_EXPORT void or_allocate(or_volume *v, const float *depth, int rows, int cols,
     for (size_t q = j; q < e; ++q) {
       if (v->count[b] >= v->bucket_size) { v->dropped_bucket++; continue; }
       int slot = b * v->bucket_size + v->count[b];
-      int ptr = v->num_alloc++;
+      int ptr = v->num_free > 0 ? v->free_list[--v->num_free] : v->num_alloc++;
+      v->live[ptr] = 1;
       v->keys[slot] = nk[q].key;
       v->ptrs[slot] = ptr;
       v->block_key[3 * ptr] = nk[q].x; v->block_key[3 * ptr + 1] = nk[q].y; v->block_key[3 * ptr + 2] = nk[q].z;
@@ -459,7 +468,8 @@ OR_EXPORT void or_integrate(or_volume *v, const float *depth, const uint8_t *rgb
   rigid_inverse(T, Ti);
   or_sensor s = {rows, cols, (real)fx, (real)fy, (real)cx, (real)cy};
 #pragma omp parallel for schedule(dynamic, 64)
-  for (int p = 0; p < v->num_alloc; ++p) integrate_block(v, p, depth, rgb, &s, Ti);
+  for (int p = 0; p < v->num_alloc; ++p)
+    if (v->live[p]) integrate_block(v, p, depth, rgb, &s, Ti);
 }
 
 /* IntegrateScan = allocation then integration, tsdf.cu:454-469 */
@@ -488,17 +498,75 @@ OR_EXPORT void or_get_table(const or_volume *v, uint64_t *keys, int32_t *counts)
   memcpy(counts, v->count, (size_t)v->num_buckets * sizeof(int32_t));
 }
 
-/* All allocated blocks: coordinates, sdf, rgb, weight, in pool order. */
+/* All allocated (live) blocks: coordinates, sdf, rgb, weight, in pool order. */
 OR_EXPORT void or_dump_blocks(const or_volume *v, int32_t *xyz, double *sdf, uint8_t *rgb, uint8_t *w) {
+  int o = 0;
   for (int p = 0; p < v->num_alloc; ++p) {
-    xyz[3 * p] = v->block_key[3 * p]; xyz[3 * p + 1] = v->block_key[3 * p + 1]; xyz[3 * p + 2] = v->block_key[3 * p + 2];
+    if (!v->live[p]) continue;
+    xyz[3 * o] = v->block_key[3 * p]; xyz[3 * o + 1] = v->block_key[3 * p + 1]; xyz[3 * o + 2] = v->block_key[3 * p + 2];
     for (int i = 0; i < 512; ++i) {
-      size_t vi = (size_t)p * 512 + i;
-      if (sdf) sdf[vi] = (double)v->sdf[vi];
-      if (rgb) { rgb[3 * vi] = v->rgb[3 * vi]; rgb[3 * vi + 1] = v->rgb[3 * vi + 1]; rgb[3 * vi + 2] = v->rgb[3 * vi + 2]; }
-      if (w) w[vi] = v->w[vi];
+      size_t vi = (size_t)o * 512 + i, pi = (size_t)p * 512 + i;
+      if (sdf) sdf[vi] = (double)v->sdf[pi];
+      if (rgb) { rgb[3 * vi] = v->rgb[3 * pi]; rgb[3 * vi + 1] = v->rgb[3 * pi + 1]; rgb[3 * vi + 2] = v->rgb[3 * pi + 2]; }
+      if (w) w[vi] = v->w[pi];
+    }
+    ++o;
+  }
+}
+
+/*
+ * HashTable::DeleteBlock (hashTable.cu:112-134) for n blocks (absent or
+ * repeated ones skipped): the block's voxels are reset and its pool index
+ * pushed on the free stack (Heap::Append, heap.cu:19-22).  D: the remaining
+ * entries of the bucket close up in their order, so the occupied slots stay
+ * a prefix.  Returns the number deleted.
+ */
+OR_EXPORT int64_t or_delete_blocks(or_volume *v, const int32_t *xyz, int64_t n) {
+  int64_t deleted = 0;
+  for (int64_t j = 0; j < n; ++j) {
+    const int x = xyz[3 * j], y = xyz[3 * j + 1], z = xyz[3 * j + 2];
+    if (!in_key_range(x, y, z)) continue;
+    const uint64_t k = or_pack_key(x, y, z);
+    const int base = or_hash(x, y, z, v->num_buckets, v->bucket_size);
+    const int b = base / v->bucket_size;
+    for (int s = 0; s < v->count[b]; ++s) {
+      if (v->keys[base + s] != k) continue;
+      const int ptr = v->ptrs[base + s];
+      for (int t = s; t + 1 < v->count[b]; ++t) {
+        v->keys[base + t] = v->keys[base + t + 1];
+        v->ptrs[base + t] = v->ptrs[base + t + 1];
+      }
+      v->count[b]--;
+      v->keys[base + v->count[b]] = 0;
+      v->ptrs[base + v->count[b]] = -1;
+      memset(v->sdf + (size_t)ptr * 512, 0, 512 * sizeof(real));
+      memset(v->rgb + (size_t)ptr * 512 * 3, 0, 512 * 3);
+      memset(v->w + (size_t)ptr * 512, 0, 512);
+      v->live[ptr] = 0;
+      v->free_list[v->num_free++] = ptr;
+      ++deleted;
+      break;
     }
   }
+  return deleted;
+}
+
+/* Pool recycling: delete every live block whose centre ((x + 1/2) * 8 *
+ * voxel_size per axis, in double) lies farther than radius from center. */
+OR_EXPORT int64_t or_recycle(or_volume *v, const double center[3], double radius) {
+  int32_t *sel = (int32_t *)malloc((size_t)(v->num_alloc > 0 ? v->num_alloc : 1) * 3 * sizeof(int32_t));
+  int64_t m = 0;
+  const double bm = 8.0 * (double)v->voxel_size, r2 = radius * radius;
+  for (int p = 0; p < v->num_alloc; ++p) {
+    if (!v->live[p]) continue;
+    const int32_t *k = v->block_key + 3 * p;
+    const double dx = ((double)k[0] + 0.5) * bm - center[0], dy = ((double)k[1] + 0.5) * bm - center[1],
+                 dz = ((double)k[2] + 0.5) * bm - center[2];
+    if (dx * dx + dy * dy + dz * dz > r2) { memcpy(sel + 3 * m, k, 3 * sizeof(int32_t)); ++m; }
+  }
+  const int64_t d = or_delete_blocks(v, sel, m);
+  free(sel);
+  return d;
 }
 
 /* ------------------------------------------------------------------ */

@@ -237,6 +237,14 @@ int main() {
     vol.Restore(snap);
     if (vol.GetNumAllocatedBlocks() != before || snap.size() < 4096) return 11;
   }
+  // HashTable::DeleteBlock: an allocated block goes, a second delete finds nothing
+  {
+    // block (0, 0, 12) (1 cm voxels, 8 cm blocks) holds the wall at z = 1 m under the centre pixel
+    const int before = vol.GetNumAllocatedBlocks();
+    if (!vol.DeleteBlock(0, 0, 12) || vol.DeleteBlock(0, 0, 12)) return 25;
+    if (vol.GetNumAllocatedBlocks() != before - 1) return 26;
+    if (vol.Recycle(float3{0.f, 0.f, 1.f}, 0.0) != before - 1 || vol.GetNumAllocatedBlocks() != 0) return 27;
+  }
   std::printf("ok\n");
   return 0;
 }

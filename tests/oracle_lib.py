@@ -32,6 +32,8 @@ _PROTOS = {
     "or_get_block": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
     "or_get_table": (None, [_vp, _vp, _vp]),
     "or_dump_blocks": (None, [_vp, _vp, _vp, _vp, _vp]),
+    "or_delete_blocks": (C.c_int64, [_vp, _vp, C.c_int64]),
+    "or_recycle": (C.c_int64, [_vp, _vp, _d]),
     "or_raycast": (None, [_vp, _i, _i, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _vp]),
     "or_vertex_map": (None, [_vp, _i, _i, _d, _d, _d, _d, _vp]),
     "or_normal_map": (None, [_vp, _i, _i, _vp]),
@@ -118,6 +120,15 @@ class OracleVolume:
         counts = np.zeros(B, np.int32)
         self.L.or_get_table(self.h, P(keys), P(counts))
         return keys, counts
+
+    def delete_blocks(self, xyz):
+        """HashTable::DeleteBlock for each block of xyz (n, 3); returns the number deleted."""
+        k = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        return int(self.L.or_delete_blocks(self.h, P(k), len(k)))
+
+    def recycle(self, center, radius):
+        c = np.ascontiguousarray(center, np.float64)
+        return int(self.L.or_recycle(self.h, P(c), float(radius)))
 
     def blocks(self):
         n = self.num_alloc()

@@ -191,21 +191,34 @@ def test_pair_and_mesh_solves_match_fp64_oracle(bend_case):
     V, Nn, pose, depth = bend_case
     src = depth.copy()
     src[:, : SENS["cols"] // 2] = 0.0
+    solver = dm.NonRigidICP()
+    # the first linearisation (identity transforms): the same gated residuals
+    g1 = dm.DefGraph(V, Nn, 0.025, 0.05)
+    s1 = solver.solve(g1, depth, SENS, pose, source_depth=src, gn_iterations=1)
+    o1 = O.OracleGraph(V, Nn, 0.025, 0.05, bits=64).solve_pair(depth, src, SENS, pose, gn_iterations=1)
+    full = O.OracleGraph(V, Nn, 0.025, 0.05, bits=64).solve(depth, SENS, pose, gn_iterations=1)
+    print("pair, first linearisation: gpu", s1, "oracle", o1, "ungated oracle residuals", full[0])
+    assert o1[0] < 0.8 * full[0]
+    assert abs(s1["residuals"] - o1[0]) <= 0.002 * o1[0]
+    assert abs(s1["data_energy"] - o1[1]) <= 0.01 * o1[1]
+    # the full solve: where the gated frame has data, both fit the bend
     g = dm.DefGraph(V, Nn, 0.025, 0.05)
     o = O.OracleGraph(V, Nn, 0.025, 0.05, bits=64)
-    solver = dm.NonRigidICP()
     solver.reserve(g, SENS["rows"] * SENS["cols"])
     sg = solver.solve(g, depth, SENS, pose, source_depth=src)
     so = o.solve_pair(depth, src, SENS, pose)
-    full = O.OracleGraph(V, Nn, 0.025, 0.05, bits=64).solve(depth, SENS, pose)
-    print("pair: gpu", sg, "oracle", so, "ungated oracle residuals", full[0])
-    assert so[0] < 0.8 * full[0]
-    assert abs(sg["residuals"] - so[0]) <= 0.01 * so[0]
+    print("pair: gpu", sg, "oracle", so)
     gv, _ = dm.DqBlender(g).blendMesh()
     ov, on = o.blend()
-    vis = V[:, 2] < -0.07
-    dn = np.abs(np.sum((gv - ov) * on, 1))[vis]
-    assert dn.mean() < 1e-4
+    Ti = np.linalg.inv(pose)
+    cam = V.astype(np.float64) @ Ti[:3, :3].T + Ti[:3, 3]
+    u = SENS["fx"] * cam[:, 0] / cam[:, 2] + SENS["cx"]
+    seen = (V[:, 2] < -0.07) & (u > SENS["cols"] // 2 + 20)  # well inside the kept half
+    assert seen.sum() > 1000
+    eg, eo = bend_err(gv, 10)[seen], bend_err(ov, 10)[seen]
+    print(f"gated solve, surface error where data is kept: gpu {1000 * eg.mean():.4f} mm, "
+          f"oracle {1000 * eo.mean():.4f} mm")
+    assert eg.mean() < 3e-4 and eo.mean() < 3e-4
     # mesh target: the analytic bend of frame 10 in the camera frame at pose
     b = 0.05 * np.sin(2 * np.pi * 10 / 40) * ((0.3 - V[:, 1].astype(np.float64)) / 0.6) ** 2
     Vt = V.astype(np.float64).copy()
@@ -223,4 +236,39 @@ def test_pair_and_mesh_solves_match_fp64_oracle(bend_case):
     ov2, on2 = o2.blend()
     dn2 = np.abs(np.sum((gv2 - ov2) * on2, 1))
     assert dn2.mean() < 1e-4
-    assert np.abs(np.sum((ov2 - Vt) * on2, 1)).mean() < 3e-4
+    # five Gauss-Newton steps from the rest shape close most of the bend
+    # (up to 5 cm); the target surface is reached along its normal
+    e0 = np.abs(np.sum((V - Vt) * Nn, 1)).mean()
+    e1 = np.abs(np.sum((ov2 - Vt) * on2, 1)).mean()
+    print(f"mesh target: normal-direction distance to the target {1000 * e0:.3f} mm -> {1000 * e1:.3f} mm")
+    assert e1 < 0.1 * e0
+
+
+def test_4096_node_graph_matches_fp64_oracle():
+    """The deforming-cylinder workload's 4096-node reading: the 200 000-vertex
+    mesh with nodes on a 13.3 mm grid (4048 nodes, the grid rule's closest
+    count to 4096) and a 2.5 cm weight radius, five frames of the bend, one
+    solve per frame, against the FP64 oracle on the same frames."""
+    V, Nn = cylinder(500, 400)
+    sens = dict(rows=480, cols=640, fx=525.0, fy=525.0, cx=319.5, cy=239.5)
+    g = dm.DefGraph(V, Nn, 0.0133, 0.025, max_nodes=8192)
+    o = O.OracleGraph(V, Nn, 0.0133, 0.025, max_nodes=8192, bits=64)
+    assert g.nodeNum == o.nn == 4048
+    assert np.array_equal(g.nodes()[0], o.nodes()[0])
+    solver = dm.NonRigidICP()
+    for f in range(5, 10):
+        pose = cyl_pose(f)
+        d = cyl_depth(f, pose, sens)
+        sg = solver.solve(g, d, sens, pose)
+        so = o.solve(d, sens, pose)
+    gv, _ = dm.DqBlender(g).blendMesh()
+    ov, on = o.blend()
+    vis = V[:, 2] < -0.07
+    dn = np.abs(np.sum((gv - ov) * on, 1))[vis]
+    eg, eo = bend_err(gv, 9)[vis], bend_err(ov, 9)[vis]
+    print(f"nodes {g.nodeNum} blocks {g.num_blocks()}: normal-direction |gpu - fp64| mean {dn.mean():.2e} "
+          f"p95 {np.percentile(dn, 95):.2e}; surface error gpu {1000 * eg.mean():.4f} mm, oracle "
+          f"{1000 * eo.mean():.4f} mm; last stats gpu {sg} oracle {so}")
+    assert abs(sg["residuals"] - so[0]) <= 0.01 * so[0]
+    assert dn.mean() < 1e-4 and np.percentile(dn, 95) < 3e-4
+    assert eg.mean() < 3e-4
