@@ -43,8 +43,10 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--frames", type=int, default=300, help="frames per session trace")
+    ap.add_argument("--no-8mm-line", action="store_true", help="skip the 16 x 640x480 / 8 mm serving line")
     ap.add_argument("--sessions", type=int, default=16)
     ap.add_argument("--poisson-hz", type=float, default=30.0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -68,10 +70,12 @@ def ate(A, B):
     return float(np.sqrt(np.mean(np.sum((A[..., :3, 3] - B[..., :3, 3]) ** 2, -1))))
 
 
-def bench_nonrigid(dm, nframes, with_cpu):
-    """Deforming-cylinder workload: 200 000-vertex canonical mesh, 2.5 cm
-    nodes, 640x480 depth of a sinusoidally bending cylinder; one solve
-    (5 Gauss-Newton iterations, block-Jacobi PCG) per frame."""
+def bench_nonrigid(dm, nframes, with_cpu, node_radius=0.0133, dgw=0.025):
+    """Deforming-cylinder workload: 200 000-vertex canonical mesh, 640x480
+    depth of a sinusoidally bending cylinder; one solve (5 Gauss-Newton
+    iterations, block-Jacobi PCG) per frame.  The default graph is the
+    workload's 4096-node reading: nodes on a 13.3 mm grid (4048 nodes, the
+    grid rule's closest count) with a 2.5 cm weight radius."""
     N = dm._native
     S = N.synth()
     nt, ny = 500, 400
@@ -88,7 +92,7 @@ def bench_nonrigid(dm, nframes, with_cpu):
                               239.5, 20261202 + 3000 + f, 1, N.ptr(d))
         frames.append(d)
     t0 = time.time()
-    g = dm.DefGraph(V, Nn, 0.025, 0.05)
+    g = dm.DefGraph(V, Nn, node_radius, dgw, max_nodes=8192)
     t_graph = time.time() - t0
     solver = dm.NonRigidICP()
     solver.solve(g, frames[0], sens, poses[0].reshape(4, 4))  # capture
@@ -101,7 +105,8 @@ def bench_nonrigid(dm, nframes, with_cpu):
     f = nframes - 1
     b = 0.05 * np.sin(2 * np.pi * f / 40) * ((0.3 - vb[:, 1]) / 0.6) ** 2
     err = np.abs(np.sqrt((vb[:, 0] - b) ** 2 + vb[:, 2] ** 2) - 0.15)[V[:, 2] < -0.07]
-    res = dict(workload="bending cylinder, 200k-vertex mesh, 640x480, 5 GN iterations x 40 PCG per frame",
+    res = dict(workload=f"bending cylinder, 200k-vertex mesh, 640x480, {g.nodeNum} nodes ({1000 * node_radius:g} mm "
+                        f"grid, {1000 * dgw:g} mm weight radius), 5 GN iterations x 40 PCG per frame",
                frames=nframes, fps=nframes / el, ms_per_frame=1000 * el / nframes, nodes=g.nodeNum,
                blocks=g.num_blocks(), graph_build_ms=1000 * t_graph, surface_err_mm=1000 * float(err.mean()),
                last_stats=st)
@@ -110,7 +115,7 @@ def bench_nonrigid(dm, nframes, with_cpu):
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
-        o = O.OracleGraph(V, Nn, 0.025, 0.05, bits=64)
+        o = O.OracleGraph(V, Nn, node_radius, dgw, max_nodes=8192, bits=64)
         t0 = time.time()
         k = 0
         while k < nframes and (k < 2 or time.time() - t0 < 5.0):
@@ -219,6 +224,79 @@ def bench_stress(dm, nframes, poisson_hz, warm=3):
     return res
 
 
+def serve_line(dm, traces, nf, K, W, args, label):
+    """One serving workload on a fresh server: every trace has nf frames.
+    Frames 0 .. nf-K-W-1 build each session's map (untimed), W warmup
+    batches follow, then K timed batches end at the trace's last frame, so
+    the timed frames see the map at its steady state.  The per-frame paths
+    run the same frames after the same untimed history; Poisson arrivals
+    (args.poisson_hz per session) cover all nf frames of fresh sessions."""
+    from dynamap_amd import configs
+    N = dm._native
+    ns = len(traces)
+    F0 = nf - K - W
+    if F0 < 0:
+        raise SystemExit(f"--steps + --warmup must not exceed the {nf} frames of a trace")
+    srv = dm.Server(ns, 480, 640, hbm_budget_bytes=250 << 30, use_graph=True)
+    for name, depth, rgb, gt in traces:
+        w = configs.WORKLOADS[name]
+        srv.add_trace(w["tsdf"], w["icp"], w["sensor"], gt[0], depth, rgb)
+    out = {}
+    # launch shapes chosen by the autotuner on this workload's steady state
+    tuning = "defaults (16x16 raycast tiles, 2048 integration workgroups per session)"
+    if not args.no_autotune:
+        srv.run(N.DM_RUN_BATCHED, 0, F0 + W)
+        tuning = srv.autotune(frame=F0 + W, reps=3)
+        srv.reset()
+        log(label, "autotune", tuning)
+    # batched closed loop
+    srv.run(N.DM_RUN_BATCHED, 0, F0 + W)
+    r = srv.run(N.DM_RUN_BATCHED, F0 + W, K, want_poses=True)
+    wall_b = r["wall"]
+    fps_b = ns * K / wall_b
+    poses_b = r["poses"]
+    log(f"{label} batched: {fps_b:.1f} frames/s, {r['batches']} batches, {1000 * wall_b / K:.3f} ms/batch "
+        f"(frames {F0 + W}..{nf - 1})")
+    out.update(fps=fps_b, wall=wall_b, tuning=tuning, timed_frames=[F0 + W, nf - 1], untimed_history=F0,
+               ms_per_batch=1000 * wall_b / K)
+    if not args.no_per_frame:
+        per = {}
+        for mode, key in ((N.DM_RUN_PER_FRAME_HOST, "host_solve"), (N.DM_RUN_PER_FRAME_DEV, "device_solve")):
+            srv.reset()
+            srv.run(N.DM_RUN_BATCHED, 0, F0)
+            srv.run(mode, F0, W)
+            rp = srv.run(mode, F0 + W, K, want_poses=True)
+            per[key] = dict(fps=ns * K / rp["wall"], ms_per_frame=1000 * rp["wall"] / (ns * K))
+            per[key]["poses"] = rp["poses"]
+            log(f"{label} per-frame {key}: {per[key]['fps']:.1f} frames/s")
+        out["per_frame"] = {k: {kk: vv for kk, vv in v.items() if kk != "poses"} for k, v in per.items()}
+        out["per_frame"]["ratio_vs_host_solve"] = fps_b / per["host_solve"]["fps"]
+        out["per_frame"]["ratio_vs_device_solve"] = fps_b / per["device_solve"]["fps"]
+        out["ate_mm"] = dict(
+            batched_vs_per_frame_device=1000 * ate(poses_b, per["device_solve"]["poses"]),
+            batched_vs_per_frame_host=1000 * ate(poses_b, per["host_solve"]["poses"]),
+            max_abs_pose_diff_batched_vs_per_frame_device=float(np.abs(poses_b - per["device_solve"]["poses"]).max()),
+            max_abs_pose_diff_batched_vs_per_frame_host=float(np.abs(poses_b - per["host_solve"]["poses"]).max()))
+    gt_all = np.stack([t[3][F0 + W:nf] for t in traces])
+    out.setdefault("ate_mm", {})["gpu_vs_ground_truth"] = 1000 * ate(poses_b, gt_all)
+    if not args.no_poisson:
+        rng = np.random.default_rng(configs.SEED_BASE + 4000)
+        gaps = rng.exponential(1.0 / args.poisson_hz, size=(ns, nf))
+        arrivals = np.cumsum(gaps, axis=1) - gaps[:, :1]
+        lat = {}
+        for mode, key in ((N.DM_RUN_BATCHED, "batched"), (N.DM_RUN_PER_FRAME_HOST, "per_frame_host_solve")):
+            srv.reset()
+            rr = srv.run(mode, 0, nf, arrivals=arrivals)
+            L = (rr["completions"] - arrivals) * 1000.0
+            lat[key] = dict(p50=float(np.percentile(L, 50)), p99=float(np.percentile(L, 99)),
+                            max=float(L.max()), served_fps=ns * nf / rr["wall"], batches=rr["batches"],
+                            samples=int(L.size))
+            log(f"{label} poisson {key}: p50 {lat[key]['p50']:.2f} ms p99 {lat[key]['p99']:.2f} ms "
+                f"over {L.size} frames")
+        out["latency_ms"] = lat
+    return srv, out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -227,10 +305,9 @@ def main():
         sys.exit(2)
     import dynamap_amd as dm
     from dynamap_amd import configs
-    N = dm._native
 
     W, K = args.warmup, args.steps
-    nf = W + K + 1  # one extra frame for the stage profile
+    nf = args.frames
     ns = args.sessions
     names = ["rigid-320x240-2cm"] * (ns // 2) + ["rigid-640x480-8mm"] * (ns - ns // 2)
     t0 = time.time()
@@ -240,65 +317,14 @@ def main():
         traces.append((name, depth, rgb, gt))
     log(f"generated {ns} x {nf} frames in {time.time() - t0:.1f}s")
 
-    def make_server():
-        srv = dm.Server(ns, 480, 640, hbm_budget_bytes=200 << 30, use_graph=True)
-        for name, depth, rgb, gt in traces:
-            w = configs.WORKLOADS[name]
-            srv.add_trace(w["tsdf"], w["icp"], w["sensor"], gt[0], depth, rgb)
-        return srv
-
-    srv = make_server()
-    # ---- launch shapes chosen by the autotuner on this workload's state
-    tuning = "defaults (16x16 raycast tiles, 2048 integration workgroups per session)"
-    if not args.no_autotune:
-        srv.run(N.DM_RUN_BATCHED, 0, W)
-        tuning = srv.autotune(frame=W, reps=3)
-        log("autotune", tuning)
-    # ---- batched closed loop
-    srv.run(N.DM_RUN_BATCHED, 0, W)
-    r = srv.run(N.DM_RUN_BATCHED, W, K, want_poses=True)
-    wall_b = r["wall"]
-    fps_b = ns * K / wall_b
-    poses_b = r["poses"]
-    log(f"batched: {fps_b:.1f} frames/s, {r['batches']} batches, {1000 * wall_b / K:.3f} ms/batch")
-    # ---- stage profile on the next frame
-    stage_ms, work = srv.profile(W + K, reps=5)
+    srv, line = serve_line(dm, traces, nf, K, W, args, "serve-16")
+    fps_b, wall_b, tuning = line.pop("fps"), line.pop("wall"), line.pop("tuning")
+    out = dict(line)
+    # stage profile of one batch at the traces' last frame (steady state)
+    srv.reset()
+    srv.run(dm._native.DM_RUN_BATCHED, 0, nf - 1)
+    stage_ms, work = srv.profile(nf - 1, reps=5)
     log("stage ms", stage_ms, work)
-
-    out = {}
-    # ---- per-frame paths
-    if not args.no_per_frame:
-        per = {}
-        for mode, label in ((N.DM_RUN_PER_FRAME_HOST, "host_solve"), (N.DM_RUN_PER_FRAME_DEV, "device_solve")):
-            srv.run(mode, 0, W)
-            rp = srv.run(mode, W, K, want_poses=True)
-            per[label] = dict(fps=ns * K / rp["wall"], ms_per_frame=1000 * rp["wall"] / (ns * K))
-            per[label]["poses"] = rp["poses"]
-            log(f"per-frame {label}: {per[label]['fps']:.1f} frames/s")
-        out["per_frame"] = {k: {kk: vv for kk, vv in v.items() if kk != "poses"} for k, v in per.items()}
-        out["per_frame"]["ratio_vs_host_solve"] = fps_b / per["host_solve"]["fps"]
-        out["per_frame"]["ratio_vs_device_solve"] = fps_b / per["device_solve"]["fps"]
-        out["ate_mm"] = dict(
-            batched_vs_per_frame_device=1000 * ate(poses_b, per["device_solve"]["poses"]),
-            batched_vs_per_frame_host=1000 * ate(poses_b, per["host_solve"]["poses"]),
-            max_abs_pose_diff_batched_vs_per_frame_device=float(np.abs(poses_b - per["device_solve"]["poses"]).max()))
-    gt_all = np.stack([t[3][W:W + K] for t in traces])
-    out.setdefault("ate_mm", {})["gpu_vs_ground_truth"] = 1000 * ate(poses_b, gt_all)
-
-    # ---- open loop with Poisson arrivals
-    if not args.no_poisson:
-        rng = np.random.default_rng(configs.SEED_BASE + 4000)
-        F = W + K
-        gaps = rng.exponential(1.0 / args.poisson_hz, size=(ns, F))
-        arrivals = np.cumsum(gaps, axis=1) - gaps[:, :1]
-        lat = {}
-        for mode, label in ((N.DM_RUN_BATCHED, "batched"), (N.DM_RUN_PER_FRAME_HOST, "per_frame_host_solve")):
-            rr = srv.run(mode, 0, F, arrivals=arrivals)
-            L = (rr["completions"] - arrivals) * 1000.0
-            lat[label] = dict(p50=float(np.percentile(L, 50)), p99=float(np.percentile(L, 99)),
-                              max=float(L.max()), served_fps=ns * F / rr["wall"], batches=rr["batches"])
-            log(f"poisson {label}: p50 {lat[label]['p50']:.2f} ms p99 {lat[label]['p99']:.2f} ms")
-        out["latency_ms"] = lat
 
     # ---- roofline of the dominant kernel
     stage_kernels = {"integrate": stage_ms["integrate"], "raycast": stage_ms["raycast"],
@@ -378,8 +404,24 @@ def main():
     # ---- marching cubes of one room volume
     if not args.no_mesh:
         out["mesh"] = bench_mesh(dm, not args.no_cpu)
-    # ---- 65-session stress mix, after the serve-16 sessions are released
     srv.close()
+    # ---- the same protocol on 16 sessions that are all 640x480 / 8 mm (the
+    # workload's 8 such sessions plus 8 more seeded ones)
+    if not args.no_8mm_line:
+        t0 = time.time()
+        t8 = [t for t in traces if t[0] == "rigid-640x480-8mm"]
+        for s in range(ns, ns + 16 - len(t8)):
+            depth, rgb, gt = configs.make_sequence("rigid-640x480-8mm", nf, session=s)
+            t8.append(("rigid-640x480-8mm", depth, rgb, gt))
+        log(f"8 mm line: {len(t8)} traces ready in {time.time() - t0:.1f}s")
+        srv8, line8 = serve_line(dm, t8, nf, K, W, args, "serve-16-8mm")
+        srv8.close()
+        del t8
+        line8.pop("wall")
+        fps8, tune8 = line8.pop("fps"), line8.pop("tuning")
+        out["serve16_8mm"] = dict(workload="16 x 640x480 @ 8 mm (2^20 buckets), geometric + photometric ICP",
+                                  batched_fps=fps8, tuning=tune8, **line8)
+    # ---- 65-session stress mix, after the serve-16 sessions are released
     if not args.no_stress:
         out["stress"] = bench_stress(dm, args.stress_frames, args.poisson_hz)
 
@@ -397,8 +439,9 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded analytic RGB-D room scenes, noise + dropout)",
         "config": {"workload": "serve-16: 8 x 320x240 @ 2 cm + 8 x 640x480 @ 8 mm", "sessions": ns,
-                   "frames_per_session": nf, "icp_iterations": [10, 5, 4], "batch": "one frame per session",
-                   "tuning": tuning},
+                   "frames_per_session": nf, "timed_frames": out.pop("timed_frames"),
+                   "untimed_history_frames": out.pop("untimed_history"), "icp_iterations": [10, 5, 4],
+                   "batch": "one frame per session", "tuning": tuning},
     }
     res.update(out)
     res["roofline"] = roofline
