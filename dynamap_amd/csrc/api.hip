@@ -49,7 +49,7 @@ size_t session_bytes(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   const size_t nc = (size_t)new_cap(p.num_blocks);
   size_t b = (size_t)p.num_buckets * bucket_stride(p.bucket_size);
   b += ((size_t)p.num_blocks + 1) * kBlockVox * 8 + (size_t)p.num_blocks * (16 + 1 + 4);
-  b += ((size_t)kCoarseWords + 3 * kWinWords) * 4;
+  b += ((size_t)kCoarseWords + (size_t)kCell4Words + 3 * kWinWords) * 4;
   b += (size_t)pow2_at_least((int)(2 * nc)) * 8 + nc * (8 + 4 * 4) + (size_t)p.num_buckets * 4;
   size_t maps = 0;
   for (int l = 0; l < kLevels; ++l) maps += (size_t)(rows >> l) * (cols >> l) * 16 * 4;
@@ -108,6 +108,7 @@ int Sess::init(const dm_tsdf_props &p, uint32_t rows, uint32_t cols) {
   DM_TRY(dalloc(&h.dilmap, kWinWords));
   DM_TRY(dalloc(&h.allocmap, kWinWords));
   DM_TRY(dalloc(&h.coarse, (size_t)kCoarseWords));
+  DM_TRY(dalloc(&h.cell4, (size_t)kCell4Words));
   DM_TRY(dalloc(&h.dset, (size_t)dsz));
   DM_TRY(dalloc(&h.newkeys, (size_t)h.newcap, false));
   DM_TRY(dalloc(&h.newdslot, (size_t)h.newcap, false));

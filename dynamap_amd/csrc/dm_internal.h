@@ -26,6 +26,9 @@ constexpr int kCoarseWords = kCoarse * kCoarse * kCoarse / 32;
 // Blocks outside the window are looked up through the hash table.
 constexpr int kWin = kCoarse * 8;
 constexpr size_t kWinWords = (size_t)kWin * kWin * kWin / 32;
+// Mid-level cells of 4^3 blocks over the same window (kCell4^3 cells).
+constexpr int kCell4 = kWin / 4;
+constexpr int kCell4Words = kCell4 * kCell4 * kCell4 / 32;
 
 // per-volume counters, int32 each
 enum Ctr : int {
@@ -80,6 +83,7 @@ struct DmView {
   uint32_t *nearmap;    // kWinWords: window bit of every block with blk_near set
   uint32_t *dilmap;     // kWinWords: window bit of every block within one block of a near block
   uint32_t *coarse;     // kCoarse^3 bits: cell of 8^3 blocks holds a block with its dilmap bit set
+  uint32_t *cell4;      // kCell4^3 bits: cell of 4^3 blocks holds a block with its dilmap bit set
   uint32_t *allocmap;   // kWinWords: window bit of every allocated block (the allocation's
                         // table test; blocks outside the window go through the hash table)
   uint64_t *dset;       // per-frame new-key set, dset_mask + 1 slots, 0 = empty
@@ -147,7 +151,7 @@ void launch_delete(const DmView *view, const int4 *req, int n, unsigned long lon
 // live blocks whose centre lies farther than radius from c, appended to req
 // (capacity: the pool size) with *nreq counting them
 void launch_select_far(const DmView *view, const double c[3], double radius, int4 *req, int *nreq, hipStream_t st);
-void rebuild_far_maps(const Launch &L);  // clear and rebuild nearmap/dilmap/coarse/allocmap (not for graph capture)
+void rebuild_far_maps(const Launch &L);  // clear and rebuild nearmap/dilmap/cell4/coarse/allocmap (not for graph capture)
 void launch_source_maps(const Launch &L);
 // ICP linearisation at one level (one FP32 record per workgroup); with solve,
 // a second kernel sums the records and takes the Gauss-Newton step on the

@@ -73,6 +73,7 @@ int sess_reset(Sess &S, const double init_pose[16]) {
   DM_HIP(hipMemsetAsync(S.h.dilmap, 0, kWinWords * 4, S.stream));
   DM_HIP(hipMemsetAsync(S.h.allocmap, 0, kWinWords * 4, S.stream));
   DM_HIP(hipMemsetAsync(S.h.coarse, 0, (size_t)kCoarseWords * 4, S.stream));
+  DM_HIP(hipMemsetAsync(S.h.cell4, 0, (size_t)kCell4Words * 4, S.stream));
   DM_HIP(hipMemsetAsync(S.h.ctr, 0, C_COUNT * 4, S.stream));
   double pd[PD_COUNT] = {0};
   for (int k = 0; k < 16; ++k) {

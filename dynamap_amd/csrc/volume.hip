@@ -396,16 +396,33 @@ __device__ __forceinline__ float far_threshold(float trunc) {
   return big + big * (1.0f / 256.0f);
 }
 
-// Coarse cell (8^3 blocks) of a block coordinate, and its bit in the window.
-__device__ __forceinline__ int coarse_of(int b) { return b >> 3; }
-__device__ __forceinline__ bool coarse_bit(int cx, int cy, int cz, int &word, int &bit) {
-  const int x = cx + kCoarse / 2, y = cy + kCoarse / 2, z = cz + kCoarse / 2;
-  if ((unsigned)x >= (unsigned)kCoarse || (unsigned)y >= (unsigned)kCoarse || (unsigned)z >= (unsigned)kCoarse)
-    return false;
-  const int idx = (z * kCoarse + y) * kCoarse + x;
+// Cell of 2^S blocks per axis of a block coordinate (S = 3: the coarse
+// cells, S = 2: the 4^3 cells), and its bit in a map of (kWin >> S)^3 cells
+// over the window; false outside it.
+template <int S> __device__ __forceinline__ int cell_of(int b) { return b >> S; }
+template <int S> __device__ __forceinline__ bool cell_bit(int cx, int cy, int cz, int &word, int &bit) {
+  constexpr int n = kWin >> S;
+  const int x = cx + n / 2, y = cy + n / 2, z = cz + n / 2;
+  if ((unsigned)x >= (unsigned)n || (unsigned)y >= (unsigned)n || (unsigned)z >= (unsigned)n) return false;
+  const int idx = (z * n + y) * n + x;
   word = idx >> 5;
   bit = idx & 31;
   return true;
+}
+__device__ __forceinline__ int coarse_of(int b) { return cell_of<3>(b); }
+__device__ __forceinline__ bool coarse_bit(int cx, int cy, int cz, int &word, int &bit) {
+  return cell_bit<3>(cx, cy, cz, word, bit);
+}
+// Sets the bits of the cells (2^S blocks per axis) that hold a block of the
+// 3^3 neighbourhood of block (x, y, z): at most 2 per axis.
+template <int S> __device__ __forceinline__ void mark_cells(uint32_t *map, int x, int y, int z) {
+  for (int cz = cell_of<S>(z - 1); cz <= cell_of<S>(z + 1); ++cz)
+    for (int cy = cell_of<S>(y - 1); cy <= cell_of<S>(y + 1); ++cy)
+      for (int cx = cell_of<S>(x - 1); cx <= cell_of<S>(x + 1); ++cx) {
+        int w, b;
+        if (!cell_bit<S>(cx, cy, cz, w, b)) continue;
+        if (!(map[w] & (1u << b))) atomicOr(&map[w], 1u << b);
+      }
 }
 
 // Bit of block (x, y, z) in the block window, false outside it.
@@ -442,11 +459,12 @@ __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ vi
   const float vs = v.voxel_size, tau = v.trunc;
   const double half = 0.5 * (double)vs * 8.0;
   const float far_thr = far_threshold(tau);
-  // the raycast's bitmaps are rebuilt by k_dilate_near: the coarse one is
+  // the raycast's bitmaps are rebuilt by k_dilate_near: the cell maps are
   // cleared whole, the block-level ones at the words that the near blocks of
   // the previous frame set (whole words: every bit in them belongs to a block
   // near one of those, and k_dilate_near sets it again if it still is)
   for (int w = blockIdx.x * 256 + threadIdx.x; w < kCoarseWords; w += gridDim.x * 256) v.coarse[w] = 0u;
+  for (int w = blockIdx.x * 256 + threadIdx.x; w < kCell4Words; w += gridDim.x * 256) v.cell4[w] = 0u;
   int nvis = 0;
   for (int blk = blockIdx.x; blk < nalloc; blk += gridDim.x) {
     const int4 key = v.block_key[blk];
@@ -491,7 +509,8 @@ __global__ __launch_bounds__(256) void k_integrate(const DmView *__restrict__ vi
 // Rebuild the raycast's bitmaps from the near blocks (after the integration
 // has cleared the words of the previous frame's near blocks): the window bit
 // of every near block, the dilated bits of its 3^3 neighbourhood, and the
-// coarse cells that hold a block of that neighbourhood (at most 8).
+// 4^3-block and coarse cells that hold a block of that neighbourhood (at most
+// 8 each).
 __global__ __launch_bounds__(256) void k_dilate_near(const DmView *__restrict__ views, const int *__restrict__ active) {
   const int slot_ = active[blockIdx.y];
   if (slot_ < 0) return;
@@ -508,15 +527,8 @@ __global__ __launch_bounds__(256) void k_dilate_near(const DmView *__restrict__ 
       if ((v.dilmap[w0] & m0) != m0) atomicOr(&v.dilmap[w0], m0);
       if (m1 && (v.dilmap[w1] & m1) != m1) atomicOr(&v.dilmap[w1], m1);
     }
-    const int cx0 = coarse_of(key.x - 1), cx1 = coarse_of(key.x + 1), cy0 = coarse_of(key.y - 1),
-              cy1 = coarse_of(key.y + 1), cz0 = coarse_of(key.z - 1), cz1 = coarse_of(key.z + 1);
-    for (int cz = cz0; cz <= cz1; ++cz)
-      for (int cy = cy0; cy <= cy1; ++cy)
-        for (int cx = cx0; cx <= cx1; ++cx) {
-          int w, b;
-          if (!coarse_bit(cx, cy, cz, w, b)) continue;
-          if (!(v.coarse[w] & (1u << b))) atomicOr(&v.coarse[w], 1u << b);
-        }
+    mark_cells<2>(v.cell4, key.x, key.y, key.z);
+    mark_cells<3>(v.coarse, key.x, key.y, key.z);
   }
 }
 
@@ -652,6 +664,7 @@ __global__ __launch_bounds__(256) void k_clear_far_maps(const DmView *__restrict
     v.allocmap[w] = 0u;
   }
   for (size_t w = (size_t)blockIdx.x * 256 + threadIdx.x; w < (size_t)kCoarseWords; w += stride) v.coarse[w] = 0u;
+  for (size_t w = (size_t)blockIdx.x * 256 + threadIdx.x; w < (size_t)kCell4Words; w += stride) v.cell4[w] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -865,17 +878,22 @@ __device__ __forceinline__ bool block_far_any(const DmView &v, int bx, int by, i
 // approximate sample position in voxel units (error about 1e-3 voxel); the
 // voxels GetInterpolatedVoxel reads at the exact position lie in
 // [floor(pv - e) - 1, floor(pv + 1 + e)] per axis (the exact indices round
-// p/vs -+ 0.5 half away from zero).  Three tests, coarsest first:
-//  - the coarse cell (8^3 blocks) of the block holding the approximate centre
-//    voxel has no dilated bit: the run extends over the cell and its
-//    one-block margin;
-//  - that block's dilated bit is clear (no near block among its 3^3
-//    neighbourhood, allocated or not): the run extends over the neighbourhood;
-//  - otherwise every block of the voxel range must be far (near bit clear,
-//    or unallocated / not near through the hash table outside the window),
-//    and the run extends over those blocks.
+// p/vs -+ 0.5 half away from zero).  B is the block holding the approximate
+// centre voxel; it lies in that range.
+//  - B's dilated bit is clear (no near block among its 3^3 neighbourhood):
+//    the run extends over the largest clear region holding B: its coarse
+//    cell (8^3 blocks) with a one-block margin, else its 4^3-block cell with
+//    that margin, else the neighbourhood.  A cell's bit is clear only when no
+//    block of it has its dilated bit set, so the cell and its margin hold no
+//    near block.  A set dilated bit implies set cell bits, so the cells are
+//    only tested here.
+//  - otherwise B itself near ends the test (B is in the voxel range), and
+//    else every block of the voxel range must be far (near bit clear, or
+//    unallocated / not near through the hash table outside the window), and
+//    the run extends over those blocks.
 // Runs end e + 1 voxel inside the region, so the exact positions up to zmax
-// keep their voxel range inside it.
+// keep their voxel range inside it.  Which region a run takes changes how
+// many tests the march makes, never its samples.
 __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, float z, float &zmax) {
   const float e = 0.05f;
   const float px = po.x + pd.x * z, py = po.y + pd.y * z, pz = po.z + pd.z * z;
@@ -883,14 +901,22 @@ __device__ __forceinline__ bool far_run(const DmView &v, f3 po, f3 pd, float z, 
   const int bx = floordiv8((int)floorf(px + 0.5f)), by = floordiv8((int)floorf(py + 0.5f)),
             bz = floordiv8((int)floorf(pz + 0.5f));
   int x0, x1, y0, y1, z0, z1;
-  const int cx = coarse_of(bx), cy = coarse_of(by), cz = coarse_of(bz);
-  int cw, cb;
   uint32_t idx;
-  if (coarse_bit(cx, cy, cz, cw, cb) && !(ldg(v.coarse + cw) & (1u << cb))) {
-    x0 = 8 * cx - 1; x1 = 8 * cx + 8; y0 = 8 * cy - 1; y1 = 8 * cy + 8; z0 = 8 * cz - 1; z1 = 8 * cz + 8;
-  } else if (win_bit(bx, by, bz, idx) && !test_bit(v.dilmap, idx)) {
-    x0 = bx - 1; x1 = bx + 1; y0 = by - 1; y1 = by + 1; z0 = bz - 1; z1 = bz + 1;
+  const bool bwin = win_bit(bx, by, bz, idx);
+  if (bwin && !test_bit(v.dilmap, idx)) {
+    int cw, cb;
+    if (coarse_bit(coarse_of(bx), coarse_of(by), coarse_of(bz), cw, cb) && !(ldg(v.coarse + cw) & (1u << cb))) {
+      const int cx = coarse_of(bx), cy = coarse_of(by), cz = coarse_of(bz);
+      x0 = 8 * cx - 1; x1 = 8 * cx + 8; y0 = 8 * cy - 1; y1 = 8 * cy + 8; z0 = 8 * cz - 1; z1 = 8 * cz + 8;
+    } else if (cell_bit<2>(cell_of<2>(bx), cell_of<2>(by), cell_of<2>(bz), cw, cb) &&
+               !(ldg(v.cell4 + cw) & (1u << cb))) {
+      const int cx = cell_of<2>(bx), cy = cell_of<2>(by), cz = cell_of<2>(bz);
+      x0 = 4 * cx - 1; x1 = 4 * cx + 4; y0 = 4 * cy - 1; y1 = 4 * cy + 4; z0 = 4 * cz - 1; z1 = 4 * cz + 4;
+    } else {
+      x0 = bx - 1; x1 = bx + 1; y0 = by - 1; y1 = by + 1; z0 = bz - 1; z1 = bz + 1;
+    }
   } else {
+    if (bwin && test_bit(v.nearmap, idx)) return false;
     x0 = floordiv8((int)floorf(px - e) - 1); x1 = floordiv8((int)floorf(px + 1.0f + e));
     y0 = floordiv8((int)floorf(py - e) - 1); y1 = floordiv8((int)floorf(py + 1.0f + e));
     z0 = floordiv8((int)floorf(pz - e) - 1); z1 = floordiv8((int)floorf(pz + 1.0f + e));
