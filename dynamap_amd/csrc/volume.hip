@@ -857,6 +857,161 @@ __device__ __forceinline__ bool interp_voxel_win(const DmView &v, f3 p, float in
                      sdf_out, col);
 }
 
+// The hit's seven interpolations (the colour sample at pw and the central
+// differences at pw +- vs e_a) from one stencil, when every axis is regular:
+// the axis samples of q - vs, q and q + vs have corners (i0 - 1, i0),
+// (i0, i0 + 1), (i0 + 1, i0 + 2) and centres c - 1, c, c + 1 with c in
+// {i0, i0 + 1}.  The samples then read the 2^3 core voxels (i0 + {0,1}, ...)
+// and per axis the 4 voxels before and the 4 after it: 32 voxels in 2^3
+// blocks at most.  Each sample's centre is its corner (c - i0, ...), the
+// same index for all seven.  Corner order, weights and sums are those of
+// interp_axes, so the results are bit for bit the same.
+__device__ __forceinline__ bool stencil_regular(const AxisSample &A, const AxisSample &P, const AxisSample &M) {
+  return A.i1 == A.i0 + 1 && P.i0 == A.i1 && P.i1 == A.i0 + 2 && M.i0 == A.i0 - 1 && M.i1 == A.i0 &&
+         (unsigned)(A.c - A.i0) <= 1u && P.c == A.c + 1 && M.c == A.c - 1;
+}
+// value of corner ci (0..7) of f, without indexing registers
+__device__ __forceinline__ float pick8(const float (&f)[8], int ci) {
+  const float a = (ci & 1) ? f[1] : f[0], b = (ci & 1) ? f[3] : f[2];
+  const float c = (ci & 1) ? f[5] : f[4], d = (ci & 1) ? f[7] : f[6];
+  const float lo = (ci & 2) ? b : a, hi = (ci & 2) ? d : c;
+  return (ci & 4) ? hi : lo;
+}
+// interp_axes' sum for one sample: corners f (index ox + 2 oy + 4 oz), bit
+// o of obs set when corner o is observed, centre corner ci
+__device__ __forceinline__ bool blend8(const float (&f)[8], uint32_t obs, int ci, f3 wt, float &out) {
+  if (!((obs >> ci) & 1u)) return false;
+  const float fc = pick8(f, ci);
+  float dist = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ox = (0xD2 >> k) & 1, oy = (0xB4 >> k) & 1, oz = (0xE8 >> k) & 1;
+    const int o = ox + 2 * oy + 4 * oz;
+    const float a = ox ? wt.x : 1.0f - wt.x;
+    const float b = oy ? wt.y : 1.0f - wt.y;
+    const float g = oz ? wt.z : 1.0f - wt.z;
+    const float coef = a * b * g;
+    dist += coef * (((obs >> o) & 1u) ? f[o] : fc);
+  }
+  out = dist;
+  return true;
+}
+__device__ __forceinline__ void hit_stencil(const DmView &v, const AxisSample &X0, const AxisSample &Y0,
+                                            const AxisSample &Z0, const float (&wp)[3], const float (&wm)[3],
+                                            BlockWin &W, float (&grad)[3], bool &ok, uint32_t (&rgb)[3],
+                                            bool &has_rgb) {
+  const int i0 = X0.i0, j0 = Y0.i0, k0 = Z0.i0;
+  // blocks of voxels i0 - 1 .. i0 + 2 per axis (two at most)
+  const int lox = (i0 - 1) >> 3, hix = (i0 + 2) >> 3, loy = (j0 - 1) >> 3, hiy = (j0 + 2) >> 3,
+            loz = (k0 - 1) >> 3, hiz = (k0 + 2) >> 3;
+  if (lox < W.wx || hix > W.wx + 1 || loy < W.wy || hiy > W.wy + 1 || loz < W.wz || hiz > W.wz + 1) {
+    W.wx = lox - (hix == lox ? (int)((W.resolved >> 8) & 1u) : 0);
+    W.wy = loy - (hiy == loy ? (int)((W.resolved >> 9) & 1u) : 0);
+    W.wz = loz - (hiz == loz ? (int)((W.resolved >> 10) & 1u) : 0);
+    W.resolved &= 0x700u;
+  }
+  // slots of the block ranges: x bits 0-1, y spread to bits 0/2, z to 0/4
+  const unsigned mx = (1u << (lox - W.wx)) | (1u << (hix - W.wx));
+  const unsigned my = (1u << (loy - W.wy)) | (1u << (hiy - W.wy));
+  const unsigned mz = (1u << (loz - W.wz)) | (1u << (hiz - W.wz));
+  const unsigned need = mx * ((my & 1u) | ((my & 2u) << 1)) * ((mz & 1u) | ((mz & 2u) << 3));
+  unsigned todo = need & ~W.resolved;
+  while (todo) {
+    const int k = __builtin_ctz(todo);
+    todo &= todo - 1u;
+    const int ptr = find_block(v, W.wx + (k & 1), W.wy + ((k >> 1) & 1), W.wz + (k >> 2));
+    W.slot[k * 256] = ptr < 0 ? v.num_blocks : ptr;
+    W.resolved |= 1u << k;
+  }
+  const uint64_t *vox = v.vox;
+  auto word = [&](int gx, int gy, int gz) -> uint64_t {
+    const int k = ((gx >> 3) - W.wx) + 2 * ((gy >> 3) - W.wy) + 4 * ((gz >> 3) - W.wz);
+    const uint32_t vi = (uint32_t)W.slot[k * 256] * (uint32_t)kBlockVox +
+                        (uint32_t)(mod8(gx) * 64 + mod8(gy) * 8 + mod8(gz));
+    return ldg(vox + vi);
+  };
+  const int ci = (X0.c - i0) + 2 * (Y0.c - j0) + 4 * (Z0.c - k0);
+  float f[8];
+  uint32_t obs = 0u;
+  {
+    uint64_t wc[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) wc[o] = word(i0 + (o & 1), j0 + ((o >> 1) & 1), k0 + (o >> 2));
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      f[o] = vox_sdf(wc[o]);
+      obs |= ((wc[o] >> 56) != 0 ? 1u : 0u) << o;
+    }
+    // colour sample: interp_axes' colour sums with the centre standing in
+    // for unobserved corners
+    has_rgb = (obs >> ci) & 1u;
+    if (has_rgb) {
+      const uint32_t cc = vox_rgbw(ci & 4 ? ((ci & 2) ? ((ci & 1) ? wc[7] : wc[6]) : ((ci & 1) ? wc[5] : wc[4]))
+                                          : ((ci & 2) ? ((ci & 1) ? wc[3] : wc[2]) : ((ci & 1) ? wc[1] : wc[0])));
+      float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int ox = (0xD2 >> k) & 1, oy = (0xB4 >> k) & 1, oz = (0xE8 >> k) & 1;
+        const int o = ox + 2 * oy + 4 * oz;
+        const float a = ox ? X0.w : 1.0f - X0.w;
+        const float b = oy ? Y0.w : 1.0f - Y0.w;
+        const float g = oz ? Z0.w : 1.0f - Z0.w;
+        const float coef = a * b * g;
+        const uint32_t rgbw = ((obs >> o) & 1u) ? vox_rgbw(wc[o]) : cc;
+        c0 += coef * (float)(rgbw & 0xff);
+        c1 += coef * (float)((rgbw >> 8) & 0xff);
+        c2 += coef * (float)((rgbw >> 16) & 0xff);
+      }
+      rgb[0] = to_u8(c0); rgb[1] = to_u8(c1); rgb[2] = to_u8(c2);
+    }
+  }
+  ok = true;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    float sides[2];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      // one side's 4 loads at a time (keeps the other words out of registers)
+      __builtin_amdgcn_sched_barrier(0);
+      // side 0: the - sample, whose corners along the axis are the 4 voxels
+      // before the core (index i - 1) and then the core's lower layer; side 1:
+      // the + sample, the core's upper layer and then the 4 voxels after it
+      // (index i + 2).  t = a1 + 2 a2 numbers the other two axes' bits in x,
+      // y, z order.
+      float fe[4];
+      uint32_t oe = 0u;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int a1 = t & 1, a2 = t >> 1;
+        const int d = side ? 2 : -1;
+        const int gx = ax == 0 ? i0 + d : i0 + a1, gy = ax == 1 ? j0 + d : j0 + (ax == 0 ? a1 : a2),
+                  gz = ax == 2 ? k0 + d : k0 + a2;
+        const uint64_t w = word(gx, gy, gz);
+        fe[t] = vox_sdf(w);
+        oe |= ((w >> 56) != 0 ? 1u : 0u) << t;
+      }
+      float F[8];
+      uint32_t O = 0u;
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        const int oa = (o >> ax) & 1;
+        const int t = ax == 0 ? ((o >> 1) & 1) + 2 * (o >> 2)
+                      : ax == 1 ? (o & 1) + 2 * (o >> 2) : (o & 1) + 2 * ((o >> 1) & 1);
+        const bool core = side ? !oa : oa;  // the core voxel on this corner, else the extra one
+        const int oc = side ? (o | (1 << ax)) : (o & ~(1 << ax));
+        F[o] = core ? f[oc] : fe[t];
+        O |= (core ? ((obs >> oc) & 1u) : ((oe >> t) & 1u)) << o;
+      }
+      const float wa = side ? wp[ax] : wm[ax];
+      const f3 wt = mk(ax == 0 ? wa : X0.w, ax == 1 ? wa : Y0.w, ax == 2 ? wa : Z0.w);
+      float sv = 0.f;
+      ok &= blend8(F, O, ci, wt, sv);
+      sides[side] = sv;
+    }
+    grad[ax] = sides[1] - sides[0];
+  }
+}
+
 // Empty-space test of the raycast.  A sample is "far" when every block that
 // can hold one of the voxels GetInterpolatedVoxel reads at it is unallocated
 // or has no observed voxel below the far threshold: the sample is then
@@ -1038,19 +1193,36 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
     // One (not unrolled) copy of the gather keeps the kernel's code small.
     const AxisSample X0 = axis_sample(pw.x, vs, inv_vs), Y0 = axis_sample(pw.y, vs, inv_vs),
                      Z0 = axis_sample(pw.z, vs, inv_vs);
-    float gx = 0.f, gy = 0.f, gz = 0.f;
-    bool ok = true;
-#pragma unroll 1
+    float wp[3], wm[3];
+    bool reg = true;
+#pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
       const float q = ax == 0 ? pw.x : ax == 1 ? pw.y : pw.z;
       const AxisSample P = axis_sample(q + vs, vs, inv_vs), M = axis_sample(q - vs, vs, inv_vs);
-      float sp, sm;
-      ok &= interp_axes(v, ax == 0 ? P : X0, ax == 1 ? P : Y0, ax == 2 ? P : Z0, c, sp, nullptr);
-      ok &= interp_axes(v, ax == 0 ? M : X0, ax == 1 ? M : Y0, ax == 2 ? M : Z0, c, sm, nullptr);
-      const float g = sp - sm;
-      if (ax == 0) gx = g; else if (ax == 1) gy = g; else gz = g;
+      reg &= stencil_regular(ax == 0 ? X0 : ax == 1 ? Y0 : Z0, P, M);
+      wp[ax] = P.w;
+      wm[ax] = M.w;
     }
-    {
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    bool ok = true;
+    if (reg) {
+      float g[3];
+      uint32_t rgb[3];
+      bool has_rgb;
+      hit_stencil(v, X0, Y0, Z0, wp, wm, c, g, ok, rgb, has_rgb);
+      gx = g[0]; gy = g[1]; gz = g[2];
+      if (has_rgb) { cr = rgb[0]; cg = rgb[1]; cb = rgb[2]; }
+    } else {
+#pragma unroll 1
+      for (int ax = 0; ax < 3; ++ax) {
+        const float q = ax == 0 ? pw.x : ax == 1 ? pw.y : pw.z;
+        const AxisSample P = axis_sample(q + vs, vs, inv_vs), M = axis_sample(q - vs, vs, inv_vs);
+        float sp, sm;
+        ok &= interp_axes(v, ax == 0 ? P : X0, ax == 1 ? P : Y0, ax == 2 ? P : Z0, c, sp, nullptr);
+        ok &= interp_axes(v, ax == 0 ? M : X0, ax == 1 ? M : Y0, ax == 2 ? M : Z0, c, sm, nullptr);
+        const float g = sp - sm;
+        if (ax == 0) gx = g; else if (ax == 1) gy = g; else gz = g;
+      }
       float sv, col[3];
       if (interp_axes(v, X0, Y0, Z0, c, sv, col)) { cr = to_u8(col[0]); cg = to_u8(col[1]); cb = to_u8(col[2]); }
     }
