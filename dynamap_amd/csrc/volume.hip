@@ -799,12 +799,12 @@ __device__ __forceinline__ bool interp_axes(const DmView &v, const AxisSample &X
   const int ky[2] = {2 * ((iy[0] >> 3) - W.wy), 2 * ((iy[1] >> 3) - W.wy)};
   const int kz[2] = {4 * ((iz[0] >> 3) - W.wz), 4 * ((iz[1] >> 3) - W.wz)};
   const int kc = ((cx >> 3) - W.wx) + 2 * ((cy >> 3) - W.wy) + 4 * ((cz >> 3) - W.wz);
-  unsigned need = 1u << kc;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int ox = (0xD2 >> k) & 1, oy = (0xB4 >> k) & 1, oz = (0xE8 >> k) & 1;
-    need |= 1u << (kx[ox] + ky[oy] + kz[oz]);
-  }
+  // slots of the 8 corners: the product of the per-axis slot sets (x bits
+  // 0-1, y spread to bits 0/2, z to bits 0/4), and the centre's slot
+  const unsigned mx = (1u << kx[0]) | (1u << kx[1]);
+  const unsigned my = (1u << (ky[0] >> 1)) | (1u << (ky[1] >> 1));
+  const unsigned mz = (1u << (kz[0] >> 2)) | (1u << (kz[1] >> 2));
+  const unsigned need = (mx * ((my & 1u) | ((my & 2u) << 1)) * ((mz & 1u) | ((mz & 2u) << 3))) | (1u << kc);
   unsigned todo = need & ~W.resolved;
   while (todo) {
     const int k = __builtin_ctz(todo);
@@ -1153,6 +1153,16 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
       zprev = z;
       z = z + big;
       const float zlim = fminf(zmax, zpre_max);  // z < max_depth <=> z <= zpre_max
+      // while (z <= zlim && it < it_last) { ++it; zprev = z; z = z + big; }, four
+      // steps at a time: the sums increase, so the fourth step's start within
+      // zlim admits all four
+      while (it + 4 <= it_last) {
+        const float z1 = z + big, z2 = z1 + big, z3 = z2 + big;
+        if (!(z3 <= zlim)) break;
+        it += 4;
+        zprev = z3;
+        z = z3 + big;
+      }
       while (z <= zlim && it < it_last) {
         ++it;
         zprev = z;
