@@ -331,6 +331,10 @@ def main():
                      "alloc_rays": stage_ms["alloc_rays"]}
     dom = max(stage_kernels, key=stage_kernels.get)
     P = work["pixels"]
+    # pixels with a colour image (the 8 mm sessions): the traversal reads
+    # their colour and writes every pixel's surface distance and colour (8 B)
+    # for the integration
+    P_rgb = sum(t[1].shape[1] * t[1].shape[2] for t in traces if t[2] is not None)
     nvis = work["visible_blocks"]
     vb = 8
     if dom == "integrate":
@@ -338,7 +342,7 @@ def main():
     elif dom == "raycast":
         bytes_ = nvis * 512 * 4 + P * (4 + 12 + 12 + 3)
     else:
-        bytes_ = 4 * P + 16 * work["block_visits"]
+        bytes_ = 4 * P + 3 * P_rgb + 8 * P + 16 * work["unique_blocks"]
     t_s = stage_kernels[dom] * 1e-3
     achieved = bytes_ / t_s / 1e9
     roofline = dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
@@ -347,7 +351,7 @@ def main():
                     stage_ms={k: round(v, 4) for k, v in stage_ms.items()})
     # every stage against the same roofline (algorithmic bytes of one batched
     # launch over all sessions / its HIP-event duration)
-    alg = {"alloc_all": 4 * P + 16 * work["unique_blocks"] + 16 * work["new_blocks"],
+    alg = {"alloc_all": 4 * P + 3 * P_rgb + 8 * P + 16 * work["unique_blocks"] + 16 * work["new_blocks"],
            "integrate": nvis * 512 * 2 * vb + P * (4 + 3),
            "raycast": nvis * 512 * 4 + P * (4 + 12 + 12 + 3),
            "icp_l0": 64 * P}

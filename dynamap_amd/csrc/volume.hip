@@ -165,6 +165,32 @@ __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r, unsigned
   }
   const long long bound64 = llabs((long long)ex - bx) + llabs((long long)ey - by) + llabs((long long)ez - bz);
   const int bound = (int)min(bound64, (long long)0x7fffffff);
+  // Every step moves one axis by one block, so the traversal stays within
+  // `bound` blocks of (bx, by, bz).  When that cube lies inside the window
+  // (and so inside the key range), the window index of the current block
+  // follows the steps, and only blocks whose allocated bit is clear go
+  // through visit_block (which starts with that same test).
+  uint32_t widx;
+  if (!kCount && bound < kWin / 2 && win_bit(bx - bound, by - bound, bz - bound, widx) &&
+      win_bit(bx + bound, by + bound, bz + bound, widx)) {
+    win_bit(bx, by, bz, widx);
+    const int ix = stx, iy = sty * kWin, iz = stz * kWin * kWin;
+    for (int it = 0; it < bound && (bx != ex || by != ey || bz != ez); ++it) {
+      const bool sx = mtx < mty && mtx < mtz;
+      const bool sy = !(mtx < mty) && mty < mtz;
+      const bool sz = !sx && !sy;
+      bx += sx ? stx : 0;
+      mtx += sx ? dtx : 0.0f;
+      by += sy ? sty : 0;
+      mty += sy ? dty : 0.0f;
+      bz += sz ? stz : 0;
+      mtz += sz ? dtz : 0.0f;
+      widx += (uint32_t)(sx ? ix : sy ? iy : iz);
+      if (!test_bit(v.allocmap, widx)) visit_block<kCount>(v, bx, by, bz, seen);
+      ++nvisit;
+    }
+    return nvisit;
+  }
   for (int it = 0; it < bound && (bx != ex || by != ey || bz != ez); ++it) {
     // the reference's nested comparisons, as selects: x when mtx is below
     // both, else y when mty < mtz, else z
