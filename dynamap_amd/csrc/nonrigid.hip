@@ -806,9 +806,21 @@ __global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_spmv(PcgState S, const 
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const double *__restrict__ Minv, int nn,
                                                            int it, const NrFrame *Fp) {
   if (S.flag[0]) return;
-  __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1], l1[1];
-  const double gamma = sum_partials(S.part_rz + (size_t)(it & 1) * S.nwg, S.nwg, l1);
-  const double delta = sum_partials(S.part_pq, S.nwg_s, l1);
+  __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1], l2[2];
+  // gamma (wavefront 0) and delta (wavefront 1) summed side by side, each in
+  // sum_partials' fixed order
+  if (threadIdx.x < 128) {
+    const int lane = threadIdx.x & 63;
+    const bool second = threadIdx.x >= 64;
+    const double *part = second ? S.part_pq : S.part_rz + (size_t)(it & 1) * S.nwg;
+    const int n = second ? S.nwg_s : S.nwg;
+    double acc = 0.0;
+    for (int k = lane; k < n; k += 64) acc += part[k];
+    acc = wave_sum(acc);
+    if (lane == 0) l2[second ? 1 : 0] = acc;
+  }
+  __syncthreads();
+  const double gamma = l2[0], delta = l2[1];
   const double tol = Fp->pcg_tol;
   const bool stop = it == 0 ? !(gamma > 0.0) : (!(gamma > tol * tol * S.rz[0]) || !(gamma > 0.0));
   double beta = 0.0, alpha = gamma / delta;
