@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--no-nonrigid", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--icp-reduce", choices=("shuffle", "mfma"), default="shuffle",
+                    help="ICP linearisation kernel: shuffle-tree reduction or FP32 matrix-core F^T F")
     ap.add_argument("--nonrigid-frames", type=int, default=20)
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--no-stress", action="store_true", help="skip the 65-session stress mix")
@@ -244,11 +246,18 @@ def serve_line(dm, traces, nf, K, W, args, label):
     out = {}
     # launch shapes chosen by the autotuner on this workload's steady state
     tuning = "defaults (16x16 raycast tiles, 2048 integration workgroups per session)"
+    shapes = dict(ray_tile_width=16, integrate_workgroups=2048)
     if not args.no_autotune:
         srv.run(N.DM_RUN_BATCHED, 0, F0 + W)
         tuning = srv.autotune(frame=F0 + W, reps=3)
         srv.reset()
         log(label, "autotune", tuning)
+        shapes = dict(ray_tile_width=int(tuning["ray_tile"].split("x")[0]),
+                      integrate_workgroups=tuning["integrate_workgroups"])
+    if args.icp_reduce != "shuffle":
+        srv.set_shapes(icp_reduce=args.icp_reduce, **shapes)
+        tuning = dict(tuning, icp_reduce=args.icp_reduce) if isinstance(tuning, dict) else \
+            f"{tuning}, ICP on the matrix cores"
     # batched closed loop
     srv.run(N.DM_RUN_BATCHED, 0, F0 + W)
     r = srv.run(N.DM_RUN_BATCHED, F0 + W, K, want_poses=True)

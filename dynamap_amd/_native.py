@@ -107,6 +107,7 @@ PROTOTYPES = {
     "dm_session_source_maps": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp]),
     "dm_session_set_model_maps": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp]),
     "dm_session_icp_system": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp]),
+    "dm_session_set_icp_reduce": (C.c_int, [_vp, C.c_int32]),
     "dm_session_icp_track": (C.c_int, [_vp, C.c_int32, _vp]),
     "dm_session_icp_increment": (C.c_int, [_vp, _vp]),
     "dm_server_create": (C.c_int, [_P(dm_server_config), _P(_vp)]),
@@ -118,6 +119,7 @@ PROTOTYPES = {
                                 _P(C.c_int64)]),
     "dm_server_reset": (C.c_int, [_vp]),
     "dm_server_autotune": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp, _vp]),
+    "dm_server_set_shapes": (C.c_int, [_vp, _vp]),
     "dm_server_profile": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp, _vp]),
 }
 

@@ -184,6 +184,7 @@ Launch Sess::launch() const {
   L.max_tiles = ((h.cols + 15) / 16) * ((h.rows + 15) / 16);
   L.max_rows = h.rows;
   L.max_cols = h.cols;
+  L.icp_mfma = icp_mfma;
   L.stream = stream;
   return L;
 }
@@ -787,6 +788,12 @@ int dm_session_set_model_maps(dm_session *s, int32_t level, const float *host_ve
   int one = 1;
   DM_HIP(hipMemcpyAsync(s->h.ctr + C_HAS_MODEL, &one, 4, hipMemcpyHostToDevice, s->stream));
   DM_HIP(hipStreamSynchronize(s->stream));
+  return DM_OK;
+}
+
+int dm_session_set_icp_reduce(dm_session *s, int32_t variant) {
+  if (!s || (variant != 0 && variant != 1)) { set_error("ICP variant is 0 (shuffle tree) or 1 (matrix cores)"); return DM_E_INVALID; }
+  s->icp_mfma = variant;
   return DM_OK;
 }
 

@@ -132,6 +132,7 @@ struct Launch {
   int max_rows, max_cols;  // largest frame among active sessions
   int ray_tw_log2 = 4;  // raycast tile: 2^ray_tw_log2 x (256 >> ray_tw_log2) pixels
   int integ_wg = 2048;  // integration workgroups per session
+  int icp_mfma = 0;     // ICP linearisation on the matrix cores (k_icp_reduce_mfma) instead of the shuffle tree
   hipStream_t stream;
 };
 

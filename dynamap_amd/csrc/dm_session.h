@@ -45,6 +45,7 @@ struct Sess {
   uint8_t *d_ray_rgb = nullptr;
   float *d_depth_f = nullptr;    // bilateral-filtered depth for tracking (when enabled)
   int filt_r = 0;                // filter radius, 0 = off
+  int icp_mfma = 0;              // ICP linearisation on the matrix cores (dm_session_set_icp_reduce)
   float filt_sd = 0.f, filt_ss = 0.f;
   size_t hbm_bytes = 0;          // device bytes owned
   int frames = 0;

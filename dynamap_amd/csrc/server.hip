@@ -55,8 +55,9 @@ struct dm_server {
   uint64_t used_bytes = 0;
   hipGraphExec_t graph = nullptr;
   int graph_iters[3] = {0, 0, 0};
-  int ray_tw_log2 = 4;               // launch shapes (dm_server_autotune)
+  int ray_tw_log2 = 4;               // launch shapes (dm_server_autotune, dm_server_set_shapes)
   int integ_wg = 2048;
+  int icp_mfma = 0;                  // ICP linearisation: 0 shuffle tree, 1 matrix cores
 };
 
 namespace dm {
@@ -162,6 +163,7 @@ static Launch server_launch(const dm_server *srv) {
   L.max_cols = (int)srv->cfg.max_cols;
   L.ray_tw_log2 = srv->ray_tw_log2;
   L.integ_wg = srv->integ_wg;
+  L.icp_mfma = srv->icp_mfma;
   L.stream = srv->stream;
   return L;
 }
@@ -517,6 +519,7 @@ int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int32_t nfr
       const uint8_t *c;
       frame_ptrs(t, f, d, c);
       double pose[16];
+      S.icp_mfma = srv->icp_mfma;  // the per-frame path runs the server's ICP variant too
       r = dm_session_process_frame_dev(reinterpret_cast<dm_session *>(&S), d, c, host_solve ? 1 : 0, pose);
       if (r != DM_OK) return r;
       const double done = secs_since(t0);
@@ -680,7 +683,7 @@ int dm_server_profile(dm_server *srv, int32_t frame, int32_t reps, double *stage
 // session on the state of `frame`, keeps the fastest of each, and captures
 // the serving graph again with them.  The sessions' volumes are advanced by
 // the timing runs: call dm_server_reset before serving.
-int dm_server_autotune(dm_server *srv, int32_t frame, int32_t reps, int32_t choice[2], double *ms) {
+int dm_server_autotune(dm_server *srv, int32_t frame, int32_t reps, int32_t choice[3], double *ms) {
   if (!srv || reps <= 0) { set_error("invalid autotune arguments"); return DM_E_INVALID; }
   std::vector<int> act;
   int64_t pixels = 0;
@@ -721,13 +724,57 @@ int dm_server_autotune(dm_server *srv, int32_t frame, int32_t reps, int32_t choi
     if (ms) ms[4 + k] = t;
     if (t < best) { best = t; best_wg = wgs[k]; }
   }
+  // the ICP linearisation at each level, shuffle tree and matrix cores
+  // (reduction only; the per-level stop flag is cleared before each launch
+  // so that converged sessions do not return at once).  Reported, not
+  // switched: the two sum in different orders, so the choice would change
+  // the poses (dm_server_set_shapes selects it).
+  for (int var = 0; var < 2; ++var) {
+    for (int level = 0; level < kLevels; ++level) {
+      Launch L = server_launch(srv);
+      L.icp_mfma = var;
+      float t = 0.f;
+      for (int k = 0; k <= reps; ++k) {  // k = 0: warm-up
+        for (int a : act) {
+          Sess &S = *srv->traces[a].sess;
+          DM_HIP(hipMemsetAsync(S.h.ctr + C_ICP_STOP, 0, sizeof(int32_t), srv->stream));
+        }
+        DM_HIP(hipEventRecord(e0, srv->stream));
+        launch_icp_reduce(L, level, 1, false);
+        DM_HIP(hipEventRecord(e1, srv->stream));
+        DM_HIP(hipEventSynchronize(e1));
+        float one = 0.f;
+        DM_HIP(hipEventElapsedTime(&one, e0, e1));
+        if (k > 0) t += one;
+      }
+      t /= (float)reps;
+      if (ms) ms[7 + 3 * var + level] = t;
+    }
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   srv->ray_tw_log2 = best_tw;
   srv->integ_wg = best_wg;
   if (srv->graph) { hipGraphExecDestroy(srv->graph); srv->graph = nullptr; }
-  if (choice) { choice[0] = 1 << best_tw; choice[1] = best_wg; }
+  if (choice) { choice[0] = 1 << best_tw; choice[1] = best_wg; choice[2] = srv->icp_mfma; }
   DM_HIP(hipGetLastError());
+  return DM_OK;
+}
+
+int dm_server_set_shapes(dm_server *srv, const int32_t shapes[3]) {
+  if (!srv || !shapes) { set_error("invalid arguments"); return DM_E_INVALID; }
+  const int tw = shapes[0], wg = shapes[1], icp = shapes[2];
+  int tw_log2 = -1;
+  for (int l = 3; l <= 6; ++l)
+    if (tw == (1 << l)) tw_log2 = l;
+  if (tw_log2 < 0 || wg < 1 || wg > 65535 || (icp != 0 && icp != 1)) {
+    set_error("shapes: tile width 8, 16, 32 or 64; integration workgroups 1..65535; ICP variant 0 or 1");
+    return DM_E_INVALID;
+  }
+  srv->ray_tw_log2 = tw_log2;
+  srv->integ_wg = wg;
+  srv->icp_mfma = icp;
+  if (srv->graph) { hipGraphExecDestroy(srv->graph); srv->graph = nullptr; }
   return DM_OK;
 }
 

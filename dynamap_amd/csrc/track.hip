@@ -97,22 +97,24 @@ __device__ __forceinline__ bool bad3(float4 n) {
 
 constexpr int kVals = 30;
 
-// Per-pixel terms of one Gauss-Newton linearisation at pyramid level `level`
-// (buildNLSKernel, gnSolver.cu:55-195) accumulated into vals (upper JtJ 21,
-// Jtr 6, geometric count, r^2, photometric count).  K: Project()
-// association, the |cos| > cos(pi/10) normal gate, r = (p - q).n,
-// J = [p x n; n].  D: source normal rotated by Tinc, pixel bounds check,
-// |p - q| <= dist_max, depth gate [icp_dmin, icp_dmax], photometric term with
-// weight w_photo.
-__device__ __forceinline__ void icp_pixel(const DmView &v, int level, const Intr &s, int i, float *vals) {
+// Association and gates of one pixel (buildNLSKernel, gnSolver.cu:55-195).
+// K: Project() association, the |cos| > cos(pi/10) normal gate, r =
+// (p - q).n, J = [p x n; n].  D: source normal rotated by Tinc, pixel bounds
+// check, |p - q| <= dist_max, depth gate [icp_dmin, icp_dmax].  The
+// transform, projection and gates are in FP64 (the pixel choice and the
+// inclusion decisions follow the FP64 geometry); residual and Jacobian are
+// FP32.
+struct IcpAssoc {
+  f3 p, pq, n;    // transformed source point, p - q, model normal
+  float qw, vw;   // model and source intensities
+  int pu, pv, j;  // associated model pixel
+};
+__device__ __forceinline__ bool icp_assoc(const DmView &v, int level, const Intr &s, int i, IcpAssoc &a) {
   const float4 V = v.svert[level][i];
   const float4 N = v.snorm[level][i];
   const float4 *tv = v.tvert[level];
   const float4 *tn = v.tnorm[level];
   bool ok = V.z >= v.icp_dmin && V.z <= v.icp_dmax && !bad3(N);
-  // D: transform, projection and gates in FP64 (the pixel choice and the
-  // inclusion decisions follow the FP64 geometry); residual and Jacobian
-  // in FP32 below
   const double *Td = v.posed + PD_TINC;
   double px = 0.0, py = 0.0, pz = 0.0;
   int pu = -1, pv = -1, j = 0;
@@ -132,7 +134,6 @@ __device__ __forceinline__ void icp_pixel(const DmView &v, int level, const Intr
     NT = tn[j];
     ok = Q.z >= v.icp_dmin && Q.z <= v.icp_dmax && !bad3(NT);
   }
-  f3 p = mk(0.f, 0.f, 0.f), pq = p;
   if (ok) {
     const double X = N.x, Y = N.y, Z = N.z;
     const double nrx = Td[0] * X + Td[1] * Y + Td[2] * Z;
@@ -141,13 +142,29 @@ __device__ __forceinline__ void icp_pixel(const DmView &v, int level, const Intr
     const double ct = (double)NT.x * nrx + (double)NT.y * nry + (double)NT.z * nrz;
     const double dqx = px - (double)Q.x, dqy = py - (double)Q.y, dqz = pz - (double)Q.z;
     ok = fabs(ct) > 0.95105651629515357 && sqrt(dqx * dqx + dqy * dqy + dqz * dqz) <= (double)v.dist_max;
-    p = mk((float)px, (float)py, (float)pz);
-    pq = mk((float)dqx, (float)dqy, (float)dqz);
+    a.p = mk((float)px, (float)py, (float)pz);
+    a.pq = mk((float)dqx, (float)dqy, (float)dqz);
   }
-  if (!ok) return;
+  a.n = mk(NT.x, NT.y, NT.z);
+  a.qw = Q.w;
+  a.vw = V.w;
+  a.pu = pu;
+  a.pv = pv;
+  a.j = j;
+  return ok;
+}
+
+// Per-pixel terms of one Gauss-Newton linearisation at pyramid level `level`
+// accumulated into vals (upper JtJ 21, Jtr 6, geometric count, r^2,
+// photometric count); the photometric term adds w_photo Jp Jp^T and
+// w_photo Jp rp.
+__device__ __forceinline__ void icp_pixel(const DmView &v, int level, const Intr &s, int i, float *vals) {
+  IcpAssoc as;
+  if (!icp_assoc(v, level, s, i, as)) return;
+  const float4 *tv = v.tvert[level];
+  const f3 p = as.p, n = as.n;
   float pvl[kVals];
-  const f3 n = mk(NT.x, NT.y, NT.z);
-  const float r = dot(pq, n);
+  const float r = dot(as.pq, n);
   const f3 c = cross(p, n);
   const float J[6] = {c.x, c.y, c.z, n.x, n.y, n.z};
   int k = 0;
@@ -160,10 +177,11 @@ __device__ __forceinline__ void icp_pixel(const DmView &v, int level, const Intr
   pvl[27] = 1.f;
   pvl[28] = r * r;
   pvl[29] = 0.f;
-  if (v.w_photo > 0.f && pu >= 1 && pv >= 1 && pu < s.cols - 1 && pv < s.rows - 1) {
+  if (v.w_photo > 0.f && as.pu >= 1 && as.pv >= 1 && as.pu < s.cols - 1 && as.pv < s.rows - 1) {
+    const int j = as.j;
     const float gx = (tv[j + 1].w - tv[j - 1].w) * 0.5f;
     const float gy = (tv[j + s.cols].w - tv[j - s.cols].w) * 0.5f;
-    const float rp = Q.w - V.w;
+    const float rp = as.qw - as.vw;
     const float iz = 1.0f / p.z;
     const f3 G = mk(gx * s.fx * iz, gy * s.fy * iz, -(gx * s.fx * p.x + gy * s.fy * p.y) * iz * iz);
     const f3 cg = cross(p, G);
@@ -180,6 +198,35 @@ __device__ __forceinline__ void icp_pixel(const DmView &v, int level, const Intr
   }
 #pragma unroll
   for (int q = 0; q < kVals; ++q) vals[q] = vals[q] + pvl[q];
+}
+
+// The feature row of one pixel for the matrix-core reduction: geometric
+// [J, r, 1] in columns 0-7 and photometric sqrt(w_photo) [Jp, rp] with a 1 in
+// columns 8-15 (zeros where a term does not apply).  F^T F summed over the
+// pixels holds J^T J and J^T r of both terms, r^2 and both counts.
+__device__ __forceinline__ void icp_features(const DmView &v, int level, const Intr &s, int i, float swp,
+                                             float *f) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k) f[k] = 0.f;
+  IcpAssoc as;
+  if (!icp_assoc(v, level, s, i, as)) return;
+  const float4 *tv = v.tvert[level];
+  const f3 p = as.p, n = as.n;
+  const float r = dot(as.pq, n);
+  const f3 c = cross(p, n);
+  f[0] = c.x; f[1] = c.y; f[2] = c.z; f[3] = n.x; f[4] = n.y; f[5] = n.z; f[6] = r; f[7] = 1.f;
+  if (v.w_photo > 0.f && as.pu >= 1 && as.pv >= 1 && as.pu < s.cols - 1 && as.pv < s.rows - 1) {
+    const int j = as.j;
+    const float gx = (tv[j + 1].w - tv[j - 1].w) * 0.5f;
+    const float gy = (tv[j + s.cols].w - tv[j - s.cols].w) * 0.5f;
+    const float rp = as.qw - as.vw;
+    const float iz = 1.0f / p.z;
+    const f3 G = mk(gx * s.fx * iz, gy * s.fy * iz, -(gx * s.fx * p.x + gy * s.fy * p.y) * iz * iz);
+    const f3 cg = cross(p, G);
+    f[8] = swp * cg.x; f[9] = swp * cg.y; f[10] = swp * cg.z;
+    f[11] = swp * G.x; f[12] = swp * G.y; f[13] = swp * G.z;
+    f[14] = swp * rp; f[15] = 1.f;
+  }
 }
 
 // The damped Gauss-Newton step from the summed records: Tinc <- exp(xi)
@@ -254,6 +301,81 @@ __global__ __launch_bounds__(256) void k_icp_reduce(const DmView *__restrict__ v
   if (threadIdx.x < kVals) {
     const int k = threadIdx.x;
     v.partials[(size_t)blockIdx.x * kIcpVals + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+  }
+}
+
+// The same linearisation on the matrix cores: each pixel's 16-float feature
+// row (icp_features) is staged through LDS and v_mfma_f32_16x16x4f32
+// accumulates F^T F over 4 rows per instruction (lane l supplies feature l %
+// 16 of row l / 16 as both operands; the 16x16 result holds column l % 16,
+// rows 4 (l / 16) .. + 3 in lane l).  The four waves' tiles are added in a
+// fixed order and the record takes J^T J = G + P (the geometric and
+// photometric 6x6 blocks), J^T r, the counts and r^2 from the tile.  Same
+// grid and record layout as k_icp_reduce; the sums differ from it by
+// rounding only (w_photo enters as sqrt(w_photo)^2).
+typedef float icp_f4 __attribute__((ext_vector_type(4)));
+constexpr int kFeatPitch = 17;  // LDS row pitch (floats): rows of a wave fall on distinct banks
+__global__ __launch_bounds__(256) void k_icp_reduce_mfma(const DmView *__restrict__ views,
+                                                          const int *__restrict__ active, int level, int it) {
+  const int slot_ = active[blockIdx.y];
+  if (slot_ < 0) return;
+  const DmView &v = views[slot_];
+  if (v.ctr[C_HAS_MODEL] == 0 || it > v.iters[level] || v.ctr[C_ICP_STOP] == level + 1) return;
+  const Intr s = level_intr(v, level);
+  const int P = s.rows * s.cols;
+  const int nwg = icp_wg_count(P);
+  if ((int)blockIdx.x >= nwg) return;
+  __shared__ float feat[4][64 * kFeatPitch];
+  __shared__ float tile[4][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float *F = feat[wave];
+  const float swp = v.w_photo > 0.f ? sqrtf(v.w_photo) : 0.f;
+  icp_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int q = 0; q < kIcpPix / 256; ++q) {
+    const int i = blockIdx.x * kIcpPix + q * 256 + threadIdx.x;
+    float f[16];
+    if (i < P) {
+      icp_features(v, level, s, i, swp, f);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) f[k] = 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) F[lane * kFeatPitch + k] = f[k];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const float a = F[(4 * m + (lane >> 4)) * kFeatPitch + (lane & 15)];
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, a, acc, 0, 0, 0);
+    }
+    __syncthreads();  // the rows are rewritten by the next group of pixels
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) tile[wave][(4 * (lane >> 4) + e) * 16 + (lane & 15)] = acc[e];
+  __syncthreads();
+  if (threadIdx.x < kVals) {
+    const int k = threadIdx.x;
+    auto T = [&](int r, int c) {
+      const int o = r * 16 + c;
+      return (tile[0][o] + tile[1][o]) + (tile[2][o] + tile[3][o]);
+    };
+    float val;
+    if (k < 21) {
+      int a = 0, kk = k;
+      while (kk >= 6 - a) { kk -= 6 - a; ++a; }
+      const int b = a + kk;
+      val = T(a, b) + T(8 + a, 8 + b);
+    } else if (k < 27) {
+      val = T(k - 21, 6) + T(8 + (k - 21), 14);
+    } else if (k == 27) {
+      val = T(7, 7);
+    } else if (k == 28) {
+      val = T(6, 6);
+    } else {
+      val = T(15, 15);
+    }
+    v.partials[(size_t)blockIdx.x * kIcpVals + k] = val;
   }
 }
 
@@ -332,8 +454,12 @@ void launch_source_maps(const Launch &L) {
 void launch_icp_reduce(const Launch &L, int level, int it, bool solve) {
   if (L.nact <= 0) return;
   const int P = L.max_pixels >> (2 * level);
-  hipLaunchKernelGGL(k_icp_reduce, dim3((unsigned)icp_wg_count(P), (unsigned)L.nact), dim3(256), 0, L.stream,
-                     L.views, L.active, level, it);
+  if (L.icp_mfma)
+    hipLaunchKernelGGL(k_icp_reduce_mfma, dim3((unsigned)icp_wg_count(P), (unsigned)L.nact), dim3(256), 0, L.stream,
+                       L.views, L.active, level, it);
+  else
+    hipLaunchKernelGGL(k_icp_reduce, dim3((unsigned)icp_wg_count(P), (unsigned)L.nact), dim3(256), 0, L.stream,
+                       L.views, L.active, level, it);
   if (solve)
     hipLaunchKernelGGL(k_icp_solve, dim3((unsigned)L.nact), dim3(256), 0, L.stream, L.views, L.active, level, it);
 }

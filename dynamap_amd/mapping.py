@@ -525,6 +525,13 @@ class Session:
         N.check(self._lib.dm_session_set_model_maps(self._h, level, N.ptr(_f32(vertex)), N.ptr(_f32(normal)),
                                                     N.ptr(None if intensity is None else _f32(intensity))))
 
+    def set_icp_reduce(self, variant):
+        """ICP linearisation kernel: 'shuffle' (shuffle-tree reduction, the
+        default) or 'mfma' (FP32 matrix-core F^T F)."""
+        if variant not in ("shuffle", "mfma"):
+            raise ValueError("variant is 'shuffle' or 'mfma'")
+        N.check(self._lib.dm_session_set_icp_reduce(self._h, 1 if variant == "mfma" else 0))
+
     def icp_system(self, level, Tinc):
         A = np.zeros((6, 6)); b = np.zeros(6); s = np.zeros(3)
         T = np.ascontiguousarray(Tinc, np.float64)
@@ -593,15 +600,25 @@ class Server:
     STAGES = ("source_maps", "icp_l0", "icp_l0_reduce_only", "alloc_all", "integrate", "raycast", "alloc_rays")
 
     def autotune(self, frame=0, reps=3):
-        """Pick the raycast tile shape and integration grid by timing them on
-        the sessions' state at `frame`; the sessions are reset afterwards."""
-        choice = np.zeros(2, np.int32)
-        ms = np.zeros(7)
+        """Pick the raycast tile shape, integration grid and ICP reduction
+        (shuffle tree or matrix cores) by timing them on the sessions' state
+        at `frame`; the sessions are reset afterwards."""
+        choice = np.zeros(3, np.int32)
+        ms = np.zeros(13)
         N.check(self._lib.dm_server_autotune(self._h, int(frame), int(reps), N.ptr(choice), N.ptr(ms)))
         N.check(self._lib.dm_server_reset(self._h))
         return dict(ray_tile=f"{int(choice[0])}x{256 // int(choice[0])}", integrate_workgroups=int(choice[1]),
+                    icp_reduce="mfma" if int(choice[2]) else "shuffle",
                     raycast_ms={f"{w}x{256 // w}": float(t) for w, t in zip((8, 16, 32, 64), ms[:4])},
-                    integrate_ms={str(n): float(t) for n, t in zip((1024, 2048, 4096), ms[4:])})
+                    integrate_ms={str(n): float(t) for n, t in zip((1024, 2048, 4096), ms[4:7])},
+                    icp_reduce_ms={"shuffle": [float(t) for t in ms[7:10]], "mfma": [float(t) for t in ms[10:13]]})
+
+    def set_shapes(self, ray_tile_width=16, integrate_workgroups=2048, icp_reduce="shuffle"):
+        """Set the serving graph's launch shapes (the autotuner's choices)."""
+        if icp_reduce not in ("shuffle", "mfma"):
+            raise ValueError("icp_reduce is 'shuffle' or 'mfma'")
+        shapes = np.array([ray_tile_width, integrate_workgroups, 1 if icp_reduce == "mfma" else 0], np.int32)
+        N.check(self._lib.dm_server_set_shapes(self._h, N.ptr(shapes)))
 
     def profile(self, frame, reps=5):
         ms = np.zeros(7)

@@ -50,6 +50,11 @@ DM_API int dm_session_source_maps(dm_session *s, const float *host_depth, const 
  * tracking in isolation. */
 DM_API int dm_session_set_model_maps(dm_session *s, int32_t level, const float *host_vertex,
                                      const float *host_normal, const float *host_intensity);
+/* Select the kernel that linearises ICP for this session (no reference
+ * counterpart; gnSolver.cu:25-196 has one): 0 the shuffle-tree reduction
+ * (default), 1 the FP32 matrix-core F^T F contraction.  Both give the same
+ * system up to summation order. */
+DM_API int dm_session_set_icp_reduce(dm_session *s, int32_t variant);
 /* buildNLSKernel, gnSolver.cu:25-196: the normal equations of one
  * linearisation at `level` for increment Tinc, using the source maps of the
  * last dm_session_source_maps call and the current model maps.
@@ -122,10 +127,23 @@ DM_API int dm_server_reset(dm_server *srv);
 /* Launch-shape autotuner (no reference counterpart): times the raycast tile
  * shapes (8x32, 16x16, 32x8, 64x4 pixels) and the integration grid (1024,
  * 2048, 4096 workgroups per session) on the admitted sessions' state at
- * `frame`, keeps the fastest of each for the serving graph.  choice (may be
- * NULL) = {tile width, integration workgroups}; ms (may be NULL) = the 7
- * timings.  Advances the sessions' volumes: dm_server_reset before serving. */
-DM_API int dm_server_autotune(dm_server *srv, int32_t frame, int32_t reps, int32_t choice[2], double *ms);
+ * `frame`, and keeps the fastest of each for the serving graph (neither
+ * changes a result).  It also times the ICP linearisation (shuffle-tree
+ * reduction and FP32 matrix-core F^T F, at the three pyramid levels) but
+ * keeps the configured variant, which dm_server_set_shapes selects: the two
+ * sum in different orders, so the poses differ at the rounding level.
+ * choice (may be NULL) = {tile width, integration workgroups, ICP variant
+ * (0 shuffle tree, 1 matrix cores)}; ms (may be NULL) = the 13 timings: 4
+ * raycast, 3 integration, ICP levels 0-2 shuffle tree, ICP levels 0-2
+ * matrix cores.  Advances the sessions' volumes: dm_server_reset before
+ * serving. */
+DM_API int dm_server_autotune(dm_server *srv, int32_t frame, int32_t reps, int32_t choice[3], double *ms);
+/* Set the serving graph's launch shapes directly (no reference counterpart):
+ * shapes = {raycast tile width 8/16/32/64, integration workgroups per
+ * session 1..65535, ICP variant 0 (shuffle tree) or 1 (matrix cores)}, the
+ * values dm_server_autotune chooses.  Poses depend on the ICP variant (the
+ * two sum in different orders). */
+DM_API int dm_server_set_shapes(dm_server *srv, const int32_t shapes[3]);
 
 #ifdef __cplusplus
 }

@@ -86,9 +86,35 @@ def test_autotune_keeps_results(traces):
     print("tuning", tune)
     assert tune["ray_tile"] in ("8x32", "16x16", "32x8", "64x4")
     assert tune["integrate_workgroups"] in (1024, 2048, 4096)
+    # the ICP variants are timed at every level but the configured one stays
+    assert tune["icp_reduce"] == "shuffle"
+    assert all(len(tune["icp_reduce_ms"][k]) == 3 and min(tune["icp_reduce_ms"][k]) > 0 for k in ("shuffle", "mfma"))
     p = srv.run(N.DM_RUN_BATCHED, 0, NF, want_poses=True)["poses"]
     srv.close()
     assert np.array_equal(p, ref)
+
+
+def test_icp_matrix_core_variant(traces):
+    """The FP32 matrix-core linearisation (dm_server_set_shapes) against the
+    shuffle-tree one: the same poses up to summation order, and batched ==
+    per-frame within the variant.  Launch shapes that do not touch the ICP
+    leave the poses bit-equal."""
+    srv = server(traces)
+    ref = srv.run(N.DM_RUN_BATCHED, 0, NF, want_poses=True)["poses"]
+    srv.set_shapes(ray_tile_width=8, integrate_workgroups=1024, icp_reduce="shuffle")
+    same = srv.run(N.DM_RUN_BATCHED, 0, NF, want_poses=True)["poses"]
+    srv.set_shapes(ray_tile_width=16, integrate_workgroups=2048, icp_reduce="mfma")
+    pm = srv.run(N.DM_RUN_BATCHED, 0, NF, want_poses=True)["poses"]
+    pd = srv.run(N.DM_RUN_PER_FRAME_DEV, 0, NF, want_poses=True)["poses"]
+    with pytest.raises(RuntimeError):
+        srv.set_shapes(ray_tile_width=12)
+    srv.close()
+    assert np.array_equal(same, ref)
+    assert np.array_equal(pm, pd)
+    dt = np.abs(pm[..., :3, 3] - ref[..., :3, 3]).max()
+    dr = np.abs(pm[..., :3, :3] - ref[..., :3, :3]).max()
+    print(f"matrix cores vs shuffle tree: max translation {dt:.3g} m, max rotation entry {dr:.3g}")
+    assert dt < 5e-4 and dr < 5e-4
 
 
 def test_continued_runs_and_their_checks(traces):

@@ -51,8 +51,11 @@ def test_source_maps_bit_exact(sess, data):
         np.testing.assert_array_equal(it, ref[l][2])
 
 
+@pytest.mark.parametrize("variant", ["shuffle", "mfma"])
 @pytest.mark.parametrize("level", [0, 1, 2])
-def test_icp_system(sess, data, level):
+def test_icp_system(sess, data, level, variant):
+    """The linearisation of both kernels (shuffle-tree reduction, FP32
+    matrix-core F^T F) against the oracle's A, b and counts."""
     depth, rgb, poses, model = data
     src = O.source_maps(depth[1], rgb[1], SENS, 32)
     sess.source_maps(depth[1], rgb[1], 0)
@@ -60,7 +63,11 @@ def test_icp_system(sess, data, level):
     xi = rng.normal(size=6) * np.array([2e-3, 2e-3, 2e-3, 5e-3, 5e-3, 5e-3])
     L = O.load(64)
     T = np.zeros(16); L.or_se3_exp(O.P(xi), O.P(T)); T = T.reshape(4, 4)
-    A, b, s = sess.icp_system(level, T)
+    sess.set_icp_reduce(variant)
+    try:
+        A, b, s = sess.icp_system(level, T)
+    finally:
+        sess.set_icp_reduce("shuffle")
     Ao, bo, so = O.icp_system(src, model, SENS, level, T, ICP, 32)
     assert s[0] == so[0] and s[2] == so[2] and s[0] > 1000
     scale = np.abs(Ao).max()
