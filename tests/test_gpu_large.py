@@ -38,19 +38,26 @@ def test_large_multiroom_mesh_bit_exact():
 def test_large_1000_frames():
     """The large session at its full 1000 frames: every key fits (no drop),
     the first frames agree with the FP64 oracle within 1 mm ATE, the
-    full-volume mesh fits the extractor, and marching cubes over a 1.5 m box
-    around the first view's surface is bit-exact against the FP32
-    transcription run on the GPU volume's blocks there.  The trajectory's
+    full-volume marching cubes after frame 300 is bit-exact against the FP32
+    transcription run on all of the GPU volume's blocks, the full-volume mesh
+    after frame 1000 fits the extractor, and marching cubes over a 1.5 m box
+    around the first view's surface is bit-exact there too.  The trajectory's
     drift against the ground truth (about 3 cm mean after 1000 frames) is
     the oracle's too: the depth noise at 4 m (sigma 2.6 cm) exceeds the
     2 cm truncation of this workload."""
     name = "large-640x480-4mm"
     w = configs.WORKLOADS[name]
-    nf, no = 1000, 10
+    nf, no, nm = 1000, 10, 300
     depth, rgb, gt = configs.make_sequence(name, nf)
     s = dm.Session(w["tsdf"], w["icp"], w["sensor"], gt[0])
-    G = np.stack([s.process_frame(depth[f], rgb[f]) for f in range(nf)])
+    G = [s.process_frame(depth[f], rgb[f]) for f in range(nm)]
     vol = s.volume
+    m = vol.ExtractMesh(max_triangles=1 << 26)
+    print(f"after {nm} frames: blocks {vol.GetNumAllocatedBlocks()}, triangles {len(m)}", flush=True)
+    assert m.num_triangles_ == len(m) > 100000
+    check_same((m.triangles, m.colors, m.edge_ids), O.mc_extract(mirror(vol)))
+    del m
+    G = np.stack(G + [s.process_frame(depth[f], rgb[f]) for f in range(nm, nf)])
     cnt = vol.counters()
     print("counters", cnt, flush=True)
     assert cnt["dropped_bucket"] == 0 and cnt["dropped_pool"] == 0 and cnt["dropped_keyset"] == 0
