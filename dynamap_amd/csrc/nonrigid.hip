@@ -751,7 +751,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgState S, const doub
 // block entries l / 6, l / 6 + 10, ... of the node's block row, so all the
 // entries' loads are in flight together; the 10 lane sums of a row are
 // added in a fixed order through LDS.  Partials of w.u per workgroup.
-constexpr int kSpmvNodes = 4;  // wavefronts (nodes) per workgroup
+constexpr int kSpmvNodes = 16;  // wavefronts (nodes) per workgroup (fewer partials for the update to add)
 __global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_spmv(PcgState S, const float *__restrict__ H,
                                                              const int *__restrict__ row_start,
                                                              const uint32_t *__restrict__ row_ent,
