@@ -557,9 +557,13 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
-// One wavefront per block: H_ab (and for a == b the rhs, the damping and the
-// FP64 block itself, inverted by k_nr_invdiag).  Lanes take contributions strided by 64; the
-// lane sums are combined by a fixed xor tree.
+// kAsmLanes lanes per block (4 blocks per wavefront): H_ab (and for a == b
+// the rhs, the damping and the FP64 block itself, inverted by k_nr_invdiag).
+// Lanes take contributions strided by kAsmLanes; the lane sums are combined
+// by a fixed xor tree within the block's lanes.  16 lanes give each lane
+// several independent gathers in flight and a 4-level tree (a wavefront per
+// block took about 153 us per assembly at 4048 nodes, these about 80 us).
+constexpr int kAsmLanes = 16;
 __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
                                                     const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
                                                     int nb, const float *__restrict__ vdata,
@@ -568,9 +572,9 @@ __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk
                                                     const double *__restrict__ edata, const NrFrame *Fp,
                                                     float *__restrict__ H, double *__restrict__ rhs,
                                                     double *__restrict__ Hd) {
-  const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (blk >= nb) return;  // uniform per wavefront
+  const int blk = blockIdx.x * (256 / kAsmLanes) + threadIdx.x / kAsmLanes;
+  const int lane = threadIdx.x % kAsmLanes;
+  if (blk >= nb) return;  // uniform per block's lanes (the xor tree stays within them)
   const NrFrame &F = *Fp;
   const int a = blk_a[blk], b = blk_b[blk];
   const bool diag = a == b;
@@ -580,7 +584,7 @@ __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk
 #pragma unroll
   for (int q = 0; q < 6; ++q) g[q] = 0.0;
   const double lam = F.lambda;
-  for (int c = cstart[blk] + lane; c < cstart[blk + 1]; c += 64) {
+  for (int c = cstart[blk] + lane; c < cstart[blk + 1]; c += kAsmLanes) {
     const uint32_t code = codes[c];
     if (!(code & kArapTag)) {
       const int64_t v = code >> 4;
@@ -633,11 +637,15 @@ __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk
       }
     }
   }
+  auto lanes_sum = [](double x) {
 #pragma unroll
-  for (int q = 0; q < 36; ++q) h[q] = wave_sum(h[q]);
-  if (diag)
+    for (int off = kAsmLanes / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+  };
 #pragma unroll
-    for (int q = 0; q < 6; ++q) g[q] = wave_sum(g[q]);
+  for (int q = 0; q < 36; ++q) h[q] = lanes_sum(h[q]);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) g[q] = lanes_sum(g[q]);  // zero off the diagonal
   if (lane != 0) return;
   if (diag)
 #pragma unroll
@@ -1199,7 +1207,7 @@ static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its, int
                          g->mesh_n, F, g->vdata);
     hipLaunchKernelGGL(k_nr_arap, dim3(grid_of(ne)), dim3(256), 0, st, g->T, g->verts, g->node_vid, g->nbr,
                        g->nbr_w, nn, g->edata);
-    hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 4)), dim3(256), 0, st, g->blk_a, g->blk_b, g->cstart,
+    hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 256 / kAsmLanes)), dim3(256), 0, st, g->blk_a, g->blk_b, g->cstart,
                        g->codes, g->nb, g->vdata, g->vnn, g->vnn_wn, g->node_of_vertex, g->nbr, g->edata, F, g->H,
                        g->rhs, g->Hd);
     hipLaunchKernelGGL(k_nr_invdiag, dim3(grid_of(nn)), dim3(256), 0, st, g->Hd, nn, g->Minv);
