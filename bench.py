@@ -114,6 +114,26 @@ def bench_nonrigid(dm, nframes, with_cpu, node_radius=0.0133, dgw=0.025):
                last_stats=st)
     log(f"non-rigid: {res['fps']:.1f} frames/s ({res['ms_per_frame']:.2f} ms), nodes {g.nodeNum}, "
         f"surface err {res['surface_err_mm']:.4f} mm")
+    # several deforming objects tracked at once: one graph each (same mesh and
+    # graph rule), object k on frame (f + 10 k) of the sequence, all of a
+    # round's solves enqueued together (dm_nonrigid_solve_many)
+    K = 4
+    gs = [dm.DefGraph(V, Nn, node_radius, dgw, max_nodes=8192) for _ in range(K)]
+    pick = lambda f: [(f + 10 * k) % nframes for k in range(K)]
+    solver.solve_many(gs, [frames[i] for i in pick(0)], sens, [poses[i].reshape(4, 4) for i in pick(0)])  # capture
+    for gk in gs:
+        gk.set_transforms(np.tile(np.eye(4), (gk.nodeNum, 1, 1)))
+    t0 = time.time()
+    for f in range(nframes):
+        solver.solve_many(gs, [frames[i] for i in pick(f)], sens, [poses[i].reshape(4, 4) for i in pick(f)])
+    elk = time.time() - t0
+    for gk in gs:
+        gk.close()
+    res["concurrent"] = dict(graphs=K, rounds=nframes, fps_aggregate=K * nframes / elk,
+                             ms_per_round=1000 * elk / nframes,
+                             note="one solve per graph per round, enqueued on the graphs' own streams together")
+    log(f"non-rigid, {K} graphs at once: {K * nframes / elk:.1f} frames/s aggregate "
+        f"({1000 * elk / nframes:.2f} ms per round of {K})")
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O

@@ -955,6 +955,7 @@ struct dm_graph {
   size_t pixels_cap = 0;
   float *mesh_v = nullptr, *mesh_n = nullptr;  // mesh target (nv x 3 each), allocated on first use
   NrFrame *d_frame = nullptr, *h_frame = nullptr;  // device record and its pinned host copy
+  double *h_stats = nullptr;  // pinned: the last solve's 3 energies and 2 PCG flags (as doubles' storage)
   // the captured solve and what it was captured for
   hipGraphExec_t exec = nullptr;
   int key_gn = -1, key_pcg = -1, key_rows = -1, key_cols = -1, key_mode = -1;
@@ -972,6 +973,8 @@ struct dm_graph {
     exec = nullptr;
     if (h_frame) (void)hipHostFree(h_frame);
     h_frame = nullptr;
+    if (h_stats) (void)hipHostFree(h_stats);
+    h_stats = nullptr;
     for (void *p : allocs) (void)hipFree(p);
     allocs.clear();
     if (stream) (void)hipStreamDestroy(stream);
@@ -1245,8 +1248,10 @@ static int nr_reserve(dm_graph *g, size_t P, int pcg_its) {
 }
 
 // One solve call: mode 0 = depth target (src may gate it), 1 = mesh target.
-static int nr_solve(dm_graph *g, int mode, const float *depth, const float *src, const float *mv, const float *mn,
-                    const dm_sensor *sensor, const double pose[16], const dm_nonrigid_props *pr, double stats[4]) {
+// Enqueue one solve on the graph's stream (uploads, the captured graph, the
+// statistics' copy into pinned memory); nr_finish waits for it.
+static int nr_launch(dm_graph *g, int mode, const float *depth, const float *src, const float *mv, const float *mn,
+                     const dm_sensor *sensor, const double pose[16], const dm_nonrigid_props *pr) {
   if (!g || !pose || !pr || (mode == 0 && (!sensor || sensor->rows == 0 || sensor->cols == 0)) ||
       pr->gn_iterations < 0 || pr->pcg_iterations < 0 || pr->pcg_iterations > 1000) {
     set_error("dm_nonrigid_solve: invalid argument");
@@ -1293,16 +1298,29 @@ static int nr_solve(dm_graph *g, int mode, const float *depth, const float *src,
     g->key_rows = F.rows; g->key_cols = F.cols; g->key_mode = mode;
   }
   DM_HIP(hipGraphLaunch(g->exec, st));
-  const PcgState S = g->pcg;
-  double hs[3] = {0, 0, 0};
-  int flags[2] = {0, 0};
+  if (!g->h_stats) DM_HIP(hipHostMalloc((void **)&g->h_stats, 8 * sizeof(double), hipHostMallocDefault));
+  for (int k = 0; k < 8; ++k) g->h_stats[k] = 0.0;
   if (pr->gn_iterations > 0) {
-    DM_HIP(hipMemcpyAsync(hs, g->stats, 24, hipMemcpyDeviceToHost, st));
-    DM_HIP(hipMemcpyAsync(flags, S.flag, 8, hipMemcpyDeviceToHost, st));
+    DM_HIP(hipMemcpyAsync(g->h_stats, g->stats, 24, hipMemcpyDeviceToHost, st));
+    DM_HIP(hipMemcpyAsync(g->h_stats + 3, g->pcg.flag, 8, hipMemcpyDeviceToHost, st));
   }
-  DM_HIP(hipStreamSynchronize(st));
-  if (stats) { stats[0] = hs[0]; stats[1] = hs[1]; stats[2] = hs[2]; stats[3] = flags[1]; }
   return DM_OK;
+}
+
+static int nr_finish(dm_graph *g, double stats[4]) {
+  DM_HIP(hipStreamSynchronize(g->stream));
+  if (stats) {
+    int flags[2];
+    memcpy(flags, g->h_stats + 3, sizeof flags);
+    stats[0] = g->h_stats[0]; stats[1] = g->h_stats[1]; stats[2] = g->h_stats[2]; stats[3] = flags[1];
+  }
+  return DM_OK;
+}
+
+static int nr_solve(dm_graph *g, int mode, const float *depth, const float *src, const float *mv, const float *mn,
+                    const dm_sensor *sensor, const double pose[16], const dm_nonrigid_props *pr, double stats[4]) {
+  DM_TRY(nr_launch(g, mode, depth, src, mv, mn, sensor, pose, pr));
+  return nr_finish(g, stats);
 }
 
 extern "C" {
@@ -1414,6 +1432,26 @@ int dm_nonrigid_solve(dm_graph *g, const float *depth, const dm_sensor *sensor, 
                       const dm_nonrigid_props *pr, double stats[4]) {
   if (!depth) { set_error("dm_nonrigid_solve: invalid argument"); return DM_E_INVALID; }
   return nr_solve(g, 0, depth, nullptr, nullptr, nullptr, sensor, pose, pr, stats);
+}
+
+int dm_nonrigid_solve_many(dm_graph *const *graphs, int32_t n, const float *const *depths, const dm_sensor *sensor,
+                           const double *poses, const dm_nonrigid_props *pr, double *stats) {
+  if (!graphs || n < 1 || !depths || !poses) { set_error("dm_nonrigid_solve_many: invalid argument"); return DM_E_INVALID; }
+  for (int i = 0; i < n; ++i) {
+    if (!graphs[i] || !depths[i]) { set_error("dm_nonrigid_solve_many: null graph or frame"); return DM_E_INVALID; }
+    for (int j = 0; j < i; ++j)
+      if (graphs[j] == graphs[i]) { set_error("dm_nonrigid_solve_many: a graph appears twice"); return DM_E_INVALID; }
+  }
+  // every solve enqueued on its graph's stream before any is waited for, so
+  // that the solves' short kernels overlap
+  int rc = DM_OK, launched = 0;
+  for (; launched < n && rc == DM_OK; ++launched)
+    rc = nr_launch(graphs[launched], 0, depths[launched], nullptr, nullptr, nullptr, sensor, poses + 16 * launched, pr);
+  for (int i = 0; i < launched; ++i) {
+    const int r = nr_finish(graphs[i], stats ? stats + 4 * i : nullptr);
+    if (rc == DM_OK) rc = r;
+  }
+  return rc;
 }
 
 int dm_nonrigid_solve_pair(dm_graph *g, const float *target_depth, const float *source_depth,

@@ -394,6 +394,23 @@ class NonRigidICP:
                                                      N.ptr(pose), C.byref(pr), N.ptr(st)))
         return self._stats(st)
 
+    def solve_many(self, graphs, depths, sensor, poses, **override):
+        """One solve per graph (distinct graphs), each against its own depth
+        frame and pose, enqueued together so that their kernels overlap;
+        returns the stats of each."""
+        pr = self._props(override)
+        n = len(graphs)
+        if n != len(depths) or n != len(poses):
+            raise ValueError("one depth frame and one pose per graph")
+        ds = [_f32(d) for d in depths]
+        dptr = (C.c_void_p * n)(*[d.ctypes.data for d in ds])
+        gptr = (C.c_void_p * n)(*[g._h.value for g in graphs])
+        P = np.ascontiguousarray(np.stack([np.asarray(p, np.float64).reshape(16) for p in poses]))
+        st = np.zeros((n, 4), np.float64)
+        N.check(graphs[0]._lib.dm_nonrigid_solve_many(gptr, n, dptr, C.byref(_sensor(sensor)), N.ptr(P),
+                                                     C.byref(pr), N.ptr(st)))
+        return [self._stats(s) for s in st]
+
     def solve_mesh(self, graph, target_vertices, target_normals, pose, **override):
         """nonRigidICP::solve(warpField, blender, targetMesh, pose)
         (nonRigidICP.cu:706-852): target vertex v corresponds to canonical

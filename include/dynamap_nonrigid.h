@@ -88,6 +88,16 @@ DM_API int dm_blend(dm_graph *g, float *out_vertices, float *out_normals);
  * iterations of the last step. */
 DM_API int dm_nonrigid_solve(dm_graph *g, const float *depth, const dm_sensor *sensor, const double pose[16],
                              const dm_nonrigid_props *props, double stats[4]);
+/* dm_nonrigid_solve for n distinct graphs at once (no reference
+ * counterpart; for callers that track several deforming objects): graph i
+ * solves against depths[i] seen from poses[16 i .. 16 i + 15], all with the
+ * same sensor and props.  Every solve is enqueued on its graph's own stream
+ * before any is waited for, so that the solves' short kernels overlap; each
+ * result equals that of dm_nonrigid_solve on the same graph and frame.
+ * stats (may be NULL): 4 per graph, as dm_nonrigid_solve. */
+DM_API int dm_nonrigid_solve_many(dm_graph *const *graphs, int32_t n, const float *const *depths,
+                                  const dm_sensor *sensor, const double *poses, const dm_nonrigid_props *props,
+                                  double *stats);
 /* nonRigidICP::init(warpField, Prop, problemSize) (nonRigidICP.cu:18-40):
  * sizes the solver's frame buffers for `observations` pixels (the
  * reference's problemSize, rows * cols) and pcg_iterations, so that later

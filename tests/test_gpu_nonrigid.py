@@ -272,3 +272,30 @@ def test_4096_node_graph_matches_fp64_oracle():
     assert abs(sg["residuals"] - so[0]) <= 0.01 * so[0]
     assert dn.mean() < 1e-4 and np.percentile(dn, 95) < 3e-4
     assert eg.mean() < 3e-4
+
+
+def test_solve_many_equals_one_at_a_time():
+    """dm_nonrigid_solve_many (the graphs' solves enqueued together on their
+    own streams) against the same solves one after another: node transforms
+    and stats bit-equal, over two frames."""
+    V, Nn = cylinder(200, 100)
+    frames, poses = [], []
+    for f in range(6):
+        P = cyl_pose(f)
+        frames.append(cyl_depth(f, P))
+        poses.append(P)
+    solver = dm.NonRigidICP()
+    seq = [dm.DefGraph(V, Nn, 0.025, 0.05) for _ in range(3)]
+    many = [dm.DefGraph(V, Nn, 0.025, 0.05) for _ in range(3)]
+    st_seq, st_many = [], []
+    for step in range(2):
+        idx = [k + 3 * step for k in range(3)]  # graph k sees frame k (+3 on the second step)
+        st_seq.append([solver.solve(seq[k], frames[i], SENS, poses[i]) for k, i in enumerate(idx)])
+        st_many.append(solver.solve_many(many, [frames[i] for i in idx], SENS, [poses[i] for i in idx]))
+    for k in range(3):
+        assert np.array_equal(seq[k].transforms(), many[k].transforms()), f"graph {k}"
+    assert st_seq == st_many
+    with pytest.raises(RuntimeError, match="twice"):
+        solver.solve_many([many[0], many[0]], frames[:2], SENS, poses[:2])
+    for g in seq + many:
+        g.close()
