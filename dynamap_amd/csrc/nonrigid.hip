@@ -699,22 +699,6 @@ struct PcgState {
   int nwg_s;                  // workgroups of the SpMV (kSpmvNodes nodes each)
 };
 
-// fixed-order sum of the workgroup partials, computed by every workgroup:
-// lane l of the first wavefront adds partials l, l + 64, ...; the xor tree
-// of wave_sum adds the lanes, and lane 0's total is the result
-__device__ double sum_partials(const double *part, int n, double *lds1) {
-  if (threadIdx.x < 64) {
-    double s = 0.0;
-    for (int k = threadIdx.x; k < n; k += 64) s += part[k];
-    s = wave_sum(s);
-    if (threadIdx.x == 0) lds1[0] = s;
-  }
-  __syncthreads();
-  const double s = lds1[0];
-  __syncthreads();
-  return s;
-}
-
 __device__ void wg_partial(double x, double *lds, double *part) {
   const double w = wave_sum(x);
   __syncthreads();  // lds may still be read by an earlier sum of this workgroup
@@ -815,8 +799,10 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const do
                                                            int it, const NrFrame *Fp) {
   if (S.flag[0]) return;
   __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1], l2[2];
-  // gamma (wavefront 0) and delta (wavefront 1) summed side by side, each in
-  // sum_partials' fixed order
+  // gamma (wavefront 0) and delta (wavefront 1) from the workgroup partials,
+  // side by side, each in a fixed order computed identically by every
+  // workgroup: lane l adds partials l, l + 64, ..., then the xor tree of
+  // wave_sum adds the lanes
   if (threadIdx.x < 128) {
     const int lane = threadIdx.x & 63;
     const bool second = threadIdx.x >= 64;
