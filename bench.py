@@ -127,13 +127,29 @@ def bench_nonrigid(dm, nframes, with_cpu, node_radius=0.0133, dgw=0.025):
     for f in range(nframes):
         solver.solve_many(gs, [frames[i] for i in pick(f)], sens, [poses[i].reshape(4, 4) for i in pick(f)])
     elk = time.time() - t0
-    for gk in gs:
-        gk.close()
     res["concurrent"] = dict(graphs=K, rounds=nframes, fps_aggregate=K * nframes / elk,
                              ms_per_round=1000 * elk / nframes,
                              note="one solve per graph per round, enqueued on the graphs' own streams together")
     log(f"non-rigid, {K} graphs at once: {K * nframes / elk:.1f} frames/s aggregate "
         f"({1000 * elk / nframes:.2f} ms per round of {K})")
+    # the same rounds through a batch: every kernel launched once for all K
+    # graphs (graph = blockIdx.y), one captured HIP graph per round
+    batch = solver.batch(gs)
+    batch.solve([frames[i] for i in pick(0)], sens, [poses[i].reshape(4, 4) for i in pick(0)])  # capture
+    for gk in gs:
+        gk.set_transforms(np.tile(np.eye(4), (gk.nodeNum, 1, 1)))
+    t0 = time.time()
+    for f in range(nframes):
+        batch.solve([frames[i] for i in pick(f)], sens, [poses[i].reshape(4, 4) for i in pick(f)])
+    elb = time.time() - t0
+    batch.close()
+    for gk in gs:
+        gk.close()
+    res["batched"] = dict(graphs=K, rounds=nframes, fps_aggregate=K * nframes / elb,
+                          ms_per_round=1000 * elb / nframes,
+                          note="one solve per graph per round, each kernel launched once for all graphs")
+    log(f"non-rigid, {K} graphs batched: {K * nframes / elb:.1f} frames/s aggregate "
+        f"({1000 * elb / nframes:.2f} ms per round of {K})")
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O

@@ -11,8 +11,8 @@ kernels of libdynamap_hip.so through its C ABI.
 """
 from . import _native
 from .mapping import (TsdfVolume, Session, Server, Mesh, MeshExtractor, mc_tri_table, DefGraph, DqBlender,
-                      NonRigidICP, bilateral_filter, depth_pyramid)
+                      NonRigidICP, NonRigidBatch, bilateral_filter, depth_pyramid)
 from . import configs
 
 __all__ = ["TsdfVolume", "Session", "Server", "Mesh", "MeshExtractor", "mc_tri_table", "DefGraph", "DqBlender",
-           "NonRigidICP", "bilateral_filter", "depth_pyramid", "configs", "_native"]
+           "NonRigidICP", "NonRigidBatch", "bilateral_filter", "depth_pyramid", "configs", "_native"]

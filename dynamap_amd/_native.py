@@ -93,6 +93,9 @@ PROTOTYPES = {
     "dm_nonrigid_reserve": (C.c_int, [_vp, C.c_int64, C.c_int32]),
     "dm_nonrigid_solve_pair": (C.c_int, [_vp, _vp, _vp, _P(dm_sensor), _vp, _P(dm_nonrigid_props), _vp]),
     "dm_nonrigid_solve_many": (C.c_int, [_vp, C.c_int32, _vp, _P(dm_sensor), _vp, _P(dm_nonrigid_props), _vp]),
+    "dm_nonrigid_batch_create": (C.c_int, [_vp, C.c_int32, _P(_vp)]),
+    "dm_nonrigid_batch_solve": (C.c_int, [_vp, _vp, _P(dm_sensor), _vp, _P(dm_nonrigid_props), _vp]),
+    "dm_nonrigid_batch_destroy": (C.c_int, [_vp]),
     "dm_nonrigid_solve_mesh": (C.c_int, [_vp, _vp, _vp, _vp, _P(dm_nonrigid_props), _vp]),
     "dm_graph_create_nodes": (C.c_int, [_vp, _vp, C.c_int64, _vp, C.c_int32, C.c_float, _P(_vp)]),
     "dm_graph_update_nodes": (C.c_int, [_vp]),

@@ -328,7 +328,7 @@ __device__ __forceinline__ void q_rotate(float w, float x, float y, float z, con
 
 // dqBlender::blendVertex (dqBlender.cu:162-186) with Kavan's sign rule and
 // one normalisation; a node vertex takes its node's dq (K)
-__global__ __launch_bounds__(256) void k_nr_blend(const float *__restrict__ verts, const float *__restrict__ normals,
+__device__ __forceinline__ void d_nr_blend(const float *__restrict__ verts, const float *__restrict__ normals,
                                                  int64_t nv, const int *__restrict__ node_of_vertex,
                                                  const int *__restrict__ vnn, const float *__restrict__ vnn_w,
                                                  const dq8 *__restrict__ dq, float *__restrict__ out_v,
@@ -371,6 +371,29 @@ __global__ __launch_bounds__(256) void k_nr_blend(const float *__restrict__ vert
     out_n[3 * v] = rn[0]; out_n[3 * v + 1] = rn[1]; out_n[3 * v + 2] = rn[2];
   }
 }
+__global__ __launch_bounds__(256) void k_nr_blend(const float *__restrict__ verts, const float *__restrict__ normals,
+                                                 int64_t nv, const int *__restrict__ node_of_vertex,
+                                                 const int *__restrict__ vnn, const float *__restrict__ vnn_w,
+                                                 const dq8 *__restrict__ dq, float *__restrict__ out_v,
+                                                 float *__restrict__ out_n) { d_nr_blend(verts, normals, nv, node_of_vertex, vnn, vnn_w, dq, out_v, out_n); }
+struct b_nr_blend {
+  const float * verts;
+  const float * normals;
+  int64_t nv;
+  const int * node_of_vertex;
+  const int * vnn;
+  const float * vnn_w;
+  const dq8 * dq;
+  float * out_v;
+  float * out_n;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_nr_blend for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(256) void kb_nr_blend(const b_nr_blend *__restrict__ t) {
+  const b_nr_blend &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_nr_blend(a.verts, a.normals, a.nv, a.node_of_vertex, a.vnn, a.vnn_w, a.dq, a.out_v, a.out_n);
+}
 
 // ---------------------------------------------------------------- Gauss-Newton
 struct NrFrame {
@@ -382,7 +405,7 @@ struct NrFrame {
 };
 
 // target vertex (camera) per pixel, components rounded to float (w = 1 valid)
-__global__ __launch_bounds__(256) void k_nr_target_vert(const float *__restrict__ depth, const NrFrame *Fp,
+__device__ __forceinline__ void d_nr_target_vert(const float *__restrict__ depth, const NrFrame *Fp,
                                                        float4 *__restrict__ tv) {
   const NrFrame &F = *Fp;
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -393,10 +416,24 @@ __global__ __launch_bounds__(256) void k_nr_target_vert(const float *__restrict_
   const double dd = (double)d;
   tv[i] = make_float4((float)((((double)uu - F.cx) * dd) / F.fx), (float)((((double)vv - F.cy) * dd) / F.fy), d, 1.f);
 }
+__global__ __launch_bounds__(256) void k_nr_target_vert(const float *__restrict__ depth, const NrFrame *Fp,
+                                                       float4 *__restrict__ tv) { d_nr_target_vert(depth, Fp, tv); }
+struct b_nr_target_vert {
+  const float * depth;
+  const NrFrame *Fp;
+  float4 * tv;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_nr_target_vert for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(256) void kb_nr_target_vert(const b_nr_target_vert *__restrict__ t) {
+  const b_nr_target_vert &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_nr_target_vert(a.depth, a.Fp, a.tv);
+}
 
 // target normal (camera): cross(V[i+2]-V[i-2], V[i+W]-V[i-W]) in FP64 with
 // out-of-range neighbours replaced by the pixel itself; w = 1 valid
-__global__ __launch_bounds__(256) void k_nr_target_norm(const float4 *__restrict__ tv, const NrFrame *Fp,
+__device__ __forceinline__ void d_nr_target_norm(const float4 *__restrict__ tv, const NrFrame *Fp,
                                                        float4 *__restrict__ tn) {
   const NrFrame &F = *Fp;
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -418,6 +455,20 @@ __global__ __launch_bounds__(256) void k_nr_target_norm(const float4 *__restrict
   const double nn = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
   if (!(nn > 0)) { tn[i] = make_float4(0.f, 0.f, 0.f, 0.f); return; }
   tn[i] = make_float4((float)(c0 / nn), (float)(c1 / nn), (float)(c2 / nn), 1.f);
+}
+__global__ __launch_bounds__(256) void k_nr_target_norm(const float4 *__restrict__ tv, const NrFrame *Fp,
+                                                       float4 *__restrict__ tn) { d_nr_target_norm(tv, Fp, tn); }
+struct b_nr_target_norm {
+  const float4 * tv;
+  const NrFrame *Fp;
+  float4 * tn;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_nr_target_norm for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(256) void kb_nr_target_norm(const b_nr_target_norm *__restrict__ t) {
+  const b_nr_target_norm &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_nr_target_norm(a.tv, a.Fp, a.tn);
 }
 
 // per-vertex data term: g = [p' x n, n] (world), residual r = n . (p' - q);
@@ -441,7 +492,7 @@ __device__ __forceinline__ void nr_data_record(const double p[3], const double n
 
 // source depth (sdepth) gates a pixel when F.use_src: the reference's range
 // test on the source image (nonRigidICP.cu:291-292)
-__global__ __launch_bounds__(256) void k_nr_data(const float *__restrict__ pv, const float *__restrict__ pn,
+__device__ __forceinline__ void d_nr_data(const float *__restrict__ pv, const float *__restrict__ pn,
                                                 int64_t nv, const float4 *__restrict__ tv,
                                                 const float4 *__restrict__ tn, const float *__restrict__ depth,
                                                 const float *__restrict__ sdepth, const NrFrame *Fp,
@@ -480,11 +531,34 @@ __global__ __launch_bounds__(256) void k_nr_data(const float *__restrict__ pv, c
   if (fabs(ncam[0] * nt[0] + ncam[1] * nt[1] + ncam[2] * nt[2]) < F.normal_cos) return;
   nr_data_record(p, nt, q, pose, o);
 }
+__global__ __launch_bounds__(256) void k_nr_data(const float *__restrict__ pv, const float *__restrict__ pn,
+                                                int64_t nv, const float4 *__restrict__ tv,
+                                                const float4 *__restrict__ tn, const float *__restrict__ depth,
+                                                const float *__restrict__ sdepth, const NrFrame *Fp,
+                                                float *__restrict__ out) { d_nr_data(pv, pn, nv, tv, tn, depth, sdepth, Fp, out); }
+struct b_nr_data {
+  const float * pv;
+  const float * pn;
+  int64_t nv;
+  const float4 * tv;
+  const float4 * tn;
+  const float * depth;
+  const float * sdepth;
+  const NrFrame *Fp;
+  float * out;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_nr_data for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(256) void kb_nr_data(const b_nr_data *__restrict__ t) {
+  const b_nr_data &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_nr_data(a.pv, a.pn, a.nv, a.tv, a.tn, a.depth, a.sdepth, a.Fp, a.out);
+}
 
 // mesh target (nonRigidICP.cu:354-376): target vertex v and its normal, in
 // camera coordinates, correspond to canonical vertex v; same gates and
 // record as the depth target
-__global__ __launch_bounds__(256) void k_nr_data_mesh(const float *__restrict__ pv, const float *__restrict__ pn,
+__device__ __forceinline__ void d_nr_data_mesh(const float *__restrict__ pv, const float *__restrict__ pn,
                                                      int64_t nv, const float *__restrict__ mv,
                                                      const float *__restrict__ mn, const NrFrame *Fp,
                                                      float *__restrict__ out) {
@@ -510,10 +584,30 @@ __global__ __launch_bounds__(256) void k_nr_data_mesh(const float *__restrict__ 
   if (fabs(ncam[0] * nt[0] + ncam[1] * nt[1] + ncam[2] * nt[2]) < F.normal_cos) return;
   nr_data_record(p, nt, q, pose, o);
 }
+__global__ __launch_bounds__(256) void k_nr_data_mesh(const float *__restrict__ pv, const float *__restrict__ pn,
+                                                     int64_t nv, const float *__restrict__ mv,
+                                                     const float *__restrict__ mn, const NrFrame *Fp,
+                                                     float *__restrict__ out) { d_nr_data_mesh(pv, pn, nv, mv, mn, Fp, out); }
+struct b_nr_data_mesh {
+  const float * pv;
+  const float * pn;
+  int64_t nv;
+  const float * mv;
+  const float * mn;
+  const NrFrame *Fp;
+  float * out;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_nr_data_mesh for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(256) void kb_nr_data_mesh(const b_nr_data_mesh *__restrict__ t) {
+  const b_nr_data_mesh &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_nr_data_mesh(a.pv, a.pn, a.nv, a.mv, a.mn, a.Fp, a.out);
+}
 
 // ARAP edge e = i * K + k (j = k-th neighbour of i): a = T_i g_j, b = T_j g_j,
 // w_ij = max(w_i[k], w_j[k]) (nonRigidICP.cu:494-515); out[8 e] = a, b, w
-__global__ __launch_bounds__(256) void k_nr_arap(const double *__restrict__ T, const float *__restrict__ verts,
+__device__ __forceinline__ void d_nr_arap(const double *__restrict__ T, const float *__restrict__ verts,
                                                 const int *__restrict__ node_vid, const int *__restrict__ nbr,
                                                 const float *__restrict__ nbr_w, int nn, double *__restrict__ out) {
   const int e = blockIdx.x * 256 + threadIdx.x;
@@ -533,6 +627,25 @@ __global__ __launch_bounds__(256) void k_nr_arap(const double *__restrict__ T, c
   if (wj > wij) wij = wj;
   o[6] = wij;
   o[7] = 0.0;
+}
+__global__ __launch_bounds__(256) void k_nr_arap(const double *__restrict__ T, const float *__restrict__ verts,
+                                                const int *__restrict__ node_vid, const int *__restrict__ nbr,
+                                                const float *__restrict__ nbr_w, int nn, double *__restrict__ out) { d_nr_arap(T, verts, node_vid, nbr, nbr_w, nn, out); }
+struct b_nr_arap {
+  const double * T;
+  const float * verts;
+  const int * node_vid;
+  const int * nbr;
+  const float * nbr_w;
+  int nn;
+  double * out;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_nr_arap for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(256) void kb_nr_arap(const b_nr_arap *__restrict__ t) {
+  const b_nr_arap &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_nr_arap(a.T, a.verts, a.node_vid, a.nbr, a.nbr_w, a.nn, a.out);
 }
 
 // ARAP Jacobian rows for component c: Ji = w [row c of -[a]x, e_c],
@@ -564,7 +677,7 @@ __device__ __forceinline__ double wave_sum(double x) {
 // several independent gathers in flight and a 4-level tree (a wavefront per
 // block took about 153 us per assembly at 4048 nodes, these about 80 us).
 constexpr int kAsmLanes = 16;
-__global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
+__device__ __forceinline__ void d_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
                                                     const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
                                                     int nb, const float *__restrict__ vdata,
                                                     const int *__restrict__ vnn, const float *__restrict__ vnn_wn,
@@ -660,11 +773,43 @@ __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk
 #pragma unroll
   for (int q = 0; q < 36; ++q) D[q] = h[q];
 }
+__global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
+                                                    const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
+                                                    int nb, const float *__restrict__ vdata,
+                                                    const int *__restrict__ vnn, const float *__restrict__ vnn_wn,
+                                                    const int *__restrict__ node_of_vertex, const int *__restrict__ nbr,
+                                                    const double *__restrict__ edata, const NrFrame *Fp,
+                                                    float *__restrict__ H, double *__restrict__ rhs,
+                                                    double *__restrict__ Hd) { d_nr_assemble(blk_a, blk_b, cstart, codes, nb, vdata, vnn, vnn_wn, node_of_vertex, nbr, edata, Fp, H, rhs, Hd); }
+struct b_nr_assemble {
+  const int * blk_a;
+  const int * blk_b;
+  const int * cstart;
+  const uint32_t * codes;
+  int nb;
+  const float * vdata;
+  const int * vnn;
+  const float * vnn_wn;
+  const int * node_of_vertex;
+  const int * nbr;
+  const double * edata;
+  const NrFrame *Fp;
+  float * H;
+  double * rhs;
+  double * Hd;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_nr_assemble for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(256) void kb_nr_assemble(const b_nr_assemble *__restrict__ t) {
+  const b_nr_assemble &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_nr_assemble(a.blk_a, a.blk_b, a.cstart, a.codes, a.nb, a.vdata, a.vnn, a.vnn_wn, a.node_of_vertex, a.nbr, a.edata, a.Fp, a.H, a.rhs, a.Hd);
+}
 
 // Block-Jacobi preconditioner: the FP64 inverse of each diagonal block (one
 // thread per node; LDL^T once, then the 6 unit columns; a block that is not
 // positive definite gets 0).
-__global__ __launch_bounds__(256) void k_nr_invdiag(const double *__restrict__ Hd, int nn,
+__device__ __forceinline__ void d_nr_invdiag(const double *__restrict__ Hd, int nn,
                                                    double *__restrict__ Minv) {
   const int a = blockIdx.x * 256 + threadIdx.x;
   if (a >= nn) return;
@@ -678,6 +823,20 @@ __global__ __launch_bounds__(256) void k_nr_invdiag(const double *__restrict__ H
     if (ok) ldlt6_solve(L, D, e6, col);
     for (int r = 0; r < 6; ++r) M[6 * r + c] = col[r];
   }
+}
+__global__ __launch_bounds__(256) void k_nr_invdiag(const double *__restrict__ Hd, int nn,
+                                                   double *__restrict__ Minv) { d_nr_invdiag(Hd, nn, Minv); }
+struct b_nr_invdiag {
+  const double * Hd;
+  int nn;
+  double * Minv;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_nr_invdiag for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(256) void kb_nr_invdiag(const b_nr_invdiag *__restrict__ t) {
+  const b_nr_invdiag &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_nr_invdiag(a.Hd, a.nn, a.Minv);
 }
 
 // ---------------------------------------------------------------- PCG
@@ -721,7 +880,7 @@ __device__ __forceinline__ double precond_row(const double *Minv, const double *
   return s;
 }
 
-__global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgState S, const double *__restrict__ rhs,
+__device__ __forceinline__ void d_pcg_init(PcgState S, const double *__restrict__ rhs,
                                                          const double *__restrict__ Minv, int nn) {
   __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1];
   const int t = blockIdx.x * kPcgThreads + threadIdx.x;
@@ -738,13 +897,28 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgState S, const doub
   wg_partial(r * z, lds, S.part_rz);
   if (t == 0) { S.flag[0] = 0; S.flag[1] = 0; }
 }
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgState S, const double *__restrict__ rhs,
+                                                         const double *__restrict__ Minv, int nn) { d_pcg_init(S, rhs, Minv, nn); }
+struct b_pcg_init {
+  PcgState S;
+  const double * rhs;
+  const double * Minv;
+  int nn;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_pcg_init for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(kPcgThreads) void kb_pcg_init(const b_pcg_init *__restrict__ t) {
+  const b_pcg_init &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_pcg_init(a.S, a.rhs, a.Minv, a.nn);
+}
 
 // w = H u, one wavefront per node: lane l < 60 takes output row l % 6 of the
 // block entries l / 6, l / 6 + 10, ... of the node's block row, so all the
 // entries' loads are in flight together; the 10 lane sums of a row are
 // added in a fixed order through LDS.  Partials of w.u per workgroup.
 constexpr int kSpmvNodes = 16;  // wavefronts (nodes) per workgroup (fewer partials for the update to add)
-__global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_spmv(PcgState S, const float *__restrict__ H,
+__device__ __forceinline__ void d_pcg_spmv(PcgState S, const float *__restrict__ H,
                                                              const int *__restrict__ row_start,
                                                              const uint32_t *__restrict__ row_ent,
                                                              const int *__restrict__ blk_a,
@@ -789,13 +963,34 @@ __global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_spmv(PcgState S, const 
     S.part_pq[blockIdx.x] = sum;
   }
 }
+__global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_spmv(PcgState S, const float *__restrict__ H,
+                                                             const int *__restrict__ row_start,
+                                                             const uint32_t *__restrict__ row_ent,
+                                                             const int *__restrict__ blk_a,
+                                                             const int *__restrict__ blk_b, int nn) { d_pcg_spmv(S, H, row_start, row_ent, blk_a, blk_b, nn); }
+struct b_pcg_spmv {
+  PcgState S;
+  const float * H;
+  const int * row_start;
+  const uint32_t * row_ent;
+  const int * blk_a;
+  const int * blk_b;
+  int nn;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_pcg_spmv for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(64 * kSpmvNodes) void kb_pcg_spmv(const b_pcg_spmv *__restrict__ t) {
+  const b_pcg_spmv &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_pcg_spmv(a.S, a.H, a.row_start, a.row_ent, a.blk_a, a.blk_b, a.nn);
+}
 
 // Iteration k: gamma = r.u and delta = w.u from the partials (fixed order,
 // the same in every workgroup); stop when gamma <= tol^2 gamma_0; beta =
 // gamma / gamma_{k-1}, alpha = gamma / (delta - beta gamma / alpha_{k-1})
 // (k = 0: beta = 0, alpha = gamma / delta); p = u + beta p; s = w + beta s;
 // x += alpha p; r -= alpha s; u = Minv r; partial r.u.
-__global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const double *__restrict__ Minv, int nn,
+__device__ __forceinline__ void d_pcg_update(PcgState S, const double *__restrict__ Minv, int nn,
                                                            int it, const NrFrame *Fp) {
   if (S.flag[0]) return;
   __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1], l2[2];
@@ -851,9 +1046,24 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const do
   }
   wg_partial(r * z, lds, S.part_rz + (size_t)((it + 1) & 1) * S.nwg);
 }
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const double *__restrict__ Minv, int nn,
+                                                           int it, const NrFrame *Fp) { d_pcg_update(S, Minv, nn, it, Fp); }
+struct b_pcg_update {
+  PcgState S;
+  const double * Minv;
+  int nn;
+  const NrFrame *Fp;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_pcg_update for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(kPcgThreads) void kb_pcg_update(const b_pcg_update *__restrict__ t, int it) {
+  const b_pcg_update &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_pcg_update(a.S, a.Minv, a.nn, it, a.Fp);
+}
 
 // T_i <- exp(x_i) T_i, then the node's dual quaternion
-__global__ __launch_bounds__(256) void k_nr_update(const double *__restrict__ x, double *__restrict__ T, int nn,
+__device__ __forceinline__ void d_nr_update(const double *__restrict__ x, double *__restrict__ T, int nn,
                                                   dq8 *__restrict__ dq) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nn) return;
@@ -863,12 +1073,27 @@ __global__ __launch_bounds__(256) void k_nr_update(const double *__restrict__ x,
   for (int k = 0; k < 16; ++k) T[16 * (size_t)i + k] = Tn[k];
   dq[i] = dq_of_T(Tn);
 }
+__global__ __launch_bounds__(256) void k_nr_update(const double *__restrict__ x, double *__restrict__ T, int nn,
+                                                  dq8 *__restrict__ dq) { d_nr_update(x, T, nn, dq); }
+struct b_nr_update {
+  const double * x;
+  double * T;
+  int nn;
+  dq8 * dq;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_nr_update for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(256) void kb_nr_update(const b_nr_update *__restrict__ t) {
+  const b_nr_update &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_nr_update(a.x, a.T, a.nn, a.dq);
+}
 
 // energies at the last linearisation: [0] residual count, [1] data, [2] ARAP.
 // kStatsWg workgroups write fixed-order partials (grid-stride over vertices
 // and edges); k_nr_stats_sum adds them in order.
 constexpr int kStatsWg = 128;
-__global__ __launch_bounds__(256) void k_nr_stats(const float *__restrict__ vdata, int64_t nv,
+__device__ __forceinline__ void d_nr_stats(const float *__restrict__ vdata, int64_t nv,
                                                  const double *__restrict__ edata, int ne, const NrFrame *Fp,
                                                  double *__restrict__ part) {
   const double lam = Fp->lambda;
@@ -895,12 +1120,42 @@ __global__ __launch_bounds__(256) void k_nr_stats(const float *__restrict__ vdat
     part[3 * blockIdx.x + threadIdx.x] = s;
   }
 }
+__global__ __launch_bounds__(256) void k_nr_stats(const float *__restrict__ vdata, int64_t nv,
+                                                 const double *__restrict__ edata, int ne, const NrFrame *Fp,
+                                                 double *__restrict__ part) { d_nr_stats(vdata, nv, edata, ne, Fp, part); }
+struct b_nr_stats {
+  const float * vdata;
+  int64_t nv;
+  const double * edata;
+  int ne;
+  const NrFrame *Fp;
+  double * part;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_nr_stats for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(256) void kb_nr_stats(const b_nr_stats *__restrict__ t) {
+  const b_nr_stats &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_nr_stats(a.vdata, a.nv, a.edata, a.ne, a.Fp, a.part);
+}
 
-__global__ void k_nr_stats_sum(const double *__restrict__ part, double *__restrict__ out) {
+__device__ __forceinline__ void d_nr_stats_sum(const double *__restrict__ part, double *__restrict__ out) {
   if (threadIdx.x >= 3) return;
   double s = 0.0;
   for (int b = 0; b < kStatsWg; ++b) s += part[3 * b + threadIdx.x];
   out[threadIdx.x] = s;
+}
+__global__ void k_nr_stats_sum(const double *__restrict__ part, double *__restrict__ out) { d_nr_stats_sum(part, out); }
+struct b_nr_stats_sum {
+  const double * part;
+  double * out;
+  unsigned grid;  // workgroups of this graph's launch
+};
+// k_nr_stats_sum for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ void kb_nr_stats_sum(const b_nr_stats_sum *__restrict__ t) {
+  const b_nr_stats_sum &a = t[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  d_nr_stats_sum(a.part, a.out);
 }
 
 __global__ void k_set_identity(double *T, int nn) {
@@ -945,6 +1200,7 @@ struct dm_graph {
   // the captured solve and what it was captured for
   hipGraphExec_t exec = nullptr;
   int key_gn = -1, key_pcg = -1, key_rows = -1, key_cols = -1, key_mode = -1;
+  int buf_gen = 0;  // counts frame-buffer reallocations (batches capture again when it moves)
 
   template <class T> int dalloc(T **p, size_t n) {
     void *q = nullptr;
@@ -1223,14 +1479,48 @@ static int nr_reserve(dm_graph *g, size_t P, int pcg_its) {
     DM_TRY(g->dalloc(&g->tn, P));
     g->pixels_cap = P;
     g->key_gn = -1;  // buffers moved: capture again
+    ++g->buf_gen;
   }
   if (pcg_its + 1 > g->pcg_cap) {
     DM_TRY(g->dalloc(&g->pcg.rz, pcg_its + 1));
     DM_TRY(g->dalloc(&g->pcg.al, pcg_its + 1));
     g->pcg_cap = pcg_its + 1;
     g->key_gn = -1;
+    ++g->buf_gen;
   }
   return DM_OK;
+}
+
+// The solve's frame record (pinned host copy; the caller uploads it).
+static void fill_frame(dm_graph *g, int mode, const dm_sensor *sensor, const double pose[16],
+                       const dm_nonrigid_props *pr, bool use_src) {
+  NrFrame &F = *g->h_frame;
+  F = NrFrame{};
+  if (mode == 0) {
+    F.rows = (int)sensor->rows; F.cols = (int)sensor->cols;
+    F.fx = sensor->fx; F.fy = sensor->fy; F.cx = sensor->cx; F.cy = sensor->cy;
+  }
+  for (int k = 0; k < 16; ++k) F.pose[k] = pose[k];
+  F.min_depth = pr->min_depth; F.max_depth = pr->max_depth; F.dist_max = pr->dist_max;
+  F.normal_cos = pr->normal_cos; F.lambda = pr->lambda_reg; F.damping = pr->damping; F.pcg_tol = pr->pcg_tol;
+  F.use_src = use_src;
+}
+
+// Copy a solve's statistics (3 energies, 2 PCG flags) into the graph's pinned
+// buffer on stream st; read_stats takes them once st is done.
+static int stats_copy(dm_graph *g, const dm_nonrigid_props *pr, hipStream_t st) {
+  if (!g->h_stats) DM_HIP(hipHostMalloc((void **)&g->h_stats, 8 * sizeof(double), hipHostMallocDefault));
+  for (int k = 0; k < 8; ++k) g->h_stats[k] = 0.0;
+  if (pr->gn_iterations > 0) {
+    DM_HIP(hipMemcpyAsync(g->h_stats, g->stats, 24, hipMemcpyDeviceToHost, st));
+    DM_HIP(hipMemcpyAsync(g->h_stats + 3, g->pcg.flag, 8, hipMemcpyDeviceToHost, st));
+  }
+  return DM_OK;
+}
+static void read_stats(const dm_graph *g, double stats[4]) {
+  int flags[2];
+  memcpy(flags, g->h_stats + 3, sizeof flags);
+  stats[0] = g->h_stats[0]; stats[1] = g->h_stats[1]; stats[2] = g->h_stats[2]; stats[3] = flags[1];
 }
 
 // One solve call: mode 0 = depth target (src may gate it), 1 = mesh target.
@@ -1251,16 +1541,8 @@ static int nr_launch(dm_graph *g, int mode, const float *depth, const float *src
     DM_TRY(g->dalloc(&g->mesh_n, (size_t)g->nv * 3));
   }
   DM_HIP(hipStreamSynchronize(st));  // the pinned frame record may still be read by the last call
-  NrFrame &F = *g->h_frame;
-  F = NrFrame{};
-  if (mode == 0) {
-    F.rows = (int)sensor->rows; F.cols = (int)sensor->cols;
-    F.fx = sensor->fx; F.fy = sensor->fy; F.cx = sensor->cx; F.cy = sensor->cy;
-  }
-  for (int k = 0; k < 16; ++k) F.pose[k] = pose[k];
-  F.min_depth = pr->min_depth; F.max_depth = pr->max_depth; F.dist_max = pr->dist_max;
-  F.normal_cos = pr->normal_cos; F.lambda = pr->lambda_reg; F.damping = pr->damping; F.pcg_tol = pr->pcg_tol;
-  F.use_src = src != nullptr;
+  fill_frame(g, mode, sensor, pose, pr, src != nullptr);
+  const NrFrame &F = *g->h_frame;
   DM_HIP(hipMemcpyAsync(g->d_frame, g->h_frame, sizeof(NrFrame), hipMemcpyHostToDevice, st));
   if (mode == 0) {
     DM_HIP(hipMemcpyAsync(g->depth, depth, P * 4, hipMemcpyHostToDevice, st));
@@ -1284,22 +1566,12 @@ static int nr_launch(dm_graph *g, int mode, const float *depth, const float *src
     g->key_rows = F.rows; g->key_cols = F.cols; g->key_mode = mode;
   }
   DM_HIP(hipGraphLaunch(g->exec, st));
-  if (!g->h_stats) DM_HIP(hipHostMalloc((void **)&g->h_stats, 8 * sizeof(double), hipHostMallocDefault));
-  for (int k = 0; k < 8; ++k) g->h_stats[k] = 0.0;
-  if (pr->gn_iterations > 0) {
-    DM_HIP(hipMemcpyAsync(g->h_stats, g->stats, 24, hipMemcpyDeviceToHost, st));
-    DM_HIP(hipMemcpyAsync(g->h_stats + 3, g->pcg.flag, 8, hipMemcpyDeviceToHost, st));
-  }
-  return DM_OK;
+  return stats_copy(g, pr, st);
 }
 
 static int nr_finish(dm_graph *g, double stats[4]) {
   DM_HIP(hipStreamSynchronize(g->stream));
-  if (stats) {
-    int flags[2];
-    memcpy(flags, g->h_stats + 3, sizeof flags);
-    stats[0] = g->h_stats[0]; stats[1] = g->h_stats[1]; stats[2] = g->h_stats[2]; stats[3] = flags[1];
-  }
+  if (stats) read_stats(g, stats);
   return DM_OK;
 }
 
@@ -1307,6 +1579,135 @@ static int nr_solve(dm_graph *g, int mode, const float *depth, const float *src,
                     const dm_sensor *sensor, const double pose[16], const dm_nonrigid_props *pr, double stats[4]) {
   DM_TRY(nr_launch(g, mode, depth, src, mv, mn, sensor, pose, pr));
   return nr_finish(g, stats);
+}
+
+// ---------------------------------------------------------------- batches
+// Several graphs' solves in shared launches: every kernel of the solve runs
+// once per step for all graphs (graph = blockIdx.y), reading that graph's
+// arguments from a table (kb_* kernels; the bodies are the single-graph
+// kernels').  The tables and one captured graph of the whole solve belong
+// to the batch; the frame buffers and statistics stay the graphs' own.
+struct dm_nonrigid_batch {
+  std::vector<dm_graph *> gs;
+  std::vector<int> gens;          // the graphs' buf_gen at capture
+  hipStream_t stream = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int key_gn = -1, key_pcg = -1, key_rows = -1, key_cols = -1;
+  char *tables = nullptr;         // device: every kernel's argument records
+  size_t table_cap = 0;
+  ~dm_nonrigid_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (tables) (void)hipFree(tables);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// Host images of the argument tables of one batch, at 256 B aligned offsets.
+struct BatchTables {
+  std::vector<char> host;
+  template <class T> size_t add(const std::vector<T> &v) {
+    const size_t off = (host.size() + 255) / 256 * 256;
+    host.resize(off + v.size() * sizeof(T));
+    memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+};
+template <class T> static unsigned max_grid(const std::vector<T> &v) {
+  unsigned m = 0;
+  for (const T &e : v) m = std::max(m, e.grid);
+  return m;
+}
+
+// Capture one solve of every graph of the batch on its stream (depth
+// target, no source gate; the frames are rows x cols).
+static int batch_capture(dm_nonrigid_batch *b, int rows, int cols, int gn, int pcg_its) {
+  const int K = (int)b->gs.size();
+  const size_t P = (size_t)rows * cols;
+  std::vector<b_nr_target_vert> tvt(K);
+  std::vector<b_nr_target_norm> tnm(K);
+  std::vector<b_nr_blend> bl(K);
+  std::vector<b_nr_data> da(K);
+  std::vector<b_nr_arap> ar(K);
+  std::vector<b_nr_assemble> as(K);
+  std::vector<b_nr_invdiag> inv(K);
+  std::vector<b_nr_stats> sta(K);
+  std::vector<b_nr_stats_sum> ssum(K);
+  std::vector<b_pcg_init> pin(K);
+  std::vector<b_pcg_spmv> spm(K);
+  std::vector<b_pcg_update> upd(K);
+  std::vector<b_nr_update> nup(K);
+  for (int i = 0; i < K; ++i) {
+    dm_graph *g = b->gs[i];
+    const int nn = g->nn, ne = nn * kNK;
+    const PcgState S = g->pcg;
+    const NrFrame *F = g->d_frame;
+    tvt[i] = {g->depth, F, g->tv, grid_of(P)};
+    tnm[i] = {g->tv, F, g->tn, grid_of(P)};
+    bl[i] = {g->verts, g->normals, g->nv, g->node_of_vertex, g->vnn, g->vnn_w, g->dq, g->pv, g->pn, grid_of(g->nv)};
+    da[i] = {g->pv, g->pn, g->nv, g->tv, g->tn, g->depth, g->sdepth, F, g->vdata, grid_of(g->nv)};
+    ar[i] = {g->T, g->verts, g->node_vid, g->nbr, g->nbr_w, nn, g->edata, grid_of(ne)};
+    as[i] = {g->blk_a, g->blk_b, g->cstart, g->codes, g->nb, g->vdata, g->vnn, g->vnn_wn, g->node_of_vertex,
+             g->nbr, g->edata, F, g->H, g->rhs, g->Hd, grid_of(g->nb, 256 / kAsmLanes)};
+    inv[i] = {g->Hd, nn, g->Minv, grid_of(nn)};
+    sta[i] = {g->vdata, g->nv, g->edata, ne, F, g->stats_part, (unsigned)kStatsWg};
+    ssum[i] = {g->stats_part, g->stats, 1u};
+    pin[i] = {S, g->rhs, g->Minv, nn, (unsigned)S.nwg};
+    spm[i] = {S, g->H, g->row_start, g->row_ent, g->blk_a, g->blk_b, nn, (unsigned)S.nwg_s};
+    upd[i] = {S, g->Minv, nn, F, (unsigned)S.nwg};
+    nup[i] = {S.x, g->T, nn, g->dq, grid_of(nn)};
+  }
+  BatchTables T;
+  const size_t o_tvt = T.add(tvt), o_tnm = T.add(tnm), o_bl = T.add(bl), o_da = T.add(da), o_ar = T.add(ar),
+               o_as = T.add(as), o_inv = T.add(inv), o_sta = T.add(sta), o_ssum = T.add(ssum), o_pin = T.add(pin),
+               o_spm = T.add(spm), o_upd = T.add(upd), o_nup = T.add(nup);
+  if (T.host.size() > b->table_cap) {
+    if (b->tables) DM_HIP(hipFree(b->tables));
+    b->tables = nullptr;
+    DM_HIP(hipMalloc((void **)&b->tables, T.host.size()));
+    b->table_cap = T.host.size();
+  }
+  DM_HIP(hipMemcpy(b->tables, T.host.data(), T.host.size(), hipMemcpyHostToDevice));
+  char *D = b->tables;
+  hipStream_t st = b->stream;
+  const unsigned Ky = (unsigned)K;
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  hipGraph_t graph = nullptr;
+  DM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(kb_nr_target_vert, dim3(max_grid(tvt), Ky), dim3(256), 0, st,
+                     (const b_nr_target_vert *)(D + o_tvt));
+  hipLaunchKernelGGL(kb_nr_target_norm, dim3(max_grid(tnm), Ky), dim3(256), 0, st,
+                     (const b_nr_target_norm *)(D + o_tnm));
+  for (int it = 0; it < gn; ++it) {
+    hipLaunchKernelGGL(kb_nr_blend, dim3(max_grid(bl), Ky), dim3(256), 0, st, (const b_nr_blend *)(D + o_bl));
+    hipLaunchKernelGGL(kb_nr_data, dim3(max_grid(da), Ky), dim3(256), 0, st, (const b_nr_data *)(D + o_da));
+    hipLaunchKernelGGL(kb_nr_arap, dim3(max_grid(ar), Ky), dim3(256), 0, st, (const b_nr_arap *)(D + o_ar));
+    hipLaunchKernelGGL(kb_nr_assemble, dim3(max_grid(as), Ky), dim3(256), 0, st,
+                       (const b_nr_assemble *)(D + o_as));
+    hipLaunchKernelGGL(kb_nr_invdiag, dim3(max_grid(inv), Ky), dim3(256), 0, st, (const b_nr_invdiag *)(D + o_inv));
+    if (it + 1 == gn) {
+      hipLaunchKernelGGL(kb_nr_stats, dim3(max_grid(sta), Ky), dim3(256), 0, st, (const b_nr_stats *)(D + o_sta));
+      hipLaunchKernelGGL(kb_nr_stats_sum, dim3(1, Ky), dim3(64), 0, st, (const b_nr_stats_sum *)(D + o_ssum));
+    }
+    hipLaunchKernelGGL(kb_pcg_init, dim3(max_grid(pin), Ky), dim3(kPcgThreads), 0, st,
+                       (const b_pcg_init *)(D + o_pin));
+    for (int k = 0; k < pcg_its; ++k) {
+      hipLaunchKernelGGL(kb_pcg_spmv, dim3(max_grid(spm), Ky), dim3(64 * kSpmvNodes), 0, st,
+                         (const b_pcg_spmv *)(D + o_spm));
+      hipLaunchKernelGGL(kb_pcg_update, dim3(max_grid(upd), Ky), dim3(kPcgThreads), 0, st,
+                         (const b_pcg_update *)(D + o_upd), k);
+    }
+    hipLaunchKernelGGL(kb_nr_update, dim3(max_grid(nup), Ky), dim3(256), 0, st, (const b_nr_update *)(D + o_nup));
+  }
+  hipError_t ce = hipStreamEndCapture(st, &graph);
+  if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+  ce = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ce != hipSuccess) { b->exec = nullptr; return hip_fail(ce, "hipGraphInstantiate"); }
+  b->key_gn = gn; b->key_pcg = pcg_its; b->key_rows = rows; b->key_cols = cols;
+  b->gens.resize(K);
+  for (int i = 0; i < K; ++i) b->gens[i] = b->gs[i]->buf_gen;
+  return DM_OK;
 }
 
 extern "C" {
@@ -1438,6 +1839,63 @@ int dm_nonrigid_solve_many(dm_graph *const *graphs, int32_t n, const float *cons
     if (rc == DM_OK) rc = r;
   }
   return rc;
+}
+
+int dm_nonrigid_batch_create(dm_graph *const *graphs, int32_t n, dm_nonrigid_batch **out) {
+  if (!graphs || n < 1 || n > 65535 || !out) { set_error("dm_nonrigid_batch_create: invalid argument"); return DM_E_INVALID; }
+  for (int i = 0; i < n; ++i) {
+    if (!graphs[i]) { set_error("dm_nonrigid_batch_create: null graph"); return DM_E_INVALID; }
+    for (int j = 0; j < i; ++j)
+      if (graphs[j] == graphs[i]) { set_error("dm_nonrigid_batch_create: a graph appears twice"); return DM_E_INVALID; }
+  }
+  auto *b = new dm_nonrigid_batch();
+  b->gs.assign(graphs, graphs + n);
+  const hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete b; return hip_fail(e, "hipStreamCreateWithFlags"); }
+  *out = b;
+  return DM_OK;
+}
+
+int dm_nonrigid_batch_solve(dm_nonrigid_batch *b, const float *const *depths, const dm_sensor *sensor,
+                            const double *poses, const dm_nonrigid_props *pr, double *stats) {
+  if (!b || !depths || !sensor || !poses || !pr || sensor->rows == 0 || sensor->cols == 0 ||
+      pr->gn_iterations < 0 || pr->pcg_iterations < 0 || pr->pcg_iterations > 1000) {
+    set_error("dm_nonrigid_batch_solve: invalid argument");
+    return DM_E_INVALID;
+  }
+  const int K = (int)b->gs.size();
+  for (int i = 0; i < K; ++i)
+    if (!depths[i]) { set_error("dm_nonrigid_batch_solve: null frame"); return DM_E_INVALID; }
+  const size_t P = (size_t)sensor->rows * sensor->cols;
+  DM_HIP(hipStreamSynchronize(b->stream));  // the pinned frame records may still be read by the last call
+  for (dm_graph *g : b->gs) {
+    DM_HIP(hipStreamSynchronize(g->stream));  // single-graph work on the graph's own stream
+    DM_TRY(nr_reserve(g, P, pr->pcg_iterations));
+  }
+  for (int i = 0; i < K; ++i) {
+    dm_graph *g = b->gs[i];
+    fill_frame(g, 0, sensor, poses + 16 * (size_t)i, pr, false);
+    DM_HIP(hipMemcpyAsync(g->d_frame, g->h_frame, sizeof(NrFrame), hipMemcpyHostToDevice, b->stream));
+    DM_HIP(hipMemcpyAsync(g->depth, depths[i], P * 4, hipMemcpyHostToDevice, b->stream));
+  }
+  bool moved = !b->exec || b->key_gn != pr->gn_iterations || b->key_pcg != pr->pcg_iterations ||
+               b->key_rows != (int)sensor->rows || b->key_cols != (int)sensor->cols || (int)b->gens.size() != K;
+  for (int i = 0; i < K && !moved; ++i) moved = b->gens[i] != b->gs[i]->buf_gen;
+  if (moved) {
+    DM_HIP(hipStreamSynchronize(b->stream));
+    DM_TRY(batch_capture(b, (int)sensor->rows, (int)sensor->cols, pr->gn_iterations, pr->pcg_iterations));
+  }
+  DM_HIP(hipGraphLaunch(b->exec, b->stream));
+  for (dm_graph *g : b->gs) DM_TRY(stats_copy(g, pr, b->stream));
+  DM_HIP(hipStreamSynchronize(b->stream));
+  if (stats)
+    for (int i = 0; i < K; ++i) read_stats(b->gs[i], stats + 4 * (size_t)i);
+  return DM_OK;
+}
+
+int dm_nonrigid_batch_destroy(dm_nonrigid_batch *b) {
+  delete b;
+  return DM_OK;
 }
 
 int dm_nonrigid_solve_pair(dm_graph *g, const float *target_depth, const float *source_depth,

@@ -411,6 +411,11 @@ class NonRigidICP:
                                                      C.byref(pr), N.ptr(st)))
         return [self._stats(s) for s in st]
 
+    def batch(self, graphs):
+        """A NonRigidBatch of these distinct graphs, solved with this
+        solver's properties."""
+        return NonRigidBatch(self, graphs)
+
     def solve_mesh(self, graph, target_vertices, target_normals, pose, **override):
         """nonRigidICP::solve(warpField, blender, targetMesh, pose)
         (nonRigidICP.cu:706-852): target vertex v corresponds to canonical
@@ -426,6 +431,48 @@ class NonRigidICP:
                                                  N.ptr(st)))
         return self._stats(st)
 
+
+class NonRigidBatch:
+    """A fixed set of distinct graphs whose solves share launches: every
+    kernel of the solve runs once per step for all of them (graph =
+    blockIdx.y), replayed from one captured HIP graph.  Holds references to
+    its graphs; close it before closing them."""
+
+    def __init__(self, solver, graphs):
+        self.solver = solver
+        self.graphs = list(graphs)
+        self._lib = self.graphs[0]._lib
+        n = len(self.graphs)
+        gptr = (C.c_void_p * n)(*[g._h.value for g in self.graphs])
+        h = C.c_void_p()
+        N.check(self._lib.dm_nonrigid_batch_create(gptr, n, C.byref(h)))
+        self._h = h
+
+    def solve(self, depths, sensor, poses, **override):
+        """One solve per graph, graph i against depths[i] seen from poses[i];
+        returns the stats of each (as NonRigidICP.solve)."""
+        pr = self.solver._props(override)
+        n = len(self.graphs)
+        if n != len(depths) or n != len(poses):
+            raise ValueError("one depth frame and one pose per graph")
+        ds = [_f32(d) for d in depths]
+        dptr = (C.c_void_p * n)(*[d.ctypes.data for d in ds])
+        P = np.ascontiguousarray(np.stack([np.asarray(p, np.float64).reshape(16) for p in poses]))
+        st = np.zeros((n, 4), np.float64)
+        N.check(self._lib.dm_nonrigid_batch_solve(self._h, dptr, C.byref(_sensor(sensor)), N.ptr(P), C.byref(pr),
+                                                  N.ptr(st)))
+        return [self.solver._stats(s) for s in st]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dm_nonrigid_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def bilateral_filter(depth, radius=3, sigma_depth=0.03, sigma_space=2.0):

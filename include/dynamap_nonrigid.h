@@ -98,6 +98,22 @@ DM_API int dm_nonrigid_solve(dm_graph *g, const float *depth, const dm_sensor *s
 DM_API int dm_nonrigid_solve_many(dm_graph *const *graphs, int32_t n, const float *const *depths,
                                   const dm_sensor *sensor, const double *poses, const dm_nonrigid_props *props,
                                   double *stats);
+/* A fixed set of n distinct graphs solved in shared launches (no reference
+ * counterpart; for callers that track several deforming objects at frame
+ * rate): each kernel of the solve runs once per step for all n graphs
+ * (graph = blockIdx.y, its arguments in a device table), and the whole
+ * solve is one captured HIP graph replayed per call.  The batch keeps
+ * pointers to the graphs: destroy it before any of them.  A graph may be in
+ * several batches and solved on its own between batch calls. */
+typedef struct dm_nonrigid_batch dm_nonrigid_batch;
+DM_API int dm_nonrigid_batch_create(dm_graph *const *graphs, int32_t n, dm_nonrigid_batch **out);
+/* One dm_nonrigid_solve per graph of the batch: graph i against depths[i]
+ * seen from poses[16 i .. 16 i + 15], same sensor and props; each result
+ * equals that of dm_nonrigid_solve on the same graph and frame.  stats (may
+ * be NULL): 4 per graph, as dm_nonrigid_solve. */
+DM_API int dm_nonrigid_batch_solve(dm_nonrigid_batch *b, const float *const *depths, const dm_sensor *sensor,
+                                   const double *poses, const dm_nonrigid_props *props, double *stats);
+DM_API int dm_nonrigid_batch_destroy(dm_nonrigid_batch *b);
 /* nonRigidICP::init(warpField, Prop, problemSize) (nonRigidICP.cu:18-40):
  * sizes the solver's frame buffers for `observations` pixels (the
  * reference's problemSize, rows * cols) and pcg_iterations, so that later

@@ -299,3 +299,44 @@ def test_solve_many_equals_one_at_a_time():
         solver.solve_many([many[0], many[0]], frames[:2], SENS, poses[:2])
     for g in seq + many:
         g.close()
+
+
+def test_batch_equals_one_at_a_time():
+    """dm_nonrigid_batch (every kernel of the solve launched once for all
+    graphs, graph = blockIdx.y, replayed from one captured HIP graph)
+    against the same solves one after another on their own: node transforms
+    bit-equal and stats equal, over three calls.  The graphs differ in size
+    (node radius and vertex count), so each kernel's batched grid is the
+    largest graph's and the smaller graphs' extra workgroups must return;
+    the third call uses 160x120 frames, for which the batch captures its
+    solve again."""
+    meshes = [cylinder(200, 100), cylinder(200, 100), cylinder(120, 60)]
+    radii = [0.025, 0.035, 0.03]
+    frames, poses = [], []
+    for f in range(9):
+        P = cyl_pose(f)
+        frames.append(cyl_depth(f, P))
+        poses.append(P)
+    solver = dm.NonRigidICP()
+    seq = [dm.DefGraph(V, Nn, r, 0.05) for (V, Nn), r in zip(meshes, radii)]
+    bat = [dm.DefGraph(V, Nn, r, 0.05) for (V, Nn), r in zip(meshes, radii)]
+    assert len({g.nodeNum for g in bat}) == 3
+    batch = solver.batch(bat)
+    half = dict(rows=120, cols=160, fx=131.25, fy=131.25, cx=79.5, cy=59.5)
+    st_seq, st_bat = [], []
+    for step in range(3):
+        idx = [k + 3 * step for k in range(3)]
+        sens = SENS if step < 2 else half
+        fr = [frames[i] if step < 2 else cyl_depth(i, poses[i], half) for i in idx]
+        st_seq.append([solver.solve(seq[k], fr[k], sens, poses[i]) for k, i in enumerate(idx)])
+        st_bat.append(batch.solve(fr, sens, [poses[i] for i in idx]))
+        for k in range(3):
+            assert np.array_equal(seq[k].transforms().view(np.uint64), bat[k].transforms().view(np.uint64)), \
+                f"call {step} graph {k}"
+    assert st_seq == st_bat
+    assert all(s["residuals"] > 0 for row in st_bat for s in row)
+    with pytest.raises(RuntimeError, match="twice"):
+        solver.batch([bat[0], bat[1], bat[0]])
+    batch.close()
+    for g in seq + bat:
+        g.close()
