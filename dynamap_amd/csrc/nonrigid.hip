@@ -43,7 +43,6 @@ constexpr int kNK = 4;              // neighbours per vertex and per node
 constexpr int kMaxNodes = 1 << 15;  // node ids fit 15 bits of the block keys
 constexpr int kNodeBits = 15;
 constexpr uint32_t kArapTag = 0x80000000u;
-constexpr int kPcgThreads = 192;    // 32 nodes x 6 rows per workgroup
 
 struct dq8 { float rw, rx, ry, rz, dw, dx, dy, dz; };
 
@@ -276,6 +275,15 @@ __global__ __launch_bounds__(256) void k_nr_row_start(const uint32_t *__restrict
   const int r = (int)(keys[i] >> kNodeBits);
   const int rp = i == 0 ? -1 : (int)(keys[i - 1] >> kNodeBits);
   for (int q = rp + 1; q <= r; ++q) row_start[q] = (int)i;
+}
+
+// SpMV row entries: the block (and transpose bit) with the column node (the
+// sorted key's low bits) beside it, so that one load gives both
+__global__ __launch_bounds__(256) void k_nr_row_ec(const uint32_t *__restrict__ keys,
+                                                  const uint32_t *__restrict__ ent, long long n,
+                                                  uint2 *__restrict__ row_ec) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) row_ec[i] = make_uint2(ent[i], keys[i] & ((1u << kNodeBits) - 1u));
 }
 
 // ---------------------------------------------------------------- dual quaternions
@@ -808,207 +816,293 @@ __global__ __launch_bounds__(256) void kb_nr_assemble(const b_nr_assemble *__res
 
 // Block-Jacobi preconditioner: the FP64 inverse of each diagonal block (one
 // thread per node; LDL^T once, then the 6 unit columns; a block that is not
-// positive definite gets 0).
-__device__ __forceinline__ void d_nr_invdiag(const double *__restrict__ Hd, int nn,
-                                                   double *__restrict__ Minv) {
+// positive definite gets 0), and the PCG's start vectors of the node: x = 0,
+// r = rhs, u = M r (row sums in column order).
+__device__ __forceinline__ void d_nr_invdiag(const double *__restrict__ Hd, int nn, double *__restrict__ Minv,
+                                             const double *__restrict__ rhs, double *__restrict__ x,
+                                             double *__restrict__ r, double *__restrict__ u) {
   const int a = blockIdx.x * 256 + threadIdx.x;
   if (a >= nn) return;
-  double h[36], L[36], D[6];
+  double h[36], L[36], D[6], b[6], uo[6] = {0, 0, 0, 0, 0, 0};
   for (int q = 0; q < 36; ++q) h[q] = Hd[36 * (size_t)a + q];
+  for (int q = 0; q < 6; ++q) b[q] = rhs[6 * (size_t)a + q];
   const bool ok = ldlt6_factor(h, L, D) == 0;
   double *M = Minv + 36 * (size_t)a;
   for (int c = 0; c < 6; ++c) {
     double e6[6] = {0, 0, 0, 0, 0, 0}, col[6] = {0, 0, 0, 0, 0, 0};
     e6[c] = 1.0;
     if (ok) ldlt6_solve(L, D, e6, col);
-    for (int r = 0; r < 6; ++r) M[6 * r + c] = col[r];
+    for (int k = 0; k < 6; ++k) {
+      M[6 * k + c] = col[k];
+      uo[k] += col[k] * b[c];
+    }
+  }
+  for (int k = 0; k < 6; ++k) {
+    x[6 * (size_t)a + k] = 0.0;
+    r[6 * (size_t)a + k] = b[k];
+    u[6 * (size_t)a + k] = uo[k];
   }
 }
-__global__ __launch_bounds__(256) void k_nr_invdiag(const double *__restrict__ Hd, int nn,
-                                                   double *__restrict__ Minv) { d_nr_invdiag(Hd, nn, Minv); }
+__global__ __launch_bounds__(256) void k_nr_invdiag(const double *__restrict__ Hd, int nn, double *__restrict__ Minv,
+                                                   const double *__restrict__ rhs, double *__restrict__ x,
+                                                   double *__restrict__ r, double *__restrict__ u) {
+  d_nr_invdiag(Hd, nn, Minv, rhs, x, r, u);
+}
 struct b_nr_invdiag {
-  const double * Hd;
+  const double *Hd;
   int nn;
-  double * Minv;
+  double *Minv;
+  const double *rhs;
+  double *x, *r, *u;
   unsigned grid;  // workgroups of this graph's launch
 };
 // k_nr_invdiag for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
 __global__ __launch_bounds__(256) void kb_nr_invdiag(const b_nr_invdiag *__restrict__ t) {
   const b_nr_invdiag &a = t[blockIdx.y];
   if (blockIdx.x >= a.grid) return;
-  d_nr_invdiag(a.Hd, a.nn, a.Minv);
+  d_nr_invdiag(a.Hd, a.nn, a.Minv, a.rhs, a.x, a.r, a.u);
 }
 
 // ---------------------------------------------------------------- PCG
-// Block-Jacobi PCG in the Chronopoulos-Gear form: with u = M r and w = A u,
-// s = A p is carried along (s = w + beta s), so an iteration is one SpMV
-// kernel (w = A u, partials of w.u) and one update kernel (alpha, beta from
-// the partials; p, s, x, r, u; partials of r.u).  Mathematically the
-// standard PCG iteration, with one kernel boundary less per iteration.
+// Block-Jacobi PCG in the pipelined form of Ghysels and Vanroose ("Hiding
+// global synchronization latency in the preconditioned Conjugate Gradient
+// algorithm", Parallel Computing 40 (2014), algorithm 3).  Besides x, r,
+// u = M r and w = A u it carries p, s = A p, q = M s and t = A q by
+// recurrences, and m = M w, so that an iteration's SpMV (n = A m) needs
+// nothing of the iteration's dot products.  One kernel per iteration adds
+// the previous kernel's partials of r.u and w.u (the same fixed order in
+// every workgroup), computes n = A m for its nodes, updates the eight
+// vectors of its nodes' rows and writes m and the partials for the next
+// iteration.  Mathematically the standard PCG iteration (the same x_k), with
+// one kernel boundary per iteration.
+constexpr int kSpmvNodes = 16;  // wavefronts (nodes) per workgroup
+
 struct PcgState {
-  double *x, *r, *z, *p, *q, *s;  // 6 N each; z = u = M r, q = w = A u, s = A p
-  double *part_pq;            // per-workgroup partials of w.u (written by spmv, read by update)
-  double *part_rz;            // per-workgroup partials of r.u, two halves of nwg: init writes
-                              // half 0, update k reads half k & 1 and writes half (k + 1) & 1, so
-                              // no workgroup overwrites a partial another one may still read
-  double *rz;                 // rz[k] = r_k . u_k (k = 0 .. max)
-  double *al;                 // al[k]: step length of iteration k
-  int *flag;                  // [0] converged, [1] iterations run
-  int nwg;                    // workgroups of init/update (kPcgThreads rows each)
-  int nwg_s;                  // workgroups of the SpMV (kSpmvNodes nodes each)
+  double *x, *r, *u, *w;  // 6 N each: iterate, residual, u = M r, w = A u
+  double *p, *s, *q, *t;  // direction p, s = A p, q = M s, t = A q
+  double *m0, *m1;        // m = M w: iteration k reads m_(k & 1), writes m_((k + 1) & 1)
+  double *part;           // per-workgroup partials, two halves of (r.u [nwg], w.u [nwg]): the
+                          // start kernel writes half 0, iteration k reads half k & 1 and writes
+                          // half (k + 1) & 1, so no workgroup overwrites a partial another may still read
+  double *rz;             // rz[k] = r_k . u_k (k = 0 .. max)
+  double *al;             // al[k]: step length of iteration k
+  int *flag;              // [0] converged, [1] iterations run
+  int nwg;                // workgroups of the start and iteration kernels (kSpmvNodes nodes each)
 };
 
-__device__ void wg_partial(double x, double *lds, double *part) {
-  const double w = wave_sum(x);
-  __syncthreads();  // lds may still be read by an earlier sum of this workgroup
-  const int wid = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) lds[wid] = w;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int k = 0; k < kPcgThreads / 64; ++k) s += lds[k];
-    part[blockIdx.x] = s;
-  }
-}
-
-// z = Minv r for the node of this thread (all 6 rows of a node are in the workgroup)
-__device__ __forceinline__ double precond_row(const double *Minv, const double *rl, int node, int row) {
-  const double *M = Minv + 36 * (size_t)node + 6 * row;
-  double s = 0.0;
-#pragma unroll
-  for (int c = 0; c < 6; ++c) s += M[c] * rl[c];
-  return s;
-}
-
-__device__ __forceinline__ void d_pcg_init(PcgState S, const double *__restrict__ rhs,
-                                                         const double *__restrict__ Minv, int nn) {
-  __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1];
-  const int t = blockIdx.x * kPcgThreads + threadIdx.x;
-  const int node = t / 6, row = t - node * 6;
-  const bool act = node < nn;
-  const double r = act ? rhs[t] : 0.0;
-  rl[threadIdx.x] = r;
-  __syncthreads();
-  double z = 0.0;
-  if (act) {
-    z = precond_row(Minv, rl + (threadIdx.x - row), node, row);
-    S.x[t] = 0.0; S.r[t] = r; S.z[t] = z;
-  }
-  wg_partial(r * z, lds, S.part_rz);
-  if (t == 0) { S.flag[0] = 0; S.flag[1] = 0; }
-}
-__global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgState S, const double *__restrict__ rhs,
-                                                         const double *__restrict__ Minv, int nn) { d_pcg_init(S, rhs, Minv, nn); }
-struct b_pcg_init {
-  PcgState S;
-  const double * rhs;
-  const double * Minv;
-  int nn;
-  unsigned grid;  // workgroups of this graph's launch
-};
-// k_pcg_init for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
-__global__ __launch_bounds__(kPcgThreads) void kb_pcg_init(const b_pcg_init *__restrict__ t) {
-  const b_pcg_init &a = t[blockIdx.y];
-  if (blockIdx.x >= a.grid) return;
-  d_pcg_init(a.S, a.rhs, a.Minv, a.nn);
-}
-
-// w = H u, one wavefront per node: lane l < 60 takes output row l % 6 of the
-// block entries l / 6, l / 6 + 10, ... of the node's block row, so all the
-// entries' loads are in flight together; the 10 lane sums of a row are
-// added in a fixed order through LDS.  Partials of w.u per workgroup.
-constexpr int kSpmvNodes = 16;  // wavefronts (nodes) per workgroup (fewer partials for the update to add)
-__device__ __forceinline__ void d_pcg_spmv(PcgState S, const float *__restrict__ H,
-                                                             const int *__restrict__ row_start,
-                                                             const uint32_t *__restrict__ row_ent,
-                                                             const int *__restrict__ blk_a,
-                                                             const int *__restrict__ blk_b, int nn) {
-  if (S.flag[0]) return;
-  __shared__ double acc_l[64 * kSpmvNodes], lds[kSpmvNodes];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int node = blockIdx.x * kSpmvNodes + wv;
-  double acc = 0.0;
-  if (node < nn && lane < 60) {
-    const int k0 = row_start[node], n6 = 6 * (row_start[node + 1] - k0);
-    const int row = lane % 6;
-    for (int pair = lane; pair < n6; pair += 60) {
-      const uint32_t ent = row_ent[k0 + pair / 6];
-      const int blk = (int)(ent & 0x7fffffffu);
-      const bool tr = ent >> 31;
-      const int col = tr ? blk_a[blk] : blk_b[blk];
+// Row `lane` (< 6) of (A v) at the wavefront's node, 0 in the other lanes.
+// Lane j takes the node's block entries j, j + 64, ... whole (a 6x6 block,
+// transposed for the lower triangle, times the column node's 6 values), so
+// a node's entries are one round of loads; the six row sums are then added
+// over the lanes by a fixed xor tree.
+//
+// kLowReg: the same products and sums, with each block loaded in two halves
+// of three rows, the second after the first's products (fewer registers:
+// two workgroups per CU for batched solves); results are bit-identical.
+template <bool kLowReg>
+__device__ __forceinline__ double spmv_node(const float *__restrict__ H, const int *__restrict__ row_start,
+                                            const uint2 *__restrict__ row_ec,
+                                            const double *__restrict__ v, int node, int nn, int lane) {
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  if (node < nn) {
+    const int k1 = row_start[node + 1];  // node is uniform in the wavefront: scalar loads
+    for (int k = row_start[node] + lane; k < k1; k += 64) {
+      const uint2 ec = row_ec[k];
+      const int blk = (int)(ec.x & 0x7fffffffu);
+      const bool tr = ec.x >> 31;
       const float *B = H + 36 * (size_t)blk;
-      const double *pc = S.z + 6 * (size_t)col;
-      double sv = 0.0;
+      const double2 *V2 = reinterpret_cast<const double2 *>(v + 6 * (size_t)ec.y);
+      double vc[6];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) sv += (double)(tr ? B[6 * c + row] : B[6 * row + c]) * pc[c];
-      acc += sv;
+      for (int q = 0; q < 3; ++q) {
+        const double2 d = V2[q];
+        vc[2 * q] = d.x; vc[2 * q + 1] = d.y;
+      }
+      if constexpr (!kLowReg) {
+        const float4 *B4 = reinterpret_cast<const float4 *>(B);
+        float b[36];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          const float4 f = B4[q];
+          b[4 * q] = f.x; b[4 * q + 1] = f.y; b[4 * q + 2] = f.z; b[4 * q + 3] = f.w;
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          double sv = 0.0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) sv += (double)(tr ? b[6 * c + r] : b[6 * r + c]) * vc[c];
+          acc[r] += sv;
+        }
+      } else {
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+          float b[18];  // entries (r, c) of rows r = 3g .. 3g + 2
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) b[6 * r + c] = B[tr ? 6 * c + 3 * g + r : 6 * (3 * g + r) + c];
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            double sv = 0.0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) sv += (double)b[6 * r + c] * vc[c];
+            acc[3 * g + r] += sv;
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
     }
   }
-  acc_l[threadIdx.x] = acc;
-  __syncthreads();
-  double contrib = 0.0;
-  if (node < nn && lane < 6) {
-    double q = 0.0;
-    for (int j = lane; j < 60; j += 6) q += acc_l[64 * wv + j];
-    const size_t t = 6 * (size_t)node + lane;
-    S.q[t] = q;
-    contrib = S.z[t] * q;
+  double out = 0.0;
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    const double t = wave_sum(acc[r]);
+    if (lane == r) out = t;
   }
-  const double w = wave_sum(contrib);
-  if (lane == 0) lds[wv] = w;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double sum = 0.0;
-    for (int k = 0; k < kSpmvNodes; ++k) sum += lds[k];
-    S.part_pq[blockIdx.x] = sum;
-  }
+  return node < nn ? out : 0.0;
 }
-__global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_spmv(PcgState S, const float *__restrict__ H,
-                                                             const int *__restrict__ row_start,
-                                                             const uint32_t *__restrict__ row_ent,
-                                                             const int *__restrict__ blk_a,
-                                                             const int *__restrict__ blk_b, int nn) { d_pcg_spmv(S, H, row_start, row_ent, blk_a, blk_b, nn); }
-struct b_pcg_spmv {
+
+// The workgroup's partials of r.u and w.u into half `half` of the partial
+// array: the values of threads 0 .. 6 kSpmvNodes - 1 (wavefronts 0 and 1; the
+// others pass 0), a fixed xor tree per wavefront, then wavefront 0's sum plus
+// wavefront 1's.
+__device__ __forceinline__ void pcg_partials(const PcgState &S, double ru, double wu, int half, double *lds) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (wv < 2) {
+    ru = wave_sum(ru);
+    wu = wave_sum(wu);
+    if (lane == 0) { lds[2 * wv] = ru; lds[2 * wv + 1] = wu; }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2)
+    S.part[(size_t)half * 2 * S.nwg + (size_t)threadIdx.x * S.nwg + blockIdx.x] = lds[threadIdx.x] + lds[2 + threadIdx.x];
+}
+
+// Rows of the workgroup: thread i < 6 kSpmvNodes takes row 6 kSpmvNodes
+// blockIdx.x + i (node blockIdx.x kSpmvNodes + i / 6).
+constexpr int kPcgRows = 6 * kSpmvNodes;
+
+// Start: w = A u, m = M w, partials of r_0.u_0 and w_0.u_0 (x = 0, r = rhs and
+// u = M r are written by the preconditioner kernel).
+template <bool kLowReg>
+__device__ __forceinline__ void d_pcg_start(PcgState S, const float *__restrict__ H,
+                                            const int *__restrict__ row_start, const uint2 *__restrict__ row_ec,
+                                            const double *__restrict__ Minv, int nn) {
+  __shared__ double nl[kPcgRows], lds[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = threadIdx.x;
+  const size_t e = (size_t)kPcgRows * blockIdx.x + i;
+  const bool own = i < kPcgRows && e < 6 * (size_t)nn;
+  double mrow[6] = {0, 0, 0, 0, 0, 0}, rv = 0.0, uv = 0.0;
+  auto load_rows = [&] {
+    if (own) {
+      rv = S.r[e];
+      uv = S.u[e];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) mrow[c] = Minv[6 * e + c];
+    }
+  };
+  if constexpr (!kLowReg) load_rows();
+  const int node = blockIdx.x * kSpmvNodes + __builtin_amdgcn_readfirstlane(wv);
+  const double wn = spmv_node<kLowReg>(H, row_start, row_ec, S.u, node, nn, lane);
+  if (lane < 6) nl[6 * wv + lane] = wn;
+  __syncthreads();
+  if constexpr (kLowReg) load_rows();
+  double w = 0.0;
+  if (own) { w = nl[i]; S.w[e] = w; }
+  double mn = 0.0;
+  if (own) {
+    const int r0 = i - (i % 6);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) mn += mrow[c] * nl[r0 + c];
+    S.m0[e] = mn;
+  }
+  pcg_partials(S, own ? rv * uv : 0.0, own ? w * uv : 0.0, 0, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) { S.flag[0] = 0; S.flag[1] = 0; }
+}
+__global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_start(PcgState S, const float *__restrict__ H,
+                                                              const int *__restrict__ row_start,
+                                                              const uint2 *__restrict__ row_ec,
+                                                              const double *__restrict__ Minv, int nn) {
+  d_pcg_start<false>(S, H, row_start, row_ec, Minv, nn);
+}
+struct b_pcg_start {
   PcgState S;
-  const float * H;
-  const int * row_start;
-  const uint32_t * row_ent;
-  const int * blk_a;
-  const int * blk_b;
+  const float *H;
+  const int *row_start;
+  const uint2 *row_ec;
+  const double *Minv;
   int nn;
   unsigned grid;  // workgroups of this graph's launch
 };
-// k_pcg_spmv for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
-__global__ __launch_bounds__(64 * kSpmvNodes) void kb_pcg_spmv(const b_pcg_spmv *__restrict__ t) {
-  const b_pcg_spmv &a = t[blockIdx.y];
+// k_pcg_start for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(64 * kSpmvNodes) __attribute__((amdgpu_waves_per_eu(8))) void kb_pcg_start(
+    const b_pcg_start *__restrict__ t) {
+  const b_pcg_start &a = t[blockIdx.y];
   if (blockIdx.x >= a.grid) return;
-  d_pcg_spmv(a.S, a.H, a.row_start, a.row_ent, a.blk_a, a.blk_b, a.nn);
+  d_pcg_start<true>(a.S, a.H, a.row_start, a.row_ec, a.Minv, a.nn);
 }
 
-// Iteration k: gamma = r.u and delta = w.u from the partials (fixed order,
-// the same in every workgroup); stop when gamma <= tol^2 gamma_0; beta =
-// gamma / gamma_{k-1}, alpha = gamma / (delta - beta gamma / alpha_{k-1})
-// (k = 0: beta = 0, alpha = gamma / delta); p = u + beta p; s = w + beta s;
-// x += alpha p; r -= alpha s; u = Minv r; partial r.u.
-__device__ __forceinline__ void d_pcg_update(PcgState S, const double *__restrict__ Minv, int nn,
-                                                           int it, const NrFrame *Fp) {
-  if (S.flag[0]) return;
-  __shared__ double rl[kPcgThreads], lds[kPcgThreads / 64 + 1], l2[2];
-  // gamma (wavefront 0) and delta (wavefront 1) from the workgroup partials,
-  // side by side, each in a fixed order computed identically by every
-  // workgroup: lane l adds partials l, l + 64, ..., then the xor tree of
-  // wave_sum adds the lanes
-  if (threadIdx.x < 128) {
-    const int lane = threadIdx.x & 63;
-    const bool second = threadIdx.x >= 64;
-    const double *part = second ? S.part_pq : S.part_rz + (size_t)(it & 1) * S.nwg;
-    const int n = second ? S.nwg_s : S.nwg;
-    double acc = 0.0;
-    for (int k = lane; k < n; k += 64) acc += part[k];
-    acc = wave_sum(acc);
-    if (lane == 0) l2[second ? 1 : 0] = acc;
+// Iteration k: gamma = r.u and delta = w.u from the partials (wavefronts 2
+// and 3 side by side, lane l adding partials l, l + 64, ... and then the xor
+// tree: the same order in every workgroup); stop when gamma <= tol^2 gamma_0;
+// beta = gamma / gamma_{k-1}, alpha = gamma / (delta - beta gamma /
+// alpha_{k-1}) (k = 0: beta = 0, alpha = gamma / delta); n = A m;
+// t = n + beta t; q = m + beta q; s = w + beta s; p = u + beta p;
+// x += alpha p; r -= alpha s; u -= alpha q; w -= alpha t; m = M w; partials
+// of r.u and w.u.  The partials, the rows' operands and the SpMV's loads
+// are all issued before the first wait.
+template <bool kLowReg>
+__device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__ H,
+                                           const int *__restrict__ row_start, const uint2 *__restrict__ row_ec,
+                                           const double *__restrict__ Minv, int nn, int it, const NrFrame *Fp) {
+  __shared__ double nl[kPcgRows], wl[kPcgRows], lds[4], l2[2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const bool done = S.flag[0] != 0;
+  // wavefronts 2 and 3: lane l adds partials l, l + 64, ... in order; the
+  // first 8 are loaded here and added after the SpMV's loads are issued
+  const bool pw = wv == 2 || wv == 3;
+  const double *part = S.part + (size_t)(it & 1) * 2 * S.nwg + (wv == 3 ? S.nwg : 0);
+  // (kLowReg: these loads wait until after the SpMV)
+  double pv8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  auto load_partials = [&] {
+    if (pw) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) pv8[j] = part[min(64 * j + lane, S.nwg - 1)];  // in bounds; lanes past nwg skip it below
+    }
+  };
+  const double *m_in = (it & 1) ? S.m1 : S.m0;
+  double *m_out = (it & 1) ? S.m0 : S.m1;
+  const int i = threadIdx.x;
+  const size_t e = (size_t)kPcgRows * blockIdx.x + i;
+  const bool own = i < kPcgRows && e < 6 * (size_t)nn;
+  double mrow[6] = {0, 0, 0, 0, 0, 0}, tv = 0.0, qv = 0.0, sv = 0.0, pv = 0.0, xv = 0.0, rv = 0.0, uv = 0.0,
+         wo = 0.0, mv = 0.0;
+  auto load_rows = [&] {
+    if (own) {
+      if (it > 0) { tv = S.t[e]; qv = S.q[e]; sv = S.s[e]; pv = S.p[e]; }
+      xv = S.x[e]; rv = S.r[e]; uv = S.u[e]; wo = S.w[e]; mv = m_in[e];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) mrow[c] = Minv[6 * e + c];
+    }
+  };
+  if constexpr (!kLowReg) { load_partials(); load_rows(); }
+  const int node = blockIdx.x * kSpmvNodes + __builtin_amdgcn_readfirstlane(wv);
+  // (computed also after convergence, so that the flag's load is not on the SpMV's path)
+  const double n = spmv_node<kLowReg>(H, row_start, row_ec, m_in, node, nn, lane);
+  if (lane < 6) nl[6 * wv + lane] = n;
+  if constexpr (kLowReg) load_partials();
+  if (pw) {
+    double pacc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (64 * j + lane < S.nwg) pacc += pv8[j];
+    for (int k = 512 + lane; k < S.nwg; k += 64) pacc += part[k];
+    pacc = wave_sum(pacc);
+    if (lane == 0) l2[wv - 2] = pacc;
   }
   __syncthreads();
+  if (done) return;
   const double gamma = l2[0], delta = l2[1];
   const double tol = Fp->pcg_tol;
   const bool stop = it == 0 ? !(gamma > 0.0) : (!(gamma > tol * tol * S.rz[0]) || !(gamma > 0.0));
@@ -1024,42 +1118,55 @@ __device__ __forceinline__ void d_pcg_update(PcgState S, const double *__restric
     else S.flag[1] = it + 1;
   }
   if (stop) return;
-  const int t = blockIdx.x * kPcgThreads + threadIdx.x;
-  const int node = t / 6, row = t - node * 6;
-  const bool act = node < nn;
-  double r = 0.0;
-  if (act) {
-    const double pk = it == 0 ? S.z[t] : S.z[t] + beta * S.p[t];
-    const double sk = it == 0 ? S.q[t] : S.q[t] + beta * S.s[t];
-    S.p[t] = pk;
-    S.s[t] = sk;
-    S.x[t] += alpha * pk;
-    r = S.r[t] - alpha * sk;
-    S.r[t] = r;
+  if constexpr (kLowReg) load_rows();
+  double wn = 0.0;
+  if (own) {
+    const double nv = nl[i];
+    tv = it == 0 ? nv : nv + beta * tv;
+    qv = it == 0 ? mv : mv + beta * qv;
+    sv = it == 0 ? wo : wo + beta * sv;
+    pv = it == 0 ? uv : uv + beta * pv;
+    xv = xv + alpha * pv;
+    rv = rv - alpha * sv;
+    uv = uv - alpha * qv;
+    wn = wo - alpha * tv;
+    S.t[e] = tv; S.q[e] = qv; S.s[e] = sv; S.p[e] = pv;
+    S.x[e] = xv; S.r[e] = rv; S.u[e] = uv; S.w[e] = wn;
+    wl[i] = wn;
   }
-  rl[threadIdx.x] = r;
   __syncthreads();
-  double z = 0.0;
-  if (act) {
-    z = precond_row(Minv, rl + (threadIdx.x - row), node, row);
-    S.z[t] = z;
+  if (own) {
+    const int r0 = i - (i % 6);
+    double mn = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) mn += mrow[c] * wl[r0 + c];
+    m_out[e] = mn;
   }
-  wg_partial(r * z, lds, S.part_rz + (size_t)((it + 1) & 1) * S.nwg);
+  pcg_partials(S, own ? rv * uv : 0.0, own ? wn * uv : 0.0, (it + 1) & 1, lds);
 }
-__global__ __launch_bounds__(kPcgThreads) void k_pcg_update(PcgState S, const double *__restrict__ Minv, int nn,
-                                                           int it, const NrFrame *Fp) { d_pcg_update(S, Minv, nn, it, Fp); }
-struct b_pcg_update {
+__global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_iter(PcgState S, const float *__restrict__ H,
+                                                             const int *__restrict__ row_start,
+                                                             const uint2 *__restrict__ row_ec,
+                                                             const double *__restrict__ Minv, int nn, int it,
+                                                             const NrFrame *Fp) {
+  d_pcg_iter<false>(S, H, row_start, row_ec, Minv, nn, it, Fp);
+}
+struct b_pcg_iter {
   PcgState S;
-  const double * Minv;
+  const float *H;
+  const int *row_start;
+  const uint2 *row_ec;
+  const double *Minv;
   int nn;
   const NrFrame *Fp;
   unsigned grid;  // workgroups of this graph's launch
 };
-// k_pcg_update for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
-__global__ __launch_bounds__(kPcgThreads) void kb_pcg_update(const b_pcg_update *__restrict__ t, int it) {
-  const b_pcg_update &a = t[blockIdx.y];
+// k_pcg_iter for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(64 * kSpmvNodes) __attribute__((amdgpu_waves_per_eu(8))) void kb_pcg_iter(
+    const b_pcg_iter *__restrict__ t, int it) {
+  const b_pcg_iter &a = t[blockIdx.y];
   if (blockIdx.x >= a.grid) return;
-  d_pcg_update(a.S, a.Minv, a.nn, it, a.Fp);
+  d_pcg_iter<true>(a.S, a.H, a.row_start, a.row_ec, a.Minv, a.nn, it, a.Fp);
 }
 
 // T_i <- exp(x_i) T_i, then the node's dual quaternion
@@ -1185,6 +1292,7 @@ struct dm_graph {
   // block pattern
   int *blk_a = nullptr, *blk_b = nullptr, *cstart = nullptr, *row_start = nullptr;
   uint32_t *codes = nullptr, *row_ent = nullptr;
+  uint2 *row_ec = nullptr;  // row entries for the SpMV: (row_ent, column node)
   // per-iteration buffers
   float *pv = nullptr, *pn = nullptr, *vdata = nullptr, *H = nullptr;
   double *edata = nullptr, *rhs = nullptr, *Minv = nullptr, *Hd = nullptr, *stats = nullptr, *stats_part = nullptr;
@@ -1384,6 +1492,7 @@ static int graph_finish(dm_graph *g) {
   DM_TRY(dev_alloc(rk1, nrow * 4));
   DM_TRY(dev_alloc(rv0, nrow * 4));
   DM_TRY(g->dalloc(&g->row_ent, nrow));
+  DM_TRY(g->dalloc(&g->row_ec, nrow));
   DM_TRY(g->dalloc(&g->row_start, nn + 1));
   DM_HIP(hipMemcpyAsync(droff.p, roff.data(), nb * 8, hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_nr_row_emit, dim3(grid_of(nb)), dim3(256), 0, st, g->blk_a, g->blk_b, (int)nb,
@@ -1391,6 +1500,8 @@ static int graph_finish(dm_graph *g) {
   DM_TRY(sort_pairs(rk0.as<uint32_t>(), rk1.as<uint32_t>(), rv0.as<uint32_t>(), g->row_ent, nrow, 2 * kNodeBits, st));
   hipLaunchKernelGGL(k_nr_row_start, dim3(grid_of(nrow + 1)), dim3(256), 0, st, rk1.as<uint32_t>(), nrow,
                      g->row_start, (int)nn);
+  hipLaunchKernelGGL(k_nr_row_ec, dim3(grid_of(nrow)), dim3(256), 0, st, rk1.as<uint32_t>(), g->row_ent, nrow,
+                     g->row_ec);
   DM_HIP(hipGetLastError());
   // state and per-iteration buffers
   DM_TRY(g->dalloc(&g->T, nn * 16));
@@ -1408,16 +1519,9 @@ static int graph_finish(dm_graph *g) {
   DM_TRY(g->dalloc(&g->stats, 4));
   DM_TRY(g->dalloc(&g->stats_part, 3 * kStatsWg));
   PcgState &S = g->pcg;
-  S.nwg = (int)grid_of(6 * nn, kPcgThreads);
-  S.nwg_s = (int)grid_of(nn, kSpmvNodes);
-  DM_TRY(g->dalloc(&S.x, nn * 6));
-  DM_TRY(g->dalloc(&S.r, nn * 6));
-  DM_TRY(g->dalloc(&S.z, nn * 6));
-  DM_TRY(g->dalloc(&S.p, nn * 6));
-  DM_TRY(g->dalloc(&S.q, nn * 6));
-  DM_TRY(g->dalloc(&S.s, nn * 6));
-  DM_TRY(g->dalloc(&S.part_pq, S.nwg_s));
-  DM_TRY(g->dalloc(&S.part_rz, 2 * (size_t)S.nwg));
+  S.nwg = (int)grid_of(nn, kSpmvNodes);
+  for (double **v : {&S.x, &S.r, &S.u, &S.w, &S.p, &S.s, &S.q, &S.t, &S.m0, &S.m1}) DM_TRY(g->dalloc(v, nn * 6));
+  DM_TRY(g->dalloc(&S.part, 4 * (size_t)S.nwg));
   DM_TRY(g->dalloc(&S.flag, 2));
   DM_TRY(g->dalloc(&g->d_frame, 1));
   DM_HIP(hipHostMalloc((void **)&g->h_frame, sizeof(NrFrame), hipHostMallocDefault));
@@ -1455,18 +1559,18 @@ static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its, int
     hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 256 / kAsmLanes)), dim3(256), 0, st, g->blk_a, g->blk_b, g->cstart,
                        g->codes, g->nb, g->vdata, g->vnn, g->vnn_wn, g->node_of_vertex, g->nbr, g->edata, F, g->H,
                        g->rhs, g->Hd);
-    hipLaunchKernelGGL(k_nr_invdiag, dim3(grid_of(nn)), dim3(256), 0, st, g->Hd, nn, g->Minv);
+    hipLaunchKernelGGL(k_nr_invdiag, dim3(grid_of(nn)), dim3(256), 0, st, g->Hd, nn, g->Minv, g->rhs, S.x, S.r,
+                       S.u);
     if (it + 1 == gn) {
       hipLaunchKernelGGL(k_nr_stats, dim3(kStatsWg), dim3(256), 0, st, g->vdata, g->nv, g->edata, ne, F,
                          g->stats_part);
       hipLaunchKernelGGL(k_nr_stats_sum, dim3(1), dim3(64), 0, st, g->stats_part, g->stats);
     }
-    hipLaunchKernelGGL(k_pcg_init, dim3(pg), dim3(kPcgThreads), 0, st, S, g->rhs, g->Minv, nn);
-    for (int k = 0; k < pcg_its; ++k) {
-      hipLaunchKernelGGL(k_pcg_spmv, dim3((unsigned)S.nwg_s), dim3(64 * kSpmvNodes), 0, st, S, g->H, g->row_start,
-                         g->row_ent, g->blk_a, g->blk_b, nn);
-      hipLaunchKernelGGL(k_pcg_update, dim3(pg), dim3(kPcgThreads), 0, st, S, g->Minv, nn, k, F);
-    }
+    hipLaunchKernelGGL(k_pcg_start, dim3(pg), dim3(64 * kSpmvNodes), 0, st, S, g->H, g->row_start, g->row_ec,
+                       g->Minv, nn);
+    for (int k = 0; k < pcg_its; ++k)
+      hipLaunchKernelGGL(k_pcg_iter, dim3(pg), dim3(64 * kSpmvNodes), 0, st, S, g->H, g->row_start, g->row_ec,
+                         g->Minv, nn, k, F);
     hipLaunchKernelGGL(k_nr_update, dim3(grid_of(nn)), dim3(256), 0, st, S.x, g->T, nn, g->dq);
   }
 }
@@ -1633,9 +1737,8 @@ static int batch_capture(dm_nonrigid_batch *b, int rows, int cols, int gn, int p
   std::vector<b_nr_invdiag> inv(K);
   std::vector<b_nr_stats> sta(K);
   std::vector<b_nr_stats_sum> ssum(K);
-  std::vector<b_pcg_init> pin(K);
-  std::vector<b_pcg_spmv> spm(K);
-  std::vector<b_pcg_update> upd(K);
+  std::vector<b_pcg_start> pst(K);
+  std::vector<b_pcg_iter> pit(K);
   std::vector<b_nr_update> nup(K);
   for (int i = 0; i < K; ++i) {
     dm_graph *g = b->gs[i];
@@ -1649,18 +1752,17 @@ static int batch_capture(dm_nonrigid_batch *b, int rows, int cols, int gn, int p
     ar[i] = {g->T, g->verts, g->node_vid, g->nbr, g->nbr_w, nn, g->edata, grid_of(ne)};
     as[i] = {g->blk_a, g->blk_b, g->cstart, g->codes, g->nb, g->vdata, g->vnn, g->vnn_wn, g->node_of_vertex,
              g->nbr, g->edata, F, g->H, g->rhs, g->Hd, grid_of(g->nb, 256 / kAsmLanes)};
-    inv[i] = {g->Hd, nn, g->Minv, grid_of(nn)};
+    inv[i] = {g->Hd, nn, g->Minv, g->rhs, S.x, S.r, S.u, grid_of(nn)};
     sta[i] = {g->vdata, g->nv, g->edata, ne, F, g->stats_part, (unsigned)kStatsWg};
     ssum[i] = {g->stats_part, g->stats, 1u};
-    pin[i] = {S, g->rhs, g->Minv, nn, (unsigned)S.nwg};
-    spm[i] = {S, g->H, g->row_start, g->row_ent, g->blk_a, g->blk_b, nn, (unsigned)S.nwg_s};
-    upd[i] = {S, g->Minv, nn, F, (unsigned)S.nwg};
+    pst[i] = {S, g->H, g->row_start, g->row_ec, g->Minv, nn, (unsigned)S.nwg};
+    pit[i] = {S, g->H, g->row_start, g->row_ec, g->Minv, nn, F, (unsigned)S.nwg};
     nup[i] = {S.x, g->T, nn, g->dq, grid_of(nn)};
   }
   BatchTables T;
   const size_t o_tvt = T.add(tvt), o_tnm = T.add(tnm), o_bl = T.add(bl), o_da = T.add(da), o_ar = T.add(ar),
-               o_as = T.add(as), o_inv = T.add(inv), o_sta = T.add(sta), o_ssum = T.add(ssum), o_pin = T.add(pin),
-               o_spm = T.add(spm), o_upd = T.add(upd), o_nup = T.add(nup);
+               o_as = T.add(as), o_inv = T.add(inv), o_sta = T.add(sta), o_ssum = T.add(ssum), o_pst = T.add(pst),
+               o_pit = T.add(pit), o_nup = T.add(nup);
   if (T.host.size() > b->table_cap) {
     if (b->tables) DM_HIP(hipFree(b->tables));
     b->tables = nullptr;
@@ -1689,14 +1791,11 @@ static int batch_capture(dm_nonrigid_batch *b, int rows, int cols, int gn, int p
       hipLaunchKernelGGL(kb_nr_stats, dim3(max_grid(sta), Ky), dim3(256), 0, st, (const b_nr_stats *)(D + o_sta));
       hipLaunchKernelGGL(kb_nr_stats_sum, dim3(1, Ky), dim3(64), 0, st, (const b_nr_stats_sum *)(D + o_ssum));
     }
-    hipLaunchKernelGGL(kb_pcg_init, dim3(max_grid(pin), Ky), dim3(kPcgThreads), 0, st,
-                       (const b_pcg_init *)(D + o_pin));
-    for (int k = 0; k < pcg_its; ++k) {
-      hipLaunchKernelGGL(kb_pcg_spmv, dim3(max_grid(spm), Ky), dim3(64 * kSpmvNodes), 0, st,
-                         (const b_pcg_spmv *)(D + o_spm));
-      hipLaunchKernelGGL(kb_pcg_update, dim3(max_grid(upd), Ky), dim3(kPcgThreads), 0, st,
-                         (const b_pcg_update *)(D + o_upd), k);
-    }
+    hipLaunchKernelGGL(kb_pcg_start, dim3(max_grid(pst), Ky), dim3(64 * kSpmvNodes), 0, st,
+                       (const b_pcg_start *)(D + o_pst));
+    for (int k = 0; k < pcg_its; ++k)
+      hipLaunchKernelGGL(kb_pcg_iter, dim3(max_grid(pit), Ky), dim3(64 * kSpmvNodes), 0, st,
+                         (const b_pcg_iter *)(D + o_pit), k);
     hipLaunchKernelGGL(kb_nr_update, dim3(max_grid(nup), Ky), dim3(256), 0, st, (const b_nr_update *)(D + o_nup));
   }
   hipError_t ce = hipStreamEndCapture(st, &graph);
