@@ -232,6 +232,24 @@ __global__ __launch_bounds__(256) void k_nr_contrib_emit(const int *__restrict__
   keys[o + 2] = pair_key(i, j);  codes[o + 2] = kArapTag | (uint32_t)(e * 4 + 2);
 }
 
+// Per contribution, the weights of the block's two nodes at the vertex (the
+// normalised blend weights of slots a and b; 1 for a vertex that is itself
+// a node); ARAP contributions get 0 (unused).
+__global__ __launch_bounds__(256) void k_nr_contrib_weights(const uint32_t *__restrict__ codes, long long nc,
+                                                           const int *__restrict__ node_of_vertex,
+                                                           const float *__restrict__ vnn_wn, float2 *__restrict__ cw) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nc) return;
+  const uint32_t code = codes[i];
+  float2 w = make_float2(0.f, 0.f);
+  if (!(code & kArapTag)) {
+    const int64_t v = code >> 4;
+    if (node_of_vertex[v] >= 0) w = make_float2(1.f, 1.f);
+    else w = make_float2(vnn_wn[kNK * v + ((code >> 2) & 3)], vnn_wn[kNK * v + (code & 3)]);
+  }
+  cw[i] = w;
+}
+
 __global__ __launch_bounds__(256) void k_flag_heads(const uint32_t *__restrict__ keys, long long n, uint32_t shift,
                                                    int *__restrict__ flag) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -686,13 +704,11 @@ __device__ __forceinline__ double wave_sum(double x) {
 // block took about 153 us per assembly at 4048 nodes, these about 80 us).
 constexpr int kAsmLanes = 16;
 __device__ __forceinline__ void d_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
-                                                    const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
-                                                    int nb, const float *__restrict__ vdata,
-                                                    const int *__restrict__ vnn, const float *__restrict__ vnn_wn,
-                                                    const int *__restrict__ node_of_vertex, const int *__restrict__ nbr,
-                                                    const double *__restrict__ edata, const NrFrame *Fp,
-                                                    float *__restrict__ H, double *__restrict__ rhs,
-                                                    double *__restrict__ Hd) {
+                                              const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
+                                              const float2 *__restrict__ cw, int nb, const float *__restrict__ vdata,
+                                              const int *__restrict__ nbr, const double *__restrict__ edata,
+                                              const NrFrame *Fp, float *__restrict__ H, double *__restrict__ rhs,
+                                              double *__restrict__ Hd) {
   const int blk = blockIdx.x * (256 / kAsmLanes) + threadIdx.x / kAsmLanes;
   const int lane = threadIdx.x % kAsmLanes;
   if (blk >= nb) return;  // uniform per block's lanes (the xor tree stays within them)
@@ -705,28 +721,20 @@ __device__ __forceinline__ void d_nr_assemble(const int *__restrict__ blk_a, con
 #pragma unroll
   for (int q = 0; q < 6; ++q) g[q] = 0.0;
   const double lam = F.lambda;
-  for (int c = cstart[blk] + lane; c < cstart[blk + 1]; c += kAsmLanes) {
-    const uint32_t code = codes[c];
+  // one contribution: a vertex's data term (weights wa, wb of the block's two
+  // nodes at the vertex, from cw) or an ARAP edge's rows
+  auto add_one = [&](uint32_t code, float2 w, float4 d0, float4 d1) {
     if (!(code & kArapTag)) {
-      const int64_t v = code >> 4;
-      const float *d = vdata + 8 * v;
-      if (d[7] == 0.f) continue;
-      double wa, wb;
-      if (node_of_vertex[v] >= 0) {
-        wa = 1.0; wb = 1.0;
-      } else {
-        const int sa = (code >> 2) & 3, sb = code & 3;
-        wa = (double)vnn_wn[kNK * v + sa];
-        wb = (double)vnn_wn[kNK * v + sb];
-      }
-      const double gv[6] = {d[0], d[1], d[2], d[3], d[4], d[5]};
+      if (d1.w == 0.f) return;
+      const double wa = (double)w.x, wb = (double)w.y;
+      const double gv[6] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y};
       const double s = wa * wb;
 #pragma unroll
       for (int r = 0; r < 6; ++r)
 #pragma unroll
         for (int q = 0; q < 6; ++q) h[6 * r + q] += s * gv[r] * gv[q];
       if (diag) {
-        const double rr = (double)d[6];
+        const double rr = (double)d1.z;
 #pragma unroll
         for (int r = 0; r < 6; ++r) g[r] -= wa * gv[r] * rr;
       }
@@ -757,6 +765,22 @@ __device__ __forceinline__ void d_nr_assemble(const int *__restrict__ blk_a, con
           for (int r = 0; r < 6; ++r) g[r] -= lam * U[r] * rc;
       }
     }
+  };
+  // contributions c, c + kAsmLanes, ... in order, two at a time so that both
+  // vertices' rows are loaded together
+  const int ce = cstart[blk + 1];
+  const float4 *V4 = reinterpret_cast<const float4 *>(vdata);
+  for (int c = cstart[blk] + lane; c < ce; c += 2 * kAsmLanes) {
+    const int c1 = c + kAsmLanes;
+    const bool has1 = c1 < ce;
+    const uint32_t k0 = codes[c], k1 = has1 ? codes[c1] : kArapTag;
+    const float2 w0 = cw[c], w1 = has1 ? cw[c1] : make_float2(0.f, 0.f);
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 a0 = z4, a1 = z4, b0 = z4, b1 = z4;
+    if (!(k0 & kArapTag)) { const size_t v = k0 >> 4; a0 = V4[2 * v]; a1 = V4[2 * v + 1]; }
+    if (!(k1 & kArapTag)) { const size_t v = k1 >> 4; b0 = V4[2 * v]; b1 = V4[2 * v + 1]; }
+    add_one(k0, w0, a0, a1);
+    if (has1) add_one(k1, w1, b0, b1);
   }
   auto lanes_sum = [](double x) {
 #pragma unroll
@@ -783,35 +807,34 @@ __device__ __forceinline__ void d_nr_assemble(const int *__restrict__ blk_a, con
 }
 __global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
                                                     const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
-                                                    int nb, const float *__restrict__ vdata,
-                                                    const int *__restrict__ vnn, const float *__restrict__ vnn_wn,
-                                                    const int *__restrict__ node_of_vertex, const int *__restrict__ nbr,
+                                                    const float2 *__restrict__ cw, int nb,
+                                                    const float *__restrict__ vdata, const int *__restrict__ nbr,
                                                     const double *__restrict__ edata, const NrFrame *Fp,
                                                     float *__restrict__ H, double *__restrict__ rhs,
-                                                    double *__restrict__ Hd) { d_nr_assemble(blk_a, blk_b, cstart, codes, nb, vdata, vnn, vnn_wn, node_of_vertex, nbr, edata, Fp, H, rhs, Hd); }
+                                                    double *__restrict__ Hd) {
+  d_nr_assemble(blk_a, blk_b, cstart, codes, cw, nb, vdata, nbr, edata, Fp, H, rhs, Hd);
+}
 struct b_nr_assemble {
-  const int * blk_a;
-  const int * blk_b;
-  const int * cstart;
-  const uint32_t * codes;
+  const int *blk_a;
+  const int *blk_b;
+  const int *cstart;
+  const uint32_t *codes;
+  const float2 *cw;
   int nb;
-  const float * vdata;
-  const int * vnn;
-  const float * vnn_wn;
-  const int * node_of_vertex;
-  const int * nbr;
-  const double * edata;
+  const float *vdata;
+  const int *nbr;
+  const double *edata;
   const NrFrame *Fp;
-  float * H;
-  double * rhs;
-  double * Hd;
+  float *H;
+  double *rhs;
+  double *Hd;
   unsigned grid;  // workgroups of this graph's launch
 };
 // k_nr_assemble for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
 __global__ __launch_bounds__(256) void kb_nr_assemble(const b_nr_assemble *__restrict__ t) {
   const b_nr_assemble &a = t[blockIdx.y];
   if (blockIdx.x >= a.grid) return;
-  d_nr_assemble(a.blk_a, a.blk_b, a.cstart, a.codes, a.nb, a.vdata, a.vnn, a.vnn_wn, a.node_of_vertex, a.nbr, a.edata, a.Fp, a.H, a.rhs, a.Hd);
+  d_nr_assemble(a.blk_a, a.blk_b, a.cstart, a.codes, a.cw, a.nb, a.vdata, a.nbr, a.edata, a.Fp, a.H, a.rhs, a.Hd);
 }
 
 // Block-Jacobi preconditioner: the FP64 inverse of each diagonal block (one
@@ -1292,6 +1315,7 @@ struct dm_graph {
   // block pattern
   int *blk_a = nullptr, *blk_b = nullptr, *cstart = nullptr, *row_start = nullptr;
   uint32_t *codes = nullptr, *row_ent = nullptr;
+  float2 *cw = nullptr;  // per contribution: the weights of the block's two nodes at the vertex
   uint2 *row_ec = nullptr;  // row entries for the SpMV: (row_ent, column node)
   // per-iteration buffers
   float *pv = nullptr, *pn = nullptr, *vdata = nullptr, *H = nullptr;
@@ -1462,6 +1486,9 @@ static int graph_finish(dm_graph *g) {
                      coff.as<long long>(), nv, g->nbr, (int)nn, ndata, k0.as<uint32_t>(), c0.as<uint32_t>());
   DM_HIP(hipGetLastError());
   DM_TRY(sort_pairs(k0.as<uint32_t>(), k1.as<uint32_t>(), c0.as<uint32_t>(), g->codes, nc, 2 * kNodeBits, st));
+  DM_TRY(g->dalloc(&g->cw, nc));
+  hipLaunchKernelGGL(k_nr_contrib_weights, dim3(grid_of(nc)), dim3(256), 0, st, g->codes, nc, g->node_of_vertex,
+                     g->vnn_wn, g->cw);
   DM_TRY(dev_alloc(flag, nc * 4));
   DM_TRY(dev_alloc(frank, nc * 8));
   hipLaunchKernelGGL(k_flag_heads, dim3(grid_of(nc)), dim3(256), 0, st, k1.as<uint32_t>(), nc, 0u, flag.as<int>());
@@ -1556,9 +1583,8 @@ static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its, int
                          g->mesh_n, F, g->vdata);
     hipLaunchKernelGGL(k_nr_arap, dim3(grid_of(ne)), dim3(256), 0, st, g->T, g->verts, g->node_vid, g->nbr,
                        g->nbr_w, nn, g->edata);
-    hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 256 / kAsmLanes)), dim3(256), 0, st, g->blk_a, g->blk_b, g->cstart,
-                       g->codes, g->nb, g->vdata, g->vnn, g->vnn_wn, g->node_of_vertex, g->nbr, g->edata, F, g->H,
-                       g->rhs, g->Hd);
+    hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 256 / kAsmLanes)), dim3(256), 0, st, g->blk_a, g->blk_b,
+                       g->cstart, g->codes, g->cw, g->nb, g->vdata, g->nbr, g->edata, F, g->H, g->rhs, g->Hd);
     hipLaunchKernelGGL(k_nr_invdiag, dim3(grid_of(nn)), dim3(256), 0, st, g->Hd, nn, g->Minv, g->rhs, S.x, S.r,
                        S.u);
     if (it + 1 == gn) {
@@ -1750,8 +1776,8 @@ static int batch_capture(dm_nonrigid_batch *b, int rows, int cols, int gn, int p
     bl[i] = {g->verts, g->normals, g->nv, g->node_of_vertex, g->vnn, g->vnn_w, g->dq, g->pv, g->pn, grid_of(g->nv)};
     da[i] = {g->pv, g->pn, g->nv, g->tv, g->tn, g->depth, g->sdepth, F, g->vdata, grid_of(g->nv)};
     ar[i] = {g->T, g->verts, g->node_vid, g->nbr, g->nbr_w, nn, g->edata, grid_of(ne)};
-    as[i] = {g->blk_a, g->blk_b, g->cstart, g->codes, g->nb, g->vdata, g->vnn, g->vnn_wn, g->node_of_vertex,
-             g->nbr, g->edata, F, g->H, g->rhs, g->Hd, grid_of(g->nb, 256 / kAsmLanes)};
+    as[i] = {g->blk_a, g->blk_b, g->cstart, g->codes, g->cw, g->nb, g->vdata, g->nbr, g->edata, F, g->H, g->rhs,
+             g->Hd, grid_of(g->nb, 256 / kAsmLanes)};
     inv[i] = {g->Hd, nn, g->Minv, g->rhs, S.x, S.r, S.u, grid_of(nn)};
     sta[i] = {g->vdata, g->nv, g->edata, ne, F, g->stats_part, (unsigned)kStatsWg};
     ssum[i] = {g->stats_part, g->stats, 1u};
@@ -2041,6 +2067,9 @@ int dm_graph_create_nodes(const float *vertices, const float *normals, int64_t n
 int dm_graph_update_nodes(dm_graph *g) {
   if (!g) { set_error("null argument"); return DM_E_INVALID; }
   DM_TRY(graph_links(g));
+  hipLaunchKernelGGL(k_nr_contrib_weights, dim3(grid_of(g->ncontrib)), dim3(256), 0, g->stream, g->codes,
+                     g->ncontrib, g->node_of_vertex, g->vnn_wn, g->cw);
+  DM_HIP(hipGetLastError());
   DM_HIP(hipStreamSynchronize(g->stream));
   return DM_OK;
 }
