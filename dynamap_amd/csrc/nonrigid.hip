@@ -690,18 +690,39 @@ __device__ __forceinline__ void arap_rows(const double *ed, int c, double Ji[6],
   for (int q = 0; q < 6; ++q) { Ji[q] *= w; Jj[q] *= -w; }
 }
 
+// x moved across lanes by a DPP pattern (both halves of the double)
+template <int kCtrl> __device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), kCtrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), kCtrl, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+// Sum over each row of 16 lanes, the same fixed tree in every lane: lane
+// pairs (l, l ^ 1), (l, l ^ 2) (quad permutes), then the half-row and row
+// mirrors (lane l with 7 - l and 15 - l within its 8 and 16 lanes); each
+// step adds two equal-shaped partial sums, so every lane of the row ends
+// with the same value.
+__device__ __forceinline__ double row_sum(double x) {
+  x += dpp_f64<0xb1>(x);   // quad_perm [1, 0, 3, 2]
+  x += dpp_f64<0x4e>(x);   // quad_perm [2, 3, 0, 1]
+  x += dpp_f64<0x141>(x);  // row_half_mirror
+  x += dpp_f64<0x140>(x);  // row_mirror
+  return x;
+}
+// Sum over the wavefront: the row sums, then lanes l ^ 16 and l ^ 32.
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+  x = row_sum(x);
+  x += __shfl_xor(x, 16, 64);
+  x += __shfl_xor(x, 32, 64);
   return x;
 }
 
 // kAsmLanes lanes per block (4 blocks per wavefront): H_ab (and for a == b
 // the rhs, the damping and the FP64 block itself, inverted by k_nr_invdiag).
 // Lanes take contributions strided by kAsmLanes; the lane sums are combined
-// by a fixed xor tree within the block's lanes.  16 lanes give each lane
-// several independent gathers in flight and a 4-level tree (a wavefront per
-// block took about 153 us per assembly at 4048 nodes, these about 80 us).
+// by the fixed DPP tree of row_sum within the block's lanes (one row of 16).
+// 16 lanes give each lane several independent gathers in flight and a
+// 4-level tree (a wavefront per block took about 153 us per assembly at
+// 4048 nodes, these about 65 us).
 constexpr int kAsmLanes = 16;
 __device__ __forceinline__ void d_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
                                               const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
@@ -711,7 +732,7 @@ __device__ __forceinline__ void d_nr_assemble(const int *__restrict__ blk_a, con
                                               double *__restrict__ Hd) {
   const int blk = blockIdx.x * (256 / kAsmLanes) + threadIdx.x / kAsmLanes;
   const int lane = threadIdx.x % kAsmLanes;
-  if (blk >= nb) return;  // uniform per block's lanes (the xor tree stays within them)
+  if (blk >= nb) return;  // uniform per block's lanes (the tree stays within them)
   const NrFrame &F = *Fp;
   const int a = blk_a[blk], b = blk_b[blk];
   const bool diag = a == b;
@@ -782,11 +803,8 @@ __device__ __forceinline__ void d_nr_assemble(const int *__restrict__ blk_a, con
     add_one(k0, w0, a0, a1);
     if (has1) add_one(k1, w1, b0, b1);
   }
-  auto lanes_sum = [](double x) {
-#pragma unroll
-    for (int off = kAsmLanes / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
-  };
+  static_assert(kAsmLanes == 16, "a block's lanes are one DPP row");
+  auto lanes_sum = [](double x) { return row_sum(x); };
 #pragma unroll
   for (int q = 0; q < 36; ++q) h[q] = lanes_sum(h[q]);
 #pragma unroll
@@ -917,7 +935,7 @@ struct PcgState {
 // Lane j takes the node's block entries j, j + 64, ... whole (a 6x6 block,
 // transposed for the lower triangle, times the column node's 6 values), so
 // a node's entries are one round of loads; the six row sums are then added
-// over the lanes by a fixed xor tree.
+// over the lanes by the fixed tree of wave_sum.
 //
 // kLowReg: the same products and sums, with each block loaded in two halves
 // of three rows, the second after the first's products (fewer registers:
@@ -987,7 +1005,7 @@ __device__ __forceinline__ double spmv_node(const float *__restrict__ H, const i
 
 // The workgroup's partials of r.u and w.u into half `half` of the partial
 // array: the values of threads 0 .. 6 kSpmvNodes - 1 (wavefronts 0 and 1; the
-// others pass 0), a fixed xor tree per wavefront, then wavefront 0's sum plus
+// others pass 0), the fixed tree of wave_sum per wavefront, then wavefront 0's sum plus
 // wavefront 1's.
 __device__ __forceinline__ void pcg_partials(const PcgState &S, double ru, double wu, int half, double *lds) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
