@@ -188,4 +188,42 @@ __device__ __forceinline__ uint32_t to_u8(float x) {
   return (uint32_t)x;
 }
 
+// ---- lane sums through DPP row permutes
+// x moved across lanes by a DPP pattern
+template <int kCtrl> __device__ __forceinline__ float dpp_f32(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xf, 0xf, false));
+}
+template <int kCtrl> __device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), kCtrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), kCtrl, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+// Sum over each row of 16 lanes, the same fixed tree in every lane: lane
+// pairs (l, l ^ 1), (l, l ^ 2) (quad permutes), then the half-row and row
+// mirrors (lane l with 7 - l and 15 - l within its 8 and 16 lanes); each
+// step adds two equal-shaped partial sums, so every lane of the row ends
+// with the same value.  Every lane of the row must be active.
+template <class T> __device__ __forceinline__ T row_sum(T x) {
+  if constexpr (sizeof(T) == 8) {
+    x += dpp_f64<0xb1>(x);   // quad_perm [1, 0, 3, 2]
+    x += dpp_f64<0x4e>(x);   // quad_perm [2, 3, 0, 1]
+    x += dpp_f64<0x141>(x);  // row_half_mirror
+    x += dpp_f64<0x140>(x);  // row_mirror
+  } else {
+    x += dpp_f32<0xb1>(x);
+    x += dpp_f32<0x4e>(x);
+    x += dpp_f32<0x141>(x);
+    x += dpp_f32<0x140>(x);
+  }
+  return x;
+}
+// Sum over the wavefront (all 64 lanes active): the row sums, then lanes
+// l ^ 16 and l ^ 32; every lane ends with the same value.
+template <class T> __device__ __forceinline__ T wave_sum(T x) {
+  x = row_sum(x);
+  x += __shfl_xor(x, 16, 64);
+  x += __shfl_xor(x, 32, 64);
+  return x;
+}
+
 }  // namespace dm

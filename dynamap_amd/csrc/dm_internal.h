@@ -166,7 +166,7 @@ void launch_bilateral(const float *in, float *out, int rows, int cols, int r, fl
 constexpr int kIcpPix = 1024;   // pixels per ICP reduction workgroup (256 threads x 4)
 __host__ __device__ inline int icp_wg_count(int pixels) { return (pixels + kIcpPix - 1) / kIcpPix; }
 // The FP64 sum of the per-workgroup records in the order the device uses
-// (thread t of 256 takes records t, t+256, ...; xor tree within each
+// (thread t of 256 takes records t, t+256, ...; wave_sum's tree within each
 // wavefront; the four wavefronts as (w0 + w1) + (w2 + w3)).
 void icp_sum_records(const float *part, int nwg, double *sums);
 

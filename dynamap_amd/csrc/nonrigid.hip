@@ -690,32 +690,6 @@ __device__ __forceinline__ void arap_rows(const double *ed, int c, double Ji[6],
   for (int q = 0; q < 6; ++q) { Ji[q] *= w; Jj[q] *= -w; }
 }
 
-// x moved across lanes by a DPP pattern (both halves of the double)
-template <int kCtrl> __device__ __forceinline__ double dpp_f64(double x) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), kCtrl, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), kCtrl, 0xf, 0xf, false);
-  return __hiloint2double(hi, lo);
-}
-// Sum over each row of 16 lanes, the same fixed tree in every lane: lane
-// pairs (l, l ^ 1), (l, l ^ 2) (quad permutes), then the half-row and row
-// mirrors (lane l with 7 - l and 15 - l within its 8 and 16 lanes); each
-// step adds two equal-shaped partial sums, so every lane of the row ends
-// with the same value.
-__device__ __forceinline__ double row_sum(double x) {
-  x += dpp_f64<0xb1>(x);   // quad_perm [1, 0, 3, 2]
-  x += dpp_f64<0x4e>(x);   // quad_perm [2, 3, 0, 1]
-  x += dpp_f64<0x141>(x);  // row_half_mirror
-  x += dpp_f64<0x140>(x);  // row_mirror
-  return x;
-}
-// Sum over the wavefront: the row sums, then lanes l ^ 16 and l ^ 32.
-__device__ __forceinline__ double wave_sum(double x) {
-  x = row_sum(x);
-  x += __shfl_xor(x, 16, 64);
-  x += __shfl_xor(x, 32, 64);
-  return x;
-}
-
 // kAsmLanes lanes per block (4 blocks per wavefront): H_ab (and for a == b
 // the rhs, the damping and the FP64 block itself, inverted by k_nr_invdiag).
 // Lanes take contributions strided by kAsmLanes; the lane sums are combined

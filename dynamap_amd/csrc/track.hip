@@ -263,8 +263,8 @@ __device__ void icp_step_device(const DmView &v, int level, int it, const double
 }
 
 // One linearisation: every thread accumulates 4 pixels (strided by 256 for
-// coalescing) in FP32; a wavefront xor tree and an LDS stage reduce them to
-// one record per workgroup.
+// coalescing) in FP32; a wavefront tree (wave_sum: DPP row permutes, then
+// two shuffles) and an LDS stage reduce them to one record per workgroup.
 __global__ __launch_bounds__(256) void k_icp_reduce(const DmView *__restrict__ views, const int *__restrict__ active,
                                                      int level, int it) {
   const int slot_ = active[blockIdx.y];
@@ -283,14 +283,9 @@ __global__ __launch_bounds__(256) void k_icp_reduce(const DmView *__restrict__ v
     const int i = blockIdx.x * kIcpPix + q * 256 + threadIdx.x;
     if (i < P) icp_pixel(v, level, s, i, vals);
   }
-  // wavefront tree (pairwise) then LDS across the four waves
+  // wavefront tree (wave_sum) then LDS across the four waves
 #pragma unroll
-  for (int k = 0; k < kVals; ++k) {
-    float x = vals[k];
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-    vals[k] = x;
-  }
+  for (int k = 0; k < kVals; ++k) vals[k] = wave_sum(vals[k]);
   __shared__ float red[4][kVals];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) {
@@ -380,7 +375,7 @@ __global__ __launch_bounds__(256) void k_icp_reduce_mfma(const DmView *__restric
 }
 
 // Sum the records in FP64 in a fixed order (thread t takes records t, t+256,
-// ...; xor tree per wavefront; (w0 + w1) + (w2 + w3); host replica:
+// ...; wave_sum's tree per wavefront; (w0 + w1) + (w2 + w3); host replica:
 // icp_sum_records) and take the Gauss-Newton step (one workgroup per session).
 __global__ __launch_bounds__(256) void k_icp_solve(const DmView *__restrict__ views, const int *__restrict__ active,
                                                     int level, int it) {
@@ -398,12 +393,7 @@ __global__ __launch_bounds__(256) void k_icp_solve(const DmView *__restrict__ vi
     for (int k = 0; k < kVals; ++k) acc[k] += (double)rec[k];
   }
 #pragma unroll
-  for (int k = 0; k < kVals; ++k) {
-    double x = acc[k];
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-    acc[k] = x;
-  }
+  for (int k = 0; k < kVals; ++k) acc[k] = wave_sum(acc[k]);
   __shared__ double dred[4][kVals];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) {
@@ -472,13 +462,25 @@ void icp_sum_records(const float *part, int nwg, double *sums) {
       for (int w = t; w < nwg; w += 256) a += (double)part[(size_t)w * kIcpVals + k];
       lane[t] = a;
     }
+    // wave_sum's steps: lane i adds lane src(i) for src = i ^ 1, i ^ 2, the
+    // half-row mirror, the row mirror, i ^ 16, i ^ 32
+    auto src = [](int step, int i) {
+      switch (step) {
+        case 0: return i ^ 1;
+        case 1: return i ^ 2;
+        case 2: return (i & ~7) | (7 - (i & 7));
+        case 3: return (i & ~15) | (15 - (i & 15));
+        case 4: return i ^ 16;
+        default: return i ^ 32;
+      }
+    };
     double wv[4];
     for (int q = 0; q < 4; ++q) {
       double x[64];
       for (int i = 0; i < 64; ++i) x[i] = lane[64 * q + i];
-      for (int off = 32; off >= 1; off >>= 1) {
+      for (int step = 0; step < 6; ++step) {
         double y[64];
-        for (int i = 0; i < 64; ++i) y[i] = x[i] + x[i ^ off];
+        for (int i = 0; i < 64; ++i) y[i] = x[i] + x[src(step, i)];
         for (int i = 0; i < 64; ++i) x[i] = y[i];
       }
       wv[q] = x[0];
