@@ -1071,19 +1071,29 @@ template <bool kLowReg>
 __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__ H,
                                            const int *__restrict__ row_start, const uint2 *__restrict__ row_ec,
                                            const double *__restrict__ Minv, int nn, int it, const NrFrame *Fp) {
-  __shared__ double nl[kPcgRows], wl[kPcgRows], lds[4], l2[2];
+  __shared__ double nl[kPcgRows], wl[kPcgRows], lds[4], l2[2], sc[5];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const bool done = S.flag[0] != 0;
   // wavefronts 2 and 3: lane l adds partials l, l + 64, ... in order; the
-  // first 8 are loaded here and added after the SpMV's loads are issued
+  // first 8 are loaded here and added after the SpMV's loads are issued.
+  // Lanes 0..4 of wavefront 2 also read the tolerance, r_0.u_0, the previous
+  // iteration's gamma and alpha and the converged flag, with vector loads
+  // (per-lane addresses) that overlap the SpMV rather than scalar loads that
+  // the kernel's first scalar wait would include.
   const bool pw = wv == 2 || wv == 3;
   const double *part = S.part + (size_t)(it & 1) * 2 * S.nwg + (wv == 3 ? S.nwg : 0);
   // (kLowReg: these loads wait until after the SpMV)
-  double pv8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double pv8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, scv = 0.0;
   auto load_partials = [&] {
     if (pw) {
 #pragma unroll
       for (int j = 0; j < 8; ++j) pv8[j] = part[min(64 * j + lane, S.nwg - 1)];  // in bounds; lanes past nwg skip it below
+      const int km = it > 0 ? it - 1 : 0;
+      const double *q = lane == 0 ? &Fp->pcg_tol
+                        : lane == 1 ? S.rz
+                        : lane == 2 ? S.rz + km
+                        : lane == 3 ? S.al + km
+                                    : reinterpret_cast<const double *>(S.flag);  // flag[0] in the low word
+      if (wv == 2 && lane < 5) scv = *q;
     }
   };
   const double *m_in = (it & 1) ? S.m1 : S.m0;
@@ -1115,16 +1125,17 @@ __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__
     for (int k = 512 + lane; k < S.nwg; k += 64) pacc += part[k];
     pacc = wave_sum(pacc);
     if (lane == 0) l2[wv - 2] = pacc;
+    if (wv == 2 && lane < 5) sc[lane] = scv;
   }
   __syncthreads();
-  if (done) return;
+  if (__double2loint(sc[4]) != 0) return;  // converged at an earlier iteration
+  const double tol = sc[0], rz0 = sc[1], rzp = sc[2], alp = sc[3];
   const double gamma = l2[0], delta = l2[1];
-  const double tol = Fp->pcg_tol;
-  const bool stop = it == 0 ? !(gamma > 0.0) : (!(gamma > tol * tol * S.rz[0]) || !(gamma > 0.0));
+  const bool stop = it == 0 ? !(gamma > 0.0) : (!(gamma > tol * tol * rz0) || !(gamma > 0.0));
   double beta = 0.0, alpha = gamma / delta;
   if (it > 0) {
-    beta = gamma / S.rz[it - 1];
-    alpha = gamma / (delta - beta * gamma / S.al[it - 1]);
+    beta = gamma / rzp;
+    alpha = gamma / (delta - beta * gamma / alp);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     S.rz[it] = gamma;
