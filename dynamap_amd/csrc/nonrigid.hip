@@ -354,13 +354,11 @@ __device__ __forceinline__ void q_rotate(float w, float x, float y, float z, con
 
 // dqBlender::blendVertex (dqBlender.cu:162-186) with Kavan's sign rule and
 // one normalisation; a node vertex takes its node's dq (K)
-__device__ __forceinline__ void d_nr_blend(const float *__restrict__ verts, const float *__restrict__ normals,
-                                                 int64_t nv, const int *__restrict__ node_of_vertex,
-                                                 const int *__restrict__ vnn, const float *__restrict__ vnn_w,
-                                                 const dq8 *__restrict__ dq, float *__restrict__ out_v,
-                                                 float *__restrict__ out_n) {
-  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (v >= nv) return;
+// Blended position and (when nrm) normal of canonical vertex v.
+__device__ __forceinline__ void blend_vertex(const float *__restrict__ verts, const float *__restrict__ normals,
+                                             const int *__restrict__ node_of_vertex, const int *__restrict__ vnn,
+                                             const float *__restrict__ vnn_w, const dq8 *__restrict__ dq, int64_t v,
+                                             float pos[3], float *nrm) {
   dq8 b;
   const int nd = node_of_vertex[v];
   if (nd >= 0) {
@@ -387,40 +385,33 @@ __device__ __forceinline__ void d_nr_blend(const float *__restrict__ verts, cons
   const float tx = ((b.dx * b.rw - b.rx * b.dw) + (b.ry * b.dz - b.rz * b.dy)) * 2.f;
   const float ty = ((b.dy * b.rw - b.ry * b.dw) + (b.rz * b.dx - b.rx * b.dz)) * 2.f;
   const float tz = ((b.dz * b.rw - b.rz * b.dw) + (b.rx * b.dy - b.ry * b.dx)) * 2.f;
-  out_v[3 * v] = rp[0] + tx;
-  out_v[3 * v + 1] = rp[1] + ty;
-  out_v[3 * v + 2] = rp[2] + tz;
-  if (out_n) {
+  pos[0] = rp[0] + tx;
+  pos[1] = rp[1] + ty;
+  pos[2] = rp[2] + tz;
+  if (nrm) {
     const float n[3] = {normals[3 * v], normals[3 * v + 1], normals[3 * v + 2]};
-    float rn[3];
-    q_rotate(b.rw, b.rx, b.ry, b.rz, n, rn);
-    out_n[3 * v] = rn[0]; out_n[3 * v + 1] = rn[1]; out_n[3 * v + 2] = rn[2];
+    q_rotate(b.rw, b.rx, b.ry, b.rz, n, nrm);
   }
+}
+__device__ __forceinline__ void d_nr_blend(const float *__restrict__ verts, const float *__restrict__ normals,
+                                           int64_t nv, const int *__restrict__ node_of_vertex,
+                                           const int *__restrict__ vnn, const float *__restrict__ vnn_w,
+                                           const dq8 *__restrict__ dq, float *__restrict__ out_v,
+                                           float *__restrict__ out_n) {
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nv) return;
+  float pos[3], nrm[3];
+  blend_vertex(verts, normals, node_of_vertex, vnn, vnn_w, dq, v, pos, out_n ? nrm : nullptr);
+  out_v[3 * v] = pos[0];
+  out_v[3 * v + 1] = pos[1];
+  out_v[3 * v + 2] = pos[2];
+  if (out_n) { out_n[3 * v] = nrm[0]; out_n[3 * v + 1] = nrm[1]; out_n[3 * v + 2] = nrm[2]; }
 }
 __global__ __launch_bounds__(256) void k_nr_blend(const float *__restrict__ verts, const float *__restrict__ normals,
                                                  int64_t nv, const int *__restrict__ node_of_vertex,
                                                  const int *__restrict__ vnn, const float *__restrict__ vnn_w,
                                                  const dq8 *__restrict__ dq, float *__restrict__ out_v,
                                                  float *__restrict__ out_n) { d_nr_blend(verts, normals, nv, node_of_vertex, vnn, vnn_w, dq, out_v, out_n); }
-struct b_nr_blend {
-  const float * verts;
-  const float * normals;
-  int64_t nv;
-  const int * node_of_vertex;
-  const int * vnn;
-  const float * vnn_w;
-  const dq8 * dq;
-  float * out_v;
-  float * out_n;
-  unsigned grid;  // workgroups of this graph's launch
-};
-// k_nr_blend for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
-__global__ __launch_bounds__(256) void kb_nr_blend(const b_nr_blend *__restrict__ t) {
-  const b_nr_blend &a = t[blockIdx.y];
-  if (blockIdx.x >= a.grid) return;
-  d_nr_blend(a.verts, a.normals, a.nv, a.node_of_vertex, a.vnn, a.vnn_w, a.dq, a.out_v, a.out_n);
-}
-
 // ---------------------------------------------------------------- Gauss-Newton
 struct NrFrame {
   int rows, cols;
@@ -518,19 +509,17 @@ __device__ __forceinline__ void nr_data_record(const double p[3], const double n
 
 // source depth (sdepth) gates a pixel when F.use_src: the reference's range
 // test on the source image (nonRigidICP.cu:291-292)
-__device__ __forceinline__ void d_nr_data(const float *__restrict__ pv, const float *__restrict__ pn,
-                                                int64_t nv, const float4 *__restrict__ tv,
-                                                const float4 *__restrict__ tn, const float *__restrict__ depth,
-                                                const float *__restrict__ sdepth, const NrFrame *Fp,
-                                                float *__restrict__ out) {
+// Data-term record of vertex v at blended position pv and normal pn.
+__device__ __forceinline__ void data_vertex(const float pv[3], const float pn[3], int64_t v,
+                                            const float4 *__restrict__ tv, const float4 *__restrict__ tn,
+                                            const float *__restrict__ depth, const float *__restrict__ sdepth,
+                                            const NrFrame *Fp, float *__restrict__ out) {
   const NrFrame &F = *Fp;
-  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (v >= nv) return;
   float *o = out + 8 * v;
   o[7] = 0.f;
   const double *pose = F.pose;
-  const double p[3] = {pv[3 * v], pv[3 * v + 1], pv[3 * v + 2]};
-  const double nw[3] = {pn[3 * v], pn[3 * v + 1], pn[3 * v + 2]};
+  const double p[3] = {pv[0], pv[1], pv[2]};
+  const double nw[3] = {pn[0], pn[1], pn[2]};
   double pc[3], ncam[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -557,45 +546,18 @@ __device__ __forceinline__ void d_nr_data(const float *__restrict__ pv, const fl
   if (fabs(ncam[0] * nt[0] + ncam[1] * nt[1] + ncam[2] * nt[2]) < F.normal_cos) return;
   nr_data_record(p, nt, q, pose, o);
 }
-__global__ __launch_bounds__(256) void k_nr_data(const float *__restrict__ pv, const float *__restrict__ pn,
-                                                int64_t nv, const float4 *__restrict__ tv,
-                                                const float4 *__restrict__ tn, const float *__restrict__ depth,
-                                                const float *__restrict__ sdepth, const NrFrame *Fp,
-                                                float *__restrict__ out) { d_nr_data(pv, pn, nv, tv, tn, depth, sdepth, Fp, out); }
-struct b_nr_data {
-  const float * pv;
-  const float * pn;
-  int64_t nv;
-  const float4 * tv;
-  const float4 * tn;
-  const float * depth;
-  const float * sdepth;
-  const NrFrame *Fp;
-  float * out;
-  unsigned grid;  // workgroups of this graph's launch
-};
-// k_nr_data for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
-__global__ __launch_bounds__(256) void kb_nr_data(const b_nr_data *__restrict__ t) {
-  const b_nr_data &a = t[blockIdx.y];
-  if (blockIdx.x >= a.grid) return;
-  d_nr_data(a.pv, a.pn, a.nv, a.tv, a.tn, a.depth, a.sdepth, a.Fp, a.out);
-}
-
 // mesh target (nonRigidICP.cu:354-376): target vertex v and its normal, in
 // camera coordinates, correspond to canonical vertex v; same gates and
 // record as the depth target
-__device__ __forceinline__ void d_nr_data_mesh(const float *__restrict__ pv, const float *__restrict__ pn,
-                                                     int64_t nv, const float *__restrict__ mv,
-                                                     const float *__restrict__ mn, const NrFrame *Fp,
-                                                     float *__restrict__ out) {
+__device__ __forceinline__ void data_mesh_vertex(const float pv[3], const float pn[3], int64_t v,
+                                                 const float *__restrict__ mv, const float *__restrict__ mn,
+                                                 const NrFrame *Fp, float *__restrict__ out) {
   const NrFrame &F = *Fp;
-  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (v >= nv) return;
   float *o = out + 8 * v;
   o[7] = 0.f;
   const double *pose = F.pose;
-  const double p[3] = {pv[3 * v], pv[3 * v + 1], pv[3 * v + 2]};
-  const double nw[3] = {pn[3 * v], pn[3 * v + 1], pn[3 * v + 2]};
+  const double p[3] = {pv[0], pv[1], pv[2]};
+  const double nw[3] = {pn[0], pn[1], pn[2]};
   double pc[3], ncam[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -610,34 +572,11 @@ __device__ __forceinline__ void d_nr_data_mesh(const float *__restrict__ pv, con
   if (fabs(ncam[0] * nt[0] + ncam[1] * nt[1] + ncam[2] * nt[2]) < F.normal_cos) return;
   nr_data_record(p, nt, q, pose, o);
 }
-__global__ __launch_bounds__(256) void k_nr_data_mesh(const float *__restrict__ pv, const float *__restrict__ pn,
-                                                     int64_t nv, const float *__restrict__ mv,
-                                                     const float *__restrict__ mn, const NrFrame *Fp,
-                                                     float *__restrict__ out) { d_nr_data_mesh(pv, pn, nv, mv, mn, Fp, out); }
-struct b_nr_data_mesh {
-  const float * pv;
-  const float * pn;
-  int64_t nv;
-  const float * mv;
-  const float * mn;
-  const NrFrame *Fp;
-  float * out;
-  unsigned grid;  // workgroups of this graph's launch
-};
-// k_nr_data_mesh for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
-__global__ __launch_bounds__(256) void kb_nr_data_mesh(const b_nr_data_mesh *__restrict__ t) {
-  const b_nr_data_mesh &a = t[blockIdx.y];
-  if (blockIdx.x >= a.grid) return;
-  d_nr_data_mesh(a.pv, a.pn, a.nv, a.mv, a.mn, a.Fp, a.out);
-}
-
 // ARAP edge e = i * K + k (j = k-th neighbour of i): a = T_i g_j, b = T_j g_j,
 // w_ij = max(w_i[k], w_j[k]) (nonRigidICP.cu:494-515); out[8 e] = a, b, w
-__device__ __forceinline__ void d_nr_arap(const double *__restrict__ T, const float *__restrict__ verts,
-                                                const int *__restrict__ node_vid, const int *__restrict__ nbr,
-                                                const float *__restrict__ nbr_w, int nn, double *__restrict__ out) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= nn * kNK) return;
+__device__ __forceinline__ void arap_edge(const double *__restrict__ T, const float *__restrict__ verts,
+                                          const int *__restrict__ node_vid, const int *__restrict__ nbr,
+                                          const float *__restrict__ nbr_w, int e, double *__restrict__ out) {
   const int i = e / kNK, k = e - i * kNK, j = nbr[e];
   const double *Ti = T + 16 * (size_t)i, *Tj = T + 16 * (size_t)j;
   const int64_t gv = node_vid[j];
@@ -654,24 +593,49 @@ __device__ __forceinline__ void d_nr_arap(const double *__restrict__ T, const fl
   o[6] = wij;
   o[7] = 0.0;
 }
-__global__ __launch_bounds__(256) void k_nr_arap(const double *__restrict__ T, const float *__restrict__ verts,
-                                                const int *__restrict__ node_vid, const int *__restrict__ nbr,
-                                                const float *__restrict__ nbr_w, int nn, double *__restrict__ out) { d_nr_arap(T, verts, node_vid, nbr, nbr_w, nn, out); }
-struct b_nr_arap {
-  const double * T;
-  const float * verts;
-  const int * node_vid;
-  const int * nbr;
-  const float * nbr_w;
+// The linearisation's per-vertex and per-edge terms in one launch:
+// workgroups 0 .. vwg - 1 blend vertex v and write its data-term record
+// (depth target, or with kMesh the vertex-for-vertex mesh target), the rest
+// write the ARAP record of edge e.
+struct NrTerms {
+  const float *verts, *normals;
+  const int *node_of_vertex, *vnn;
+  const float *vnn_w;
+  const dq8 *dq;
+  int64_t nv;
+  const float4 *tv, *tn;        // depth target maps
+  const float *depth, *sdepth;  // depth target, source gate
+  const float *mv, *mn;         // mesh target
+  const NrFrame *Fp;
+  float *vdata;
+  const double *T;
+  const int *node_vid, *nbr;
+  const float *nbr_w;
   int nn;
-  double * out;
-  unsigned grid;  // workgroups of this graph's launch
+  double *edata;
+  unsigned vwg;   // workgroups of the vertex part
+  unsigned grid;  // all workgroups of this graph's launch
 };
-// k_nr_arap for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
-__global__ __launch_bounds__(256) void kb_nr_arap(const b_nr_arap *__restrict__ t) {
-  const b_nr_arap &a = t[blockIdx.y];
+template <bool kMesh> __device__ __forceinline__ void d_nr_terms(const NrTerms &a) {
+  if (blockIdx.x < a.vwg) {
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= a.nv) return;
+    float pv[3], pn[3];
+    blend_vertex(a.verts, a.normals, a.node_of_vertex, a.vnn, a.vnn_w, a.dq, v, pv, pn);
+    if constexpr (kMesh) data_mesh_vertex(pv, pn, v, a.mv, a.mn, a.Fp, a.vdata);
+    else data_vertex(pv, pn, v, a.tv, a.tn, a.depth, a.sdepth, a.Fp, a.vdata);
+  } else {
+    const int e = (int)(blockIdx.x - a.vwg) * 256 + threadIdx.x;
+    if (e >= a.nn * kNK) return;
+    arap_edge(a.T, a.verts, a.node_vid, a.nbr, a.nbr_w, e, a.edata);
+  }
+}
+template <bool kMesh> __global__ __launch_bounds__(256) void k_nr_terms(NrTerms a) { d_nr_terms<kMesh>(a); }
+// k_nr_terms (depth target) for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
+__global__ __launch_bounds__(256) void kb_nr_terms(const NrTerms *__restrict__ t) {
+  const NrTerms &a = t[blockIdx.y];
   if (blockIdx.x >= a.grid) return;
-  d_nr_arap(a.T, a.verts, a.node_vid, a.nbr, a.nbr_w, a.nn, a.out);
+  d_nr_terms<false>(a);
 }
 
 // ARAP Jacobian rows for component c: Ji = w [row c of -[a]x, e_c],
@@ -1559,6 +1523,23 @@ static int graph_finish(dm_graph *g) {
   return DM_OK;
 }
 
+// The linearisation terms' arguments of graph g (vertex part, then edges).
+static NrTerms terms_of(const dm_graph *g) {
+  NrTerms a{};
+  a.verts = g->verts; a.normals = g->normals;
+  a.node_of_vertex = g->node_of_vertex; a.vnn = g->vnn; a.vnn_w = g->vnn_w; a.dq = g->dq;
+  a.nv = g->nv;
+  a.tv = g->tv; a.tn = g->tn; a.depth = g->depth; a.sdepth = g->sdepth;
+  a.mv = g->mesh_v; a.mn = g->mesh_n;
+  a.Fp = g->d_frame;
+  a.vdata = g->vdata;
+  a.T = g->T; a.node_vid = g->node_vid; a.nbr = g->nbr; a.nbr_w = g->nbr_w; a.nn = g->nn;
+  a.edata = g->edata;
+  a.vwg = grid_of(g->nv);
+  a.grid = a.vwg + grid_of((long long)g->nn * kNK);
+  return a;
+}
+
 // The kernels of one solve call: target maps, then gn Gauss-Newton
 // iterations of (blend, data, ARAP, assembly, PCG, update).  Everything
 // that changes between calls (pose, gates, depth) is read from device
@@ -1571,21 +1552,14 @@ static void nr_enqueue(dm_graph *g, int rows, int cols, int gn, int pcg_its, int
   const PcgState S = g->pcg;
   const unsigned pg = (unsigned)S.nwg;
   const NrFrame *F = g->d_frame;
+  const NrTerms terms = terms_of(g);
   if (mode == 0) {
     hipLaunchKernelGGL(k_nr_target_vert, dim3(grid_of(P)), dim3(256), 0, st, g->depth, F, g->tv);
     hipLaunchKernelGGL(k_nr_target_norm, dim3(grid_of(P)), dim3(256), 0, st, g->tv, F, g->tn);
   }
   for (int it = 0; it < gn; ++it) {
-    hipLaunchKernelGGL(k_nr_blend, dim3(grid_of(g->nv)), dim3(256), 0, st, g->verts, g->normals, g->nv,
-                       g->node_of_vertex, g->vnn, g->vnn_w, g->dq, g->pv, g->pn);
-    if (mode == 0)
-      hipLaunchKernelGGL(k_nr_data, dim3(grid_of(g->nv)), dim3(256), 0, st, g->pv, g->pn, g->nv, g->tv, g->tn,
-                         g->depth, g->sdepth, F, g->vdata);
-    else
-      hipLaunchKernelGGL(k_nr_data_mesh, dim3(grid_of(g->nv)), dim3(256), 0, st, g->pv, g->pn, g->nv, g->mesh_v,
-                         g->mesh_n, F, g->vdata);
-    hipLaunchKernelGGL(k_nr_arap, dim3(grid_of(ne)), dim3(256), 0, st, g->T, g->verts, g->node_vid, g->nbr,
-                       g->nbr_w, nn, g->edata);
+    if (mode == 0) hipLaunchKernelGGL(k_nr_terms<false>, dim3(terms.grid), dim3(256), 0, st, terms);
+    else hipLaunchKernelGGL(k_nr_terms<true>, dim3(terms.grid), dim3(256), 0, st, terms);
     hipLaunchKernelGGL(k_nr_assemble, dim3(grid_of(g->nb, 256 / kAsmLanes)), dim3(256), 0, st, g->blk_a, g->blk_b,
                        g->cstart, g->codes, g->cw, g->nb, g->vdata, g->nbr, g->edata, F, g->H, g->rhs, g->Hd);
     hipLaunchKernelGGL(k_nr_invdiag, dim3(grid_of(nn)), dim3(256), 0, st, g->Hd, nn, g->Minv, g->rhs, S.x, S.r,
@@ -1759,9 +1733,7 @@ static int batch_capture(dm_nonrigid_batch *b, int rows, int cols, int gn, int p
   const size_t P = (size_t)rows * cols;
   std::vector<b_nr_target_vert> tvt(K);
   std::vector<b_nr_target_norm> tnm(K);
-  std::vector<b_nr_blend> bl(K);
-  std::vector<b_nr_data> da(K);
-  std::vector<b_nr_arap> ar(K);
+  std::vector<NrTerms> trm(K);
   std::vector<b_nr_assemble> as(K);
   std::vector<b_nr_invdiag> inv(K);
   std::vector<b_nr_stats> sta(K);
@@ -1776,9 +1748,7 @@ static int batch_capture(dm_nonrigid_batch *b, int rows, int cols, int gn, int p
     const NrFrame *F = g->d_frame;
     tvt[i] = {g->depth, F, g->tv, grid_of(P)};
     tnm[i] = {g->tv, F, g->tn, grid_of(P)};
-    bl[i] = {g->verts, g->normals, g->nv, g->node_of_vertex, g->vnn, g->vnn_w, g->dq, g->pv, g->pn, grid_of(g->nv)};
-    da[i] = {g->pv, g->pn, g->nv, g->tv, g->tn, g->depth, g->sdepth, F, g->vdata, grid_of(g->nv)};
-    ar[i] = {g->T, g->verts, g->node_vid, g->nbr, g->nbr_w, nn, g->edata, grid_of(ne)};
+    trm[i] = terms_of(g);
     as[i] = {g->blk_a, g->blk_b, g->cstart, g->codes, g->cw, g->nb, g->vdata, g->nbr, g->edata, F, g->H, g->rhs,
              g->Hd, grid_of(g->nb, 256 / kAsmLanes)};
     inv[i] = {g->Hd, nn, g->Minv, g->rhs, S.x, S.r, S.u, grid_of(nn)};
@@ -1789,7 +1759,7 @@ static int batch_capture(dm_nonrigid_batch *b, int rows, int cols, int gn, int p
     nup[i] = {S.x, g->T, nn, g->dq, grid_of(nn)};
   }
   BatchTables T;
-  const size_t o_tvt = T.add(tvt), o_tnm = T.add(tnm), o_bl = T.add(bl), o_da = T.add(da), o_ar = T.add(ar),
+  const size_t o_tvt = T.add(tvt), o_tnm = T.add(tnm), o_trm = T.add(trm),
                o_as = T.add(as), o_inv = T.add(inv), o_sta = T.add(sta), o_ssum = T.add(ssum), o_pst = T.add(pst),
                o_pit = T.add(pit), o_nup = T.add(nup);
   if (T.host.size() > b->table_cap) {
@@ -1810,9 +1780,7 @@ static int batch_capture(dm_nonrigid_batch *b, int rows, int cols, int gn, int p
   hipLaunchKernelGGL(kb_nr_target_norm, dim3(max_grid(tnm), Ky), dim3(256), 0, st,
                      (const b_nr_target_norm *)(D + o_tnm));
   for (int it = 0; it < gn; ++it) {
-    hipLaunchKernelGGL(kb_nr_blend, dim3(max_grid(bl), Ky), dim3(256), 0, st, (const b_nr_blend *)(D + o_bl));
-    hipLaunchKernelGGL(kb_nr_data, dim3(max_grid(da), Ky), dim3(256), 0, st, (const b_nr_data *)(D + o_da));
-    hipLaunchKernelGGL(kb_nr_arap, dim3(max_grid(ar), Ky), dim3(256), 0, st, (const b_nr_arap *)(D + o_ar));
+    hipLaunchKernelGGL(kb_nr_terms, dim3(max_grid(trm), Ky), dim3(256), 0, st, (const NrTerms *)(D + o_trm));
     hipLaunchKernelGGL(kb_nr_assemble, dim3(max_grid(as), Ky), dim3(256), 0, st,
                        (const b_nr_assemble *)(D + o_as));
     hipLaunchKernelGGL(kb_nr_invdiag, dim3(max_grid(inv), Ky), dim3(256), 0, st, (const b_nr_invdiag *)(D + o_inv));
