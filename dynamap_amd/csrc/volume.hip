@@ -1146,8 +1146,16 @@ __global__ __launch_bounds__(256, 6) void k_raycast(const DmView *__restrict__ v
   const int tiles_x = (v.cols + tw - 1) >> tw_log2;
   const int tile = blockIdx.x;
   if (tile >= tiles_x * ((v.rows + th - 1) / th)) return;
-  const int u = (tile % tiles_x) * tw + (threadIdx.x & (tw - 1));
-  const int r = (tile / tiles_x) * th + (threadIdx.x >> tw_log2);
+  // lanes of a wavefront: one 8x8 quadrant of a 16x16 tile (rays closer
+  // together take more alike marches than a 16x4 strip's), else tw x 64/tw
+  int lx = threadIdx.x & (tw - 1), ly = threadIdx.x >> tw_log2;
+  if (tw_log2 == 4) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    lx = (l & 7) + 8 * (w & 1);
+    ly = (l >> 3) + 8 * (w >> 1);
+  }
+  const int u = (tile % tiles_x) * tw + lx;
+  const int r = (tile / tiles_x) * th + ly;
   if (u >= v.cols || r >= v.rows) return;
   const int i = r * v.cols + u;
   const Intr s = intr_of(v);
