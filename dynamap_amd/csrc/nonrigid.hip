@@ -874,11 +874,6 @@ struct PcgState {
 // transposed for the lower triangle, times the column node's 6 values), so
 // a node's entries are one round of loads; the six row sums are then added
 // over the lanes by the fixed tree of wave_sum.
-//
-// kLowReg: the same products and sums, with each block loaded in two halves
-// of three rows, the second after the first's products (fewer registers:
-// two workgroups per CU for batched solves); results are bit-identical.
-template <bool kLowReg>
 __device__ __forceinline__ double spmv_node(const float *__restrict__ H, const int *__restrict__ row_start,
                                             const uint2 *__restrict__ row_ec,
                                             const double *__restrict__ v, int node, int nn, int lane) {
@@ -897,38 +892,19 @@ __device__ __forceinline__ double spmv_node(const float *__restrict__ H, const i
         const double2 d = V2[q];
         vc[2 * q] = d.x; vc[2 * q + 1] = d.y;
       }
-      if constexpr (!kLowReg) {
-        const float4 *B4 = reinterpret_cast<const float4 *>(B);
-        float b[36];
+      const float4 *B4 = reinterpret_cast<const float4 *>(B);
+      float b[36];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) {
-          const float4 f = B4[q];
-          b[4 * q] = f.x; b[4 * q + 1] = f.y; b[4 * q + 2] = f.z; b[4 * q + 3] = f.w;
-        }
+      for (int q = 0; q < 9; ++q) {
+        const float4 f = B4[q];
+        b[4 * q] = f.x; b[4 * q + 1] = f.y; b[4 * q + 2] = f.z; b[4 * q + 3] = f.w;
+      }
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
-          double sv = 0.0;
+      for (int r = 0; r < 6; ++r) {
+        double sv = 0.0;
 #pragma unroll
-          for (int c = 0; c < 6; ++c) sv += (double)(tr ? b[6 * c + r] : b[6 * r + c]) * vc[c];
-          acc[r] += sv;
-        }
-      } else {
-#pragma unroll
-        for (int g = 0; g < 2; ++g) {
-          float b[18];  // entries (r, c) of rows r = 3g .. 3g + 2
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) b[6 * r + c] = B[tr ? 6 * c + 3 * g + r : 6 * (3 * g + r) + c];
-#pragma unroll
-          for (int r = 0; r < 3; ++r) {
-            double sv = 0.0;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) sv += (double)b[6 * r + c] * vc[c];
-            acc[3 * g + r] += sv;
-          }
-          __builtin_amdgcn_sched_barrier(0);
-        }
+        for (int c = 0; c < 6; ++c) sv += (double)(tr ? b[6 * c + r] : b[6 * r + c]) * vc[c];
+        acc[r] += sv;
       }
     }
   }
@@ -963,7 +939,6 @@ constexpr int kPcgRows = 6 * kSpmvNodes;
 
 // Start: w = A u, m = M w, partials of r_0.u_0 and w_0.u_0 (x = 0, r = rhs and
 // u = M r are written by the preconditioner kernel).
-template <bool kLowReg>
 __device__ __forceinline__ void d_pcg_start(PcgState S, const float *__restrict__ H,
                                             const int *__restrict__ row_start, const uint2 *__restrict__ row_ec,
                                             const double *__restrict__ Minv, int nn) {
@@ -973,20 +948,16 @@ __device__ __forceinline__ void d_pcg_start(PcgState S, const float *__restrict_
   const size_t e = (size_t)kPcgRows * blockIdx.x + i;
   const bool own = i < kPcgRows && e < 6 * (size_t)nn;
   double mrow[6] = {0, 0, 0, 0, 0, 0}, rv = 0.0, uv = 0.0;
-  auto load_rows = [&] {
-    if (own) {
-      rv = S.r[e];
-      uv = S.u[e];
+  if (own) {
+    rv = S.r[e];
+    uv = S.u[e];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) mrow[c] = Minv[6 * e + c];
-    }
-  };
-  if constexpr (!kLowReg) load_rows();
+    for (int c = 0; c < 6; ++c) mrow[c] = Minv[6 * e + c];
+  }
   const int node = blockIdx.x * kSpmvNodes + __builtin_amdgcn_readfirstlane(wv);
-  const double wn = spmv_node<kLowReg>(H, row_start, row_ec, S.u, node, nn, lane);
+  const double wn = spmv_node(H, row_start, row_ec, S.u, node, nn, lane);
   if (lane < 6) nl[6 * wv + lane] = wn;
   __syncthreads();
-  if constexpr (kLowReg) load_rows();
   double w = 0.0;
   if (own) { w = nl[i]; S.w[e] = w; }
   double mn = 0.0;
@@ -1003,7 +974,7 @@ __global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_start(PcgState S, const
                                                               const int *__restrict__ row_start,
                                                               const uint2 *__restrict__ row_ec,
                                                               const double *__restrict__ Minv, int nn) {
-  d_pcg_start<false>(S, H, row_start, row_ec, Minv, nn);
+  d_pcg_start(S, H, row_start, row_ec, Minv, nn);
 }
 struct b_pcg_start {
   PcgState S;
@@ -1015,11 +986,10 @@ struct b_pcg_start {
   unsigned grid;  // workgroups of this graph's launch
 };
 // k_pcg_start for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
-__global__ __launch_bounds__(64 * kSpmvNodes) __attribute__((amdgpu_waves_per_eu(8))) void kb_pcg_start(
-    const b_pcg_start *__restrict__ t) {
+__global__ __launch_bounds__(64 * kSpmvNodes) void kb_pcg_start(const b_pcg_start *__restrict__ t) {
   const b_pcg_start &a = t[blockIdx.y];
   if (blockIdx.x >= a.grid) return;
-  d_pcg_start<true>(a.S, a.H, a.row_start, a.row_ec, a.Minv, a.nn);
+  d_pcg_start(a.S, a.H, a.row_start, a.row_ec, a.Minv, a.nn);
 }
 
 // Iteration k: gamma = r.u and delta = w.u from the partials (wavefronts 2
@@ -1031,7 +1001,6 @@ __global__ __launch_bounds__(64 * kSpmvNodes) __attribute__((amdgpu_waves_per_eu
 // x += alpha p; r -= alpha s; u -= alpha q; w -= alpha t; m = M w; partials
 // of r.u and w.u.  The partials, the rows' operands and the SpMV's loads
 // are all issued before the first wait.
-template <bool kLowReg>
 __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__ H,
                                            const int *__restrict__ row_start, const uint2 *__restrict__ row_ec,
                                            const double *__restrict__ Minv, int nn, int it, const NrFrame *Fp) {
@@ -1045,21 +1014,18 @@ __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__
   // the kernel's first scalar wait would include.
   const bool pw = wv == 2 || wv == 3;
   const double *part = S.part + (size_t)(it & 1) * 2 * S.nwg + (wv == 3 ? S.nwg : 0);
-  // (kLowReg: these loads wait until after the SpMV)
   double pv8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, scv = 0.0;
-  auto load_partials = [&] {
-    if (pw) {
+  if (pw) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) pv8[j] = part[min(64 * j + lane, S.nwg - 1)];  // in bounds; lanes past nwg skip it below
-      const int km = it > 0 ? it - 1 : 0;
-      const double *q = lane == 0 ? &Fp->pcg_tol
-                        : lane == 1 ? S.rz
-                        : lane == 2 ? S.rz + km
-                        : lane == 3 ? S.al + km
-                                    : reinterpret_cast<const double *>(S.flag);  // flag[0] in the low word
-      if (wv == 2 && lane < 5) scv = *q;
-    }
-  };
+    for (int j = 0; j < 8; ++j) pv8[j] = part[min(64 * j + lane, S.nwg - 1)];  // in bounds; lanes past nwg skip it below
+    const int km = it > 0 ? it - 1 : 0;
+    const double *q = lane == 0 ? &Fp->pcg_tol
+                      : lane == 1 ? S.rz
+                      : lane == 2 ? S.rz + km
+                      : lane == 3 ? S.al + km
+                                  : reinterpret_cast<const double *>(S.flag);  // flag[0] in the low word
+    if (wv == 2 && lane < 5) scv = *q;
+  }
   const double *m_in = (it & 1) ? S.m1 : S.m0;
   double *m_out = (it & 1) ? S.m0 : S.m1;
   const int i = threadIdx.x;
@@ -1067,20 +1033,16 @@ __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__
   const bool own = i < kPcgRows && e < 6 * (size_t)nn;
   double mrow[6] = {0, 0, 0, 0, 0, 0}, tv = 0.0, qv = 0.0, sv = 0.0, pv = 0.0, xv = 0.0, rv = 0.0, uv = 0.0,
          wo = 0.0, mv = 0.0;
-  auto load_rows = [&] {
-    if (own) {
-      if (it > 0) { tv = S.t[e]; qv = S.q[e]; sv = S.s[e]; pv = S.p[e]; }
-      xv = S.x[e]; rv = S.r[e]; uv = S.u[e]; wo = S.w[e]; mv = m_in[e];
+  if (own) {
+    if (it > 0) { tv = S.t[e]; qv = S.q[e]; sv = S.s[e]; pv = S.p[e]; }
+    xv = S.x[e]; rv = S.r[e]; uv = S.u[e]; wo = S.w[e]; mv = m_in[e];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) mrow[c] = Minv[6 * e + c];
-    }
-  };
-  if constexpr (!kLowReg) { load_partials(); load_rows(); }
+    for (int c = 0; c < 6; ++c) mrow[c] = Minv[6 * e + c];
+  }
   const int node = blockIdx.x * kSpmvNodes + __builtin_amdgcn_readfirstlane(wv);
   // (computed also after convergence, so that the flag's load is not on the SpMV's path)
-  const double n = spmv_node<kLowReg>(H, row_start, row_ec, m_in, node, nn, lane);
+  const double n = spmv_node(H, row_start, row_ec, m_in, node, nn, lane);
   if (lane < 6) nl[6 * wv + lane] = n;
-  if constexpr (kLowReg) load_partials();
   if (pw) {
     double pacc = 0.0;
 #pragma unroll
@@ -1108,7 +1070,6 @@ __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__
     else S.flag[1] = it + 1;
   }
   if (stop) return;
-  if constexpr (kLowReg) load_rows();
   double wn = 0.0;
   if (own) {
     const double nv = nl[i];
@@ -1139,7 +1100,7 @@ __global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_iter(PcgState S, const 
                                                              const uint2 *__restrict__ row_ec,
                                                              const double *__restrict__ Minv, int nn, int it,
                                                              const NrFrame *Fp) {
-  d_pcg_iter<false>(S, H, row_start, row_ec, Minv, nn, it, Fp);
+  d_pcg_iter(S, H, row_start, row_ec, Minv, nn, it, Fp);
 }
 struct b_pcg_iter {
   PcgState S;
@@ -1152,11 +1113,10 @@ struct b_pcg_iter {
   unsigned grid;  // workgroups of this graph's launch
 };
 // k_pcg_iter for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
-__global__ __launch_bounds__(64 * kSpmvNodes) __attribute__((amdgpu_waves_per_eu(8))) void kb_pcg_iter(
-    const b_pcg_iter *__restrict__ t, int it) {
+__global__ __launch_bounds__(64 * kSpmvNodes) void kb_pcg_iter(const b_pcg_iter *__restrict__ t, int it) {
   const b_pcg_iter &a = t[blockIdx.y];
   if (blockIdx.x >= a.grid) return;
-  d_pcg_iter<true>(a.S, a.H, a.row_start, a.row_ec, a.Minv, a.nn, it, a.Fp);
+  d_pcg_iter(a.S, a.H, a.row_start, a.row_ec, a.Minv, a.nn, it, a.Fp);
 }
 
 // T_i <- exp(x_i) T_i, then the node's dual quaternion
