@@ -661,6 +661,8 @@ __device__ __forceinline__ void arap_rows(const double *ed, int c, double Ji[6],
 // 16 lanes give each lane several independent gathers in flight and a
 // 4-level tree (a wavefront per block took about 153 us per assembly at
 // 4048 nodes, these about 65 us).
+// Launched at 3 waves per SIMD (168 VGPRs, a few spilled) rather than the
+// compiler's 2 (176 VGPRs): more blocks in flight.
 constexpr int kAsmLanes = 16;
 __device__ __forceinline__ void d_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
                                               const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
@@ -761,7 +763,7 @@ __device__ __forceinline__ void d_nr_assemble(const int *__restrict__ blk_a, con
 #pragma unroll
   for (int q = 0; q < 36; ++q) D[q] = h[q];
 }
-__global__ __launch_bounds__(256) void k_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
                                                     const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
                                                     const float2 *__restrict__ cw, int nb,
                                                     const float *__restrict__ vdata, const int *__restrict__ nbr,
@@ -787,7 +789,7 @@ struct b_nr_assemble {
   unsigned grid;  // workgroups of this graph's launch
 };
 // k_nr_assemble for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
-__global__ __launch_bounds__(256) void kb_nr_assemble(const b_nr_assemble *__restrict__ t) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kb_nr_assemble(const b_nr_assemble *__restrict__ t) {
   const b_nr_assemble &a = t[blockIdx.y];
   if (blockIdx.x >= a.grid) return;
   d_nr_assemble(a.blk_a, a.blk_b, a.cstart, a.codes, a.cw, a.nb, a.vdata, a.nbr, a.edata, a.Fp, a.H, a.rhs, a.Hd);
