@@ -862,7 +862,7 @@ struct PcgState {
   double *x, *r, *u, *w;  // 6 N each: iterate, residual, u = M r, w = A u
   double *p, *s, *q, *t;  // direction p, s = A p, q = M s, t = A q
   double *m0, *m1;        // m = M w: iteration k reads m_(k & 1), writes m_((k + 1) & 1)
-  double *part;           // per-workgroup partials, two halves of (r.u [nwg], w.u [nwg]): the
+  double *part;           // partials, two per workgroup: two halves of (r.u [2 nwg], w.u [2 nwg]): the
                           // start kernel writes half 0, iteration k reads half k & 1 and writes
                           // half (k + 1) & 1, so no workgroup overwrites a partial another may still read
   double *rz;             // rz[k] = r_k . u_k (k = 0 .. max)
@@ -919,36 +919,42 @@ __device__ __forceinline__ double spmv_node(const float *__restrict__ H, const i
   return node < nn ? out : 0.0;
 }
 
-// The workgroup's partials of r.u and w.u into half `half` of the partial
-// array: the values of threads 0 .. 6 kSpmvNodes - 1 (wavefronts 0 and 1; the
-// others pass 0), the fixed tree of wave_sum per wavefront, then wavefront 0's sum plus
-// wavefront 1's.
-__device__ __forceinline__ void pcg_partials(const PcgState &S, double ru, double wu, int half, double *lds) {
+// The partials of r.u and w.u into half `half` of the partial array, one
+// per update wavefront (0 and 1) of each workgroup: the wavefront's lane
+// values (0 in lanes without a row) by the fixed tree of wave_sum.  No
+// barrier: each wavefront writes its own entries.
+__device__ __forceinline__ void pcg_partials(const PcgState &S, double ru, double wu, int half) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (wv < 2) {
     ru = wave_sum(ru);
     wu = wave_sum(wu);
-    if (lane == 0) { lds[2 * wv] = ru; lds[2 * wv + 1] = wu; }
+    const size_t np = 2 * (size_t)S.nwg, k = 2 * (size_t)blockIdx.x + wv;
+    if (lane == 0) S.part[(size_t)half * 2 * np + k] = ru;
+    if (lane == 1) S.part[(size_t)half * 2 * np + np + k] = wu;
   }
-  __syncthreads();
-  if (threadIdx.x < 2)
-    S.part[(size_t)half * 2 * S.nwg + (size_t)threadIdx.x * S.nwg + blockIdx.x] = lds[threadIdx.x] + lds[2 + threadIdx.x];
 }
 
-// Rows of the workgroup: thread i < 6 kSpmvNodes takes row 6 kSpmvNodes
-// blockIdx.x + i (node blockIdx.x kSpmvNodes + i / 6).
+// Rows of the workgroup (6 kSpmvNodes from row 6 kSpmvNodes blockIdx.x):
+// wavefront 0 takes rows 0 .. 59 (nodes 0 .. 9) in lanes 0 .. 59, wavefront
+// 1 rows 60 .. 95 (nodes 10 .. 15) in lanes 0 .. 35, so that each node's six
+// rows are in one wavefront; -1 for a lane without a row.
 constexpr int kPcgRows = 6 * kSpmvNodes;
+static_assert(kPcgRows == 96, "two update wavefronts of 60 and 36 rows");
+__device__ __forceinline__ int pcg_row() {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  return wv == 0 ? (lane < 60 ? lane : -1) : wv == 1 ? (lane < 36 ? 60 + lane : -1) : -1;
+}
 
 // Start: w = A u, m = M w, partials of r_0.u_0 and w_0.u_0 (x = 0, r = rhs and
 // u = M r are written by the preconditioner kernel).
 __device__ __forceinline__ void d_pcg_start(PcgState S, const float *__restrict__ H,
                                             const int *__restrict__ row_start, const uint2 *__restrict__ row_ec,
                                             const double *__restrict__ Minv, int nn) {
-  __shared__ double nl[kPcgRows], lds[4];
+  __shared__ double nl[kPcgRows];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int i = threadIdx.x;
+  const int i = pcg_row();
   const size_t e = (size_t)kPcgRows * blockIdx.x + i;
-  const bool own = i < kPcgRows && e < 6 * (size_t)nn;
+  const bool own = i >= 0 && e < 6 * (size_t)nn;
   double mrow[6] = {0, 0, 0, 0, 0, 0}, rv = 0.0, uv = 0.0;
   if (own) {
     rv = S.r[e];
@@ -969,7 +975,7 @@ __device__ __forceinline__ void d_pcg_start(PcgState S, const float *__restrict_
     for (int c = 0; c < 6; ++c) mn += mrow[c] * nl[r0 + c];
     S.m0[e] = mn;
   }
-  pcg_partials(S, own ? rv * uv : 0.0, own ? w * uv : 0.0, 0, lds);
+  pcg_partials(S, own ? rv * uv : 0.0, own ? w * uv : 0.0, 0);
   if (blockIdx.x == 0 && threadIdx.x == 0) { S.flag[0] = 0; S.flag[1] = 0; }
 }
 __global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_start(PcgState S, const float *__restrict__ H,
@@ -1006,7 +1012,7 @@ __global__ __launch_bounds__(64 * kSpmvNodes) void kb_pcg_start(const b_pcg_star
 __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__ H,
                                            const int *__restrict__ row_start, const uint2 *__restrict__ row_ec,
                                            const double *__restrict__ Minv, int nn, int it, const NrFrame *Fp) {
-  __shared__ double nl[kPcgRows], wl[kPcgRows], lds[4], l2[2], sc[5];
+  __shared__ double nl[kPcgRows], wl[kPcgRows], l2[2], sc[5];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   // wavefronts 2 and 3: lane l adds partials l, l + 64, ... in order; the
   // first 8 are loaded here and added after the SpMV's loads are issued.
@@ -1015,11 +1021,12 @@ __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__
   // (per-lane addresses) that overlap the SpMV rather than scalar loads that
   // the kernel's first scalar wait would include.
   const bool pw = wv == 2 || wv == 3;
-  const double *part = S.part + (size_t)(it & 1) * 2 * S.nwg + (wv == 3 ? S.nwg : 0);
+  const int np = 2 * S.nwg;  // partials per quantity (two per workgroup)
+  const double *part = S.part + (size_t)(it & 1) * 2 * np + (wv == 3 ? np : 0);
   double pv8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, scv = 0.0;
   if (pw) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) pv8[j] = part[min(64 * j + lane, S.nwg - 1)];  // in bounds; lanes past nwg skip it below
+    for (int j = 0; j < 8; ++j) pv8[j] = part[min(64 * j + lane, np - 1)];  // in bounds; lanes past np skip it below
     const int km = it > 0 ? it - 1 : 0;
     const double *q = lane == 0 ? &Fp->pcg_tol
                       : lane == 1 ? S.rz
@@ -1030,9 +1037,9 @@ __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__
   }
   const double *m_in = (it & 1) ? S.m1 : S.m0;
   double *m_out = (it & 1) ? S.m0 : S.m1;
-  const int i = threadIdx.x;
+  const int i = pcg_row();
   const size_t e = (size_t)kPcgRows * blockIdx.x + i;
-  const bool own = i < kPcgRows && e < 6 * (size_t)nn;
+  const bool own = i >= 0 && e < 6 * (size_t)nn;
   double mrow[6] = {0, 0, 0, 0, 0, 0}, tv = 0.0, qv = 0.0, sv = 0.0, pv = 0.0, xv = 0.0, rv = 0.0, uv = 0.0,
          wo = 0.0, mv = 0.0;
   if (own) {
@@ -1049,8 +1056,8 @@ __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__
     double pacc = 0.0;
 #pragma unroll
     for (int j = 0; j < 8; ++j)
-      if (64 * j + lane < S.nwg) pacc += pv8[j];
-    for (int k = 512 + lane; k < S.nwg; k += 64) pacc += part[k];
+      if (64 * j + lane < np) pacc += pv8[j];
+    for (int k = 512 + lane; k < np; k += 64) pacc += part[k];
     pacc = wave_sum(pacc);
     if (lane == 0) l2[wv - 2] = pacc;
     if (wv == 2 && lane < 5) sc[lane] = scv;
@@ -1087,7 +1094,8 @@ __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__
     S.x[e] = xv; S.r[e] = rv; S.u[e] = uv; S.w[e] = wn;
     wl[i] = wn;
   }
-  __syncthreads();
+  // m = M w: the node's six w are in this wavefront (written to LDS just
+  // above by the same wavefront, whose LDS operations complete in order)
   if (own) {
     const int r0 = i - (i % 6);
     double mn = 0.0;
@@ -1095,7 +1103,7 @@ __device__ __forceinline__ void d_pcg_iter(PcgState S, const float *__restrict__
     for (int c = 0; c < 6; ++c) mn += mrow[c] * wl[r0 + c];
     m_out[e] = mn;
   }
-  pcg_partials(S, own ? rv * uv : 0.0, own ? wn * uv : 0.0, (it + 1) & 1, lds);
+  pcg_partials(S, own ? rv * uv : 0.0, own ? wn * uv : 0.0, (it + 1) & 1);
 }
 __global__ __launch_bounds__(64 * kSpmvNodes) void k_pcg_iter(PcgState S, const float *__restrict__ H,
                                                              const int *__restrict__ row_start,
@@ -1477,7 +1485,7 @@ static int graph_finish(dm_graph *g) {
   PcgState &S = g->pcg;
   S.nwg = (int)grid_of(nn, kSpmvNodes);
   for (double **v : {&S.x, &S.r, &S.u, &S.w, &S.p, &S.s, &S.q, &S.t, &S.m0, &S.m1}) DM_TRY(g->dalloc(v, nn * 6));
-  DM_TRY(g->dalloc(&S.part, 4 * (size_t)S.nwg));
+  DM_TRY(g->dalloc(&S.part, 8 * (size_t)S.nwg));
   DM_TRY(g->dalloc(&S.flag, 2));
   DM_TRY(g->dalloc(&g->d_frame, 1));
   DM_HIP(hipHostMalloc((void **)&g->h_frame, sizeof(NrFrame), hipHostMallocDefault));
