@@ -423,7 +423,7 @@ struct NrFrame {
 
 // target vertex (camera) per pixel, components rounded to float (w = 1 valid)
 __device__ __forceinline__ void d_nr_target_vert(const float *__restrict__ depth, const NrFrame *Fp,
-                                                       float4 *__restrict__ tv) {
+                                                 float4 *__restrict__ tv) {
   const NrFrame &F = *Fp;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= F.rows * F.cols) return;
@@ -436,9 +436,9 @@ __device__ __forceinline__ void d_nr_target_vert(const float *__restrict__ depth
 __global__ __launch_bounds__(256) void k_nr_target_vert(const float *__restrict__ depth, const NrFrame *Fp,
                                                        float4 *__restrict__ tv) { d_nr_target_vert(depth, Fp, tv); }
 struct b_nr_target_vert {
-  const float * depth;
+  const float *depth;
   const NrFrame *Fp;
-  float4 * tv;
+  float4 *tv;
   unsigned grid;  // workgroups of this graph's launch
 };
 // k_nr_target_vert for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(256) void kb_nr_target_vert(const b_nr_target_vert 
 // target normal (camera): cross(V[i+2]-V[i-2], V[i+W]-V[i-W]) in FP64 with
 // out-of-range neighbours replaced by the pixel itself; w = 1 valid
 __device__ __forceinline__ void d_nr_target_norm(const float4 *__restrict__ tv, const NrFrame *Fp,
-                                                       float4 *__restrict__ tn) {
+                                                 float4 *__restrict__ tn) {
   const NrFrame &F = *Fp;
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int size = F.rows * F.cols;
@@ -476,9 +476,9 @@ __device__ __forceinline__ void d_nr_target_norm(const float4 *__restrict__ tv, 
 __global__ __launch_bounds__(256) void k_nr_target_norm(const float4 *__restrict__ tv, const NrFrame *Fp,
                                                        float4 *__restrict__ tn) { d_nr_target_norm(tv, Fp, tn); }
 struct b_nr_target_norm {
-  const float4 * tv;
+  const float4 *tv;
   const NrFrame *Fp;
-  float4 * tn;
+  float4 *tn;
   unsigned grid;  // workgroups of this graph's launch
 };
 // k_nr_target_norm for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
@@ -763,13 +763,13 @@ __device__ __forceinline__ void d_nr_assemble(const int *__restrict__ blk_a, con
 #pragma unroll
   for (int q = 0; q < 36; ++q) D[q] = h[q];
 }
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
-                                                    const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
-                                                    const float2 *__restrict__ cw, int nb,
-                                                    const float *__restrict__ vdata, const int *__restrict__ nbr,
-                                                    const double *__restrict__ edata, const NrFrame *Fp,
-                                                    float *__restrict__ H, double *__restrict__ rhs,
-                                                    double *__restrict__ Hd) {
+#define DM_ASM_BOUNDS __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ DM_ASM_BOUNDS void k_nr_assemble(const int *__restrict__ blk_a, const int *__restrict__ blk_b,
+                                            const int *__restrict__ cstart, const uint32_t *__restrict__ codes,
+                                            const float2 *__restrict__ cw, int nb, const float *__restrict__ vdata,
+                                            const int *__restrict__ nbr, const double *__restrict__ edata,
+                                            const NrFrame *Fp, float *__restrict__ H, double *__restrict__ rhs,
+                                            double *__restrict__ Hd) {
   d_nr_assemble(blk_a, blk_b, cstart, codes, cw, nb, vdata, nbr, edata, Fp, H, rhs, Hd);
 }
 struct b_nr_assemble {
@@ -789,7 +789,7 @@ struct b_nr_assemble {
   unsigned grid;  // workgroups of this graph's launch
 };
 // k_nr_assemble for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kb_nr_assemble(const b_nr_assemble *__restrict__ t) {
+__global__ DM_ASM_BOUNDS void kb_nr_assemble(const b_nr_assemble *__restrict__ t) {
   const b_nr_assemble &a = t[blockIdx.y];
   if (blockIdx.x >= a.grid) return;
   d_nr_assemble(a.blk_a, a.blk_b, a.cstart, a.codes, a.cw, a.nb, a.vdata, a.nbr, a.edata, a.Fp, a.H, a.rhs, a.Hd);
@@ -1131,7 +1131,7 @@ __global__ __launch_bounds__(64 * kSpmvNodes) void kb_pcg_iter(const b_pcg_iter 
 
 // T_i <- exp(x_i) T_i, then the node's dual quaternion
 __device__ __forceinline__ void d_nr_update(const double *__restrict__ x, double *__restrict__ T, int nn,
-                                                  dq8 *__restrict__ dq) {
+                                            dq8 *__restrict__ dq) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nn) return;
   double E[16], Tn[16];
@@ -1143,10 +1143,10 @@ __device__ __forceinline__ void d_nr_update(const double *__restrict__ x, double
 __global__ __launch_bounds__(256) void k_nr_update(const double *__restrict__ x, double *__restrict__ T, int nn,
                                                   dq8 *__restrict__ dq) { d_nr_update(x, T, nn, dq); }
 struct b_nr_update {
-  const double * x;
-  double * T;
+  const double *x;
+  double *T;
   int nn;
-  dq8 * dq;
+  dq8 *dq;
   unsigned grid;  // workgroups of this graph's launch
 };
 // k_nr_update for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
@@ -1161,8 +1161,8 @@ __global__ __launch_bounds__(256) void kb_nr_update(const b_nr_update *__restric
 // and edges); k_nr_stats_sum adds them in order.
 constexpr int kStatsWg = 128;
 __device__ __forceinline__ void d_nr_stats(const float *__restrict__ vdata, int64_t nv,
-                                                 const double *__restrict__ edata, int ne, const NrFrame *Fp,
-                                                 double *__restrict__ part) {
+                                           const double *__restrict__ edata, int ne, const NrFrame *Fp,
+                                           double *__restrict__ part) {
   const double lam = Fp->lambda;
   __shared__ double lds[3][256 / 64];
   double c = 0.0, ed = 0.0, er = 0.0;
@@ -1191,12 +1191,12 @@ __global__ __launch_bounds__(256) void k_nr_stats(const float *__restrict__ vdat
                                                  const double *__restrict__ edata, int ne, const NrFrame *Fp,
                                                  double *__restrict__ part) { d_nr_stats(vdata, nv, edata, ne, Fp, part); }
 struct b_nr_stats {
-  const float * vdata;
+  const float *vdata;
   int64_t nv;
-  const double * edata;
+  const double *edata;
   int ne;
   const NrFrame *Fp;
-  double * part;
+  double *part;
   unsigned grid;  // workgroups of this graph's launch
 };
 // k_nr_stats for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
@@ -1214,8 +1214,8 @@ __device__ __forceinline__ void d_nr_stats_sum(const double *__restrict__ part, 
 }
 __global__ void k_nr_stats_sum(const double *__restrict__ part, double *__restrict__ out) { d_nr_stats_sum(part, out); }
 struct b_nr_stats_sum {
-  const double * part;
-  double * out;
+  const double *part;
+  double *out;
   unsigned grid;  // workgroups of this graph's launch
 };
 // k_nr_stats_sum for a batch of graphs: graph blockIdx.y, its arguments in t[blockIdx.y]
