@@ -669,9 +669,63 @@ class nonRigidICP {
     check(dm_nonrigid_solve(graph.handle(), target.depth, &target.sensor, pose, &p, st.data()));
     return st;
   }
+  // no reference counterpart: one solve per graph (distinct graphs), graph i
+  // against depths[i] (rows * cols metres) seen from poses[i], all enqueued
+  // before any is waited for; each result equals the graph's own solve
+  std::vector<std::array<double, 4>> solveMany(const std::vector<geometry::defGraph *> &graphs,
+                                               const std::vector<const float *> &depths, const rgbdSensor &sensor,
+                                               const std::vector<Matrix4d> &poses) {
+    const size_t n = graphs.size();
+    if (depths.size() != n || poses.size() != n)
+      throw Error(DM_E_INVALID, "dynamap: one depth frame and one pose per graph");
+    std::vector<dm_graph *> h(n);
+    std::vector<double> P(16 * n);
+    for (size_t i = 0; i < n; ++i) {
+      h[i] = graphs[i]->handle();
+      std::copy(poses[i].begin(), poses[i].end(), P.begin() + 16 * i);
+    }
+    std::vector<std::array<double, 4>> st(n);
+    dm_nonrigid_props p = prop_.c();
+    check(dm_nonrigid_solve_many(h.data(), (int32_t)n, depths.data(), &sensor, P.data(), &p, st[0].data()));
+    return st;
+  }
+  const NonRigidProperties &properties() const { return prop_; }
 
  private:
   NonRigidProperties prop_;
+};
+
+// no reference counterpart: a fixed set of distinct graphs whose solves share
+// launches (graph = blockIdx.y); destroy it before its graphs
+class nonRigidBatch {
+ public:
+  nonRigidBatch(const std::vector<geometry::defGraph *> &graphs, const nonRigidICP &solver)
+      : prop_(solver.properties()), n_(graphs.size()) {
+    std::vector<dm_graph *> h(n_);
+    for (size_t i = 0; i < n_; ++i) h[i] = graphs[i]->handle();
+    check(dm_nonrigid_batch_create(h.data(), (int32_t)n_, &b_));
+  }
+  nonRigidBatch(const nonRigidBatch &) = delete;
+  nonRigidBatch &operator=(const nonRigidBatch &) = delete;
+  ~nonRigidBatch() {
+    if (b_) dm_nonrigid_batch_destroy(b_);
+  }
+  std::vector<std::array<double, 4>> solve(const std::vector<const float *> &depths, const rgbdSensor &sensor,
+                                           const std::vector<Matrix4d> &poses) {
+    if (depths.size() != n_ || poses.size() != n_)
+      throw Error(DM_E_INVALID, "dynamap: one depth frame and one pose per graph");
+    std::vector<double> P(16 * n_);
+    for (size_t i = 0; i < n_; ++i) std::copy(poses[i].begin(), poses[i].end(), P.begin() + 16 * i);
+    std::vector<std::array<double, 4>> st(n_);
+    dm_nonrigid_props p = prop_.c();
+    check(dm_nonrigid_batch_solve(b_, depths.data(), &sensor, P.data(), &p, st[0].data()));
+    return st;
+  }
+
+ private:
+  NonRigidProperties prop_;
+  size_t n_ = 0;
+  dm_nonrigid_batch *b_ = nullptr;
 };
 
 }  // namespace solver

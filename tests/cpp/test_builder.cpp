@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "dynamap/dynamap.hpp"
@@ -178,6 +179,42 @@ int main() {
     for (const float3 &v : dstMesh.vertices) worst = std::max(worst, std::fabs((double)v.z - 1.02));
     std::printf("mesh-target non-rigid: max |z - 1.02| %.2e\n", worst);
     if (st[0] != 400 || worst > 1e-4) return 21;
+    // several graphs at once (no reference counterpart): solveMany on their
+    // own streams and a batch along blockIdx.y give the blended meshes of the
+    // same solves one graph at a time, bit for bit
+    {
+      std::vector<geometry::defGraph> gs(6);
+      for (auto &g : gs) {
+        g.init(srcMesh, (int)(s.rows * s.cols), 1);
+        g.defGraphUpdateNodes();
+      }
+      std::vector<float> near((size_t)s.rows * s.cols, 1.005f), far((size_t)s.rows * s.cols, 1.015f);
+      const std::vector<const float *> depths = {near.data(), far.data()};
+      const std::vector<Matrix4d> poses = {Identity4(), Identity4()};
+      solver::nonRigidICP one;
+      one.init(gs[0], OptimizerProp, (int)(s.rows * s.cols));
+      rgbdImage a{near.data(), nullptr, s}, b{far.data(), nullptr, s};
+      std::vector<geometry::defGraph *> many = {&gs[2], &gs[3]}, bat = {&gs[4], &gs[5]};
+      solver::nonRigidBatch batch(bat, one);
+      for (int f = 0; f < 2; ++f) {
+        one.solve(gs[0], blender, a, poses[0].data());
+        one.solve(gs[1], blender, b, poses[1].data());
+        one.solveMany(many, depths, s, poses);
+        batch.solve(depths, s, poses);
+      }
+      for (int k = 0; k < 2; ++k) {
+        geometry::MeshSTD m0, m1, m2;
+        blender.blendMesh(m0, gs[k]);
+        blender.blendMesh(m1, gs[2 + k]);
+        blender.blendMesh(m2, gs[4 + k]);
+        for (size_t q = 0; q < m0.vertices.size(); ++q)
+          if (std::memcmp(&m0.vertices[q], &m1.vertices[q], sizeof(float3)) != 0 ||
+              std::memcmp(&m0.vertices[q], &m2.vertices[q], sizeof(float3)) != 0)
+            return 28;
+        if (std::fabs(m0.vertices[0].z - (k ? 1.015f : 1.005f)) > 1e-4f) return 29;
+      }
+      std::printf("solveMany and nonRigidBatch: blended meshes equal the single solves'\n");
+    }
     // updateNodesDQ from (roll, pitch, yaw, tx, ty, tz): a common yaw of 0.1
     // rad and a 1 cm x shift move every vertex rigidly
     std::vector<float> dq((size_t)graph.nodeNum * 6, 0.f);
