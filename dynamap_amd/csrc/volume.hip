@@ -166,32 +166,34 @@ __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r, unsigned
   const long long bound64 = llabs((long long)ex - bx) + llabs((long long)ey - by) + llabs((long long)ez - bz);
   const int bound = (int)min(bound64, (long long)0x7fffffff);
   // Every step moves one axis by one block, so the traversal stays within
-  // `bound` blocks of (bx, by, bz).  When that cube lies inside the window
-  // (and so inside the key range), the window index of the current block
-  // follows the steps, and only blocks whose allocated bit is clear go
-  // through visit_block (which starts with that same test).
+  // `bound` blocks of (bx, by, bz), and the L1 distance to the end block,
+  // `bound` at the start, falls by at most one per step: the end block can
+  // only be reached at step `bound`, so the loop takes exactly `bound` steps
+  // and needs no end-block test.  When that cube lies inside the window (and
+  // so inside the key range), only the window index of the current block is
+  // stepped; a block whose allocated bit is clear (rare once the scene is
+  // mapped) has its coordinates decoded from that index for visit_block
+  // (which starts with the same bit test).
   uint32_t widx;
   if (!kCount && bound < kWin / 2 && win_bit(bx - bound, by - bound, bz - bound, widx) &&
       win_bit(bx + bound, by + bound, bz + bound, widx)) {
     win_bit(bx, by, bz, widx);
-    const int ix = stx, iy = sty * kWin, iz = stz * kWin * kWin;
-    for (int it = 0; it < bound && (bx != ex || by != ey || bz != ez); ++it) {
+    const uint32_t ix = (uint32_t)stx, iy = (uint32_t)(sty * kWin), iz = (uint32_t)(stz * kWin * kWin);
+    for (int it = 0; it < bound; ++it) {
       const bool sx = mtx < mty && mtx < mtz;
       const bool sy = !(mtx < mty) && mty < mtz;
       const bool sz = !sx && !sy;
-      bx += sx ? stx : 0;
       mtx += sx ? dtx : 0.0f;
-      by += sy ? sty : 0;
       mty += sy ? dty : 0.0f;
-      bz += sz ? stz : 0;
       mtz += sz ? dtz : 0.0f;
-      widx += (uint32_t)(sx ? ix : sy ? iy : iz);
-      if (!test_bit(v.allocmap, widx)) visit_block<kCount>(v, bx, by, bz, seen);
-      ++nvisit;
+      widx += sx ? ix : sy ? iy : iz;
+      if (!test_bit(v.allocmap, widx))
+        visit_block<kCount>(v, (int)(widx & (kWin - 1)) - kWin / 2, (int)((widx / kWin) & (kWin - 1)) - kWin / 2,
+                            (int)(widx / (kWin * kWin)) - kWin / 2, seen);
     }
-    return nvisit;
+    return nvisit + bound;
   }
-  for (int it = 0; it < bound && (bx != ex || by != ey || bz != ez); ++it) {
+  for (int it = 0; it < bound; ++it) {
     // the reference's nested comparisons, as selects: x when mtx is below
     // both, else y when mty < mtz, else z
     const bool sx = mtx < mty && mtx < mtz;
@@ -204,9 +206,8 @@ __device__ __forceinline__ int alloc_ray(const DmView &v, int u, int r, unsigned
     bz += sz ? stz : 0;
     mtz += sz ? dtz : 0.0f;
     visit_block<kCount>(v, bx, by, bz, seen);
-    ++nvisit;
   }
-  return nvisit;
+  return nvisit + bound;
 }
 
 template <bool kCount>
