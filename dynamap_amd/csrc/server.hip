@@ -152,6 +152,20 @@ static void retire(dm_server *srv, int ti) {
   t.next_frame = -1;
 }
 
+// Dispatch order of a batch: every batched kernel maps blockIdx.y to the
+// list's sessions in order, and workgroups are dispatched x fastest, so the
+// sessions whose rays cost most (pixels times depth range over voxel size)
+// go first and the launch's tail is made of cheaper tiles.  Each session's
+// results do not depend on its place in the list.
+static void order_by_cost(const dm_server *srv, std::vector<int> &act) {
+  auto cost = [srv](int t) {
+    const Trace &tr = srv->traces[t];
+    return (double)tr.sensor.rows * tr.sensor.cols *
+           ((double)tr.props.max_sensor_depth - tr.props.min_sensor_depth) / tr.props.voxel_size;
+  };
+  std::stable_sort(act.begin(), act.end(), [&](int a, int b) { return cost(a) > cost(b); });
+}
+
 static Launch server_launch(const dm_server *srv) {
   Launch L;
   L.views = srv->d_table;
@@ -386,6 +400,7 @@ int dm_server_run(dm_server *srv, int32_t mode, int32_t first_frame, int32_t nfr
         if (!tr.sess || next[t] >= nframes) continue;
         if (arrival(t, next[t]) <= now) act.push_back(t);
       }
+      order_by_cost(srv, act);
       if (act.empty()) {
         // wait for the earliest arrival or for an admission opportunity
         r = drain(false);
@@ -554,6 +569,7 @@ static int stage_setup(dm_server *srv, int32_t frame, std::vector<int> &act, int
   for (int t = 0; t < (int)srv->traces.size(); ++t)
     if (srv->traces[t].sess && frame < srv->traces[t].nframes) act.push_back(t);
   if (act.empty()) { set_error("no admitted session at this frame"); return DM_E_STATE; }
+  order_by_cost(srv, act);
   for (int t : act) srv->traces[t].next_frame = -1;  // the timed launches leave no continuable state
   DM_HIP(hipStreamSynchronize(srv->stream));
   DmView *tab = srv->h_table[0];
